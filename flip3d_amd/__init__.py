@@ -1,0 +1,271 @@
+"""flip3d_amd - MI355X (gfx950) FLIP step library for the flip3d hot path.
+
+Python binding over the C ABI in include/flip3d_gpu.h (libflip3d_amd.so, built
+in-tree by flip3d_amd/build.py).  The binding does no computation of its own:
+every operation runs in the HIP library, and importing fails loudly when the
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflip3d_amd.so")
+
+AIR, FLUID, WALL = 0, 1, 2
+GRID_U0, GRID_U1, GRID_U2, GRID_US0, GRID_US1, GRID_US2, GRID_A, GRID_L, GRID_PRESS, GRID_DIV = range(10)
+PRECOND_REFERENCE, PRECOND_IC0, PRECOND_LEARNED = 0, 1, 2
+(STAGE_DENSITY, STAGE_EXT_FORCE, STAGE_P2G, STAGE_PROJECT, STAGE_EXTRAP, STAGE_G2P, STAGE_ADVECT,
+ STAGE_CORRECT) = range(8)
+SCENE_REFERENCE, SCENE_BENCHMARK = 0, 1
+BENCH_SEED = 20261203
+
+EXPORTS = [
+    "flip_gpu_last_error", "flip_gpu_default_params", "flip_gpu_create", "flip_gpu_destroy",
+    "flip_gpu_synchronize", "flip_gpu_upload_particles", "flip_gpu_download_particles",
+    "flip_gpu_num_particles", "flip_gpu_set_max_dens", "flip_gpu_get_max_dens", "flip_gpu_srand",
+    "flip_gpu_set_dt", "flip_gpu_get_grid", "flip_gpu_set_grid", "flip_gpu_step", "flip_gpu_stage",
+    "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
+    "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
+]
+
+
+class FlipError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("dt", C.c_double), ("dt_policy", C.c_int), ("cfl", C.c_double),
+        ("alpha", C.c_double), ("gravity", C.c_double), ("density", C.c_double),
+        ("subcell", C.c_int), ("tol_policy", C.c_int), ("tol", C.c_double), ("precond", C.c_int),
+        ("max_iters", C.c_int), ("correction", C.c_int), ("reposition", C.c_int),
+        ("rand_seed", C.c_uint), ("refine", C.c_int), ("refine_tol", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("error2", C.c_double), ("bnorm2", C.c_double),
+        ("error2_0", C.c_double), ("dt", C.c_double), ("n_fluid_cells", C.c_int),
+        ("n_reposition", C.c_int), ("n_jitter", C.c_int), ("n_precond_unit", C.c_int),
+        ("refine_sweeps", C.c_int), ("ms_step", C.c_double), ("ms_solve", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FlipError(f"{LIB_PATH} is missing: run flip3d_amd/build.py (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.flip_gpu_last_error.restype = C.c_char_p
+        L.flip_gpu_default_params.argtypes = [C.POINTER(Params), C.c_int]
+        L.flip_gpu_create.argtypes = [C.POINTER(Params), C.POINTER(P)]
+        L.flip_gpu_destroy.argtypes = [P]
+        L.flip_gpu_synchronize.argtypes = [P]
+        L.flip_gpu_upload_particles.argtypes = [P, C.c_int64, P, P, P, P, P, P]
+        L.flip_gpu_download_particles.argtypes = [P, P, P, P, P, P]
+        L.flip_gpu_num_particles.argtypes = [P, C.POINTER(C.c_int64)]
+        L.flip_gpu_set_max_dens.argtypes = [P, C.c_float]
+        L.flip_gpu_get_max_dens.argtypes = [P, C.POINTER(C.c_float)]
+        L.flip_gpu_srand.argtypes = [P, C.c_uint]
+        L.flip_gpu_set_dt.argtypes = [P, C.c_double]
+        L.flip_gpu_get_grid.argtypes = [P, C.c_int, P]
+        L.flip_gpu_set_grid.argtypes = [P, C.c_int, P]
+        L.flip_gpu_step.argtypes = [P, C.POINTER(Stats)]
+        L.flip_gpu_stage.argtypes = [P, C.c_int, C.POINTER(Stats)]
+        L.flip_gpu_map_p2g.argtypes = [P, P, P, P]
+        L.flip_gpu_map_g2p.argtypes = [P, P, P, P, C.c_int64, P, P]
+        L.flip_gpu_solve.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(Stats)]
+        L.flip_gpu_sorted_cells.argtypes = [P, P, P]
+        L.flip_gpu_kernel_times.argtypes = [P, C.c_char_p, C.c_int, P, C.c_int]
+        L.flip_gpu_set_profiling.argtypes = [P, C.c_int]
+        L.flip_gpu_build_scene.argtypes = [P, C.c_int, C.c_uint64, C.POINTER(C.c_int64)]
+        L.flip_gpu_selftest_math.argtypes = [C.c_int64, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc < 0:
+        raise FlipError(f"flip_gpu error {rc}: {lib().flip_gpu_last_error().decode()}")
+    return rc
+
+
+def default_params(n, **kw):
+    p = Params()
+    _check(lib().flip_gpu_default_params(C.byref(p), n))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def selftest_math(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    out = np.empty((len(x), 4), np.float32)
+    _check(lib().flip_gpu_selftest_math(len(x), _ptr(x), _ptr(y), _ptr(out)))
+    return out
+
+
+class FlipContext:
+    """One device-resident simulation (a flip_ctx)."""
+
+    def __init__(self, n, **kw):
+        self.n = n
+        self.params = default_params(n, **kw)
+        h = C.c_void_p()
+        _check(lib().flip_gpu_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().flip_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- particles --
+    def upload(self, p, u, nrm, type_, dens=None, mass=None):
+        p = np.ascontiguousarray(p, np.float32).reshape(-1)
+        u = None if u is None else np.ascontiguousarray(u, np.float32).reshape(-1)
+        nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1)
+        t = np.ascontiguousarray(type_, np.int8)
+        d = None if dens is None else np.ascontiguousarray(dens, np.float32)
+        m = None if mass is None else np.ascontiguousarray(mass, np.float32)
+        _check(lib().flip_gpu_upload_particles(self.h, len(t), _ptr(p), _ptr(u), _ptr(nrm), _ptr(t), _ptr(m),
+                                               _ptr(d)))
+
+    def num_particles(self):
+        v = C.c_int64()
+        _check(lib().flip_gpu_num_particles(self.h, C.byref(v)))
+        return v.value
+
+    def download(self):
+        n = self.num_particles()
+        p = np.empty((n, 3), np.float32)
+        u = np.empty((n, 3), np.float32)
+        nr = np.empty((n, 3), np.float32)
+        t = np.empty(n, np.int8)
+        d = np.empty(n, np.float32)
+        _check(lib().flip_gpu_download_particles(self.h, _ptr(p), _ptr(u), _ptr(nr), _ptr(t), _ptr(d)))
+        return {"p": p, "u": u, "n": nr, "type": t, "dens": d}
+
+    @property
+    def max_dens(self):
+        v = C.c_float()
+        _check(lib().flip_gpu_get_max_dens(self.h, C.byref(v)))
+        return v.value
+
+    @max_dens.setter
+    def max_dens(self, v):
+        _check(lib().flip_gpu_set_max_dens(self.h, v))
+
+    def srand(self, seed):
+        _check(lib().flip_gpu_srand(self.h, seed))
+
+    def set_dt(self, dt):
+        _check(lib().flip_gpu_set_dt(self.h, dt))
+
+    def build_scene(self, scene=SCENE_BENCHMARK, seed=BENCH_SEED):
+        v = C.c_int64()
+        _check(lib().flip_gpu_build_scene(self.h, scene, seed, C.byref(v)))
+        return v.value
+
+    # -- grids --
+    def grid(self, which):
+        n = self.n
+        if which in (GRID_U0, GRID_US0):
+            shape, dt = (n + 1, n, n), np.float32
+        elif which in (GRID_U1, GRID_US1):
+            shape, dt = (n, n + 1, n), np.float32
+        elif which in (GRID_U2, GRID_US2):
+            shape, dt = (n, n, n + 1), np.float32
+        else:
+            shape, dt = (n, n, n), (np.int8 if which == GRID_A else np.float32)
+        out = np.empty(shape, dt)
+        _check(lib().flip_gpu_get_grid(self.h, which, _ptr(out)))
+        return out
+
+    def set_grid(self, which, arr):
+        dt = np.int8 if which == GRID_A else np.float32
+        a = np.ascontiguousarray(arr, dt)
+        _check(lib().flip_gpu_set_grid(self.h, which, _ptr(a)))
+
+    # -- work --
+    def step(self):
+        st = Stats()
+        _check(lib().flip_gpu_step(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def stage(self, which):
+        st = Stats()
+        _check(lib().flip_gpu_stage(self.h, which, C.byref(st)))
+        return st.as_dict()
+
+    def map_p2g(self):
+        n = self.n
+        g = [np.empty((n + 1) * n * n, np.float32) for _ in range(3)]
+        _check(lib().flip_gpu_map_p2g(self.h, _ptr(g[0]), _ptr(g[1]), _ptr(g[2])))
+        return g[0].reshape(n + 1, n, n), g[1].reshape(n, n + 1, n), g[2].reshape(n, n, n + 1)
+
+    def map_g2p(self, g0, g1, g2, pts):
+        g = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (g0, g1, g2)]
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.empty_like(pts)
+        _check(lib().flip_gpu_map_g2p(self.h, _ptr(g[0]), _ptr(g[1]), _ptr(g[2]), len(pts), _ptr(pts), _ptr(out)))
+        return out
+
+    def solve(self, A, L, x, b, subcell=1):
+        A = np.ascontiguousarray(A, np.int8)
+        L = np.ascontiguousarray(L, np.float32)
+        x = np.array(x, np.float32)
+        b = np.array(b, np.float32)
+        st = Stats()
+        _check(lib().flip_gpu_solve(self.h, _ptr(A), _ptr(L), _ptr(x), _ptr(b), subcell, C.byref(st)))
+        return x, b, st.as_dict()
+
+    def sorted_cells(self):
+        counts = np.empty(self.n ** 3, np.int32)
+        ids = np.empty(self.num_particles(), np.int32)
+        _check(lib().flip_gpu_sorted_cells(self.h, _ptr(counts), _ptr(ids)))
+        return counts, ids
+
+    def set_profiling(self, on=True):
+        _check(lib().flip_gpu_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        buf = C.create_string_buffer(8192)
+        ms = np.zeros(512, np.float64)
+        cnt = _check(lib().flip_gpu_kernel_times(self.h, buf, 8192, _ptr(ms), 512))
+        names = buf.value.decode().split(";") if cnt else []
+        return list(zip(names, ms[:cnt].tolist()))
+
+    def synchronize(self):
+        _check(lib().flip_gpu_synchronize(self.h))

@@ -1,0 +1,837 @@
+// capi.hip - C ABI (include/flip3d_gpu.h): context, particle I/O, step
+// orchestration in the order of flip3D::simulateStep (src/flip3D.cpp:1194-1264).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "flip_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+template <class F> int guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const flip_hip_error &e) {
+        return fail(FLIP_E_HIP, e.what);
+    } catch (const flip_arg_error &e) {
+        return fail(e.code, e.what);
+    } catch (const std::bad_alloc &) {
+        return fail(FLIP_E_HIP, "host allocation failed");
+    } catch (...) {
+        return fail(FLIP_E_STATE, "unexpected exception");
+    }
+}
+
+template <class T> void dalloc(T **p, size_t count)
+{
+    if (*p) FLIP_CHECK(hipFree(*p));
+    *p = nullptr;
+    if (count == 0) count = 1;
+    FLIP_CHECK(hipMalloc((void **)p, count * sizeof(T)));
+}
+
+template <class T> void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void require(bool ok, const char *msg, int code = FLIP_E_ARG)
+{
+    if (!ok) throw flip_arg_error{code, msg};
+}
+
+void free_particles(FlipCtx *c)
+{
+    for (int b = 0; b < 2; ++b) {
+        dfree(c->pos[b]);
+        dfree(c->vel[b]);
+        dfree(c->meta[b]);
+    }
+    dfree(c->nrm_id);
+    dfree(c->inv);
+    dfree(c->spos);
+    dfree(c->tmp0);
+    dfree(c->tmp1);
+    dfree(c->pkey);
+    dfree(c->pslot);
+    dfree(c->perm);
+    dfree(c->deep);
+    dfree(c->ev_list);
+    dfree(c->ev_off);
+    dfree(c->ev_rand);
+    dfree(c->partials);
+    dfree(c->scan_blocks);
+    dfree(c->wall_pos);
+    dfree(c->wall_nrm);
+    c->cap = 0;
+}
+
+void alloc_particles(FlipCtx *c, int64_t cap)
+{
+    free_particles(c);
+    size_t n = (size_t)std::max<int64_t>(cap, 1);
+    for (int b = 0; b < 2; ++b) {
+        dalloc(&c->pos[b], n);
+        dalloc(&c->vel[b], n);
+        dalloc(&c->meta[b], n);
+    }
+    dalloc(&c->nrm_id, n);
+    dalloc(&c->inv, n);
+    dalloc(&c->spos, n);
+    dalloc(&c->tmp0, n);
+    dalloc(&c->tmp1, n);
+    dalloc(&c->pkey, n);
+    dalloc(&c->pslot, n);
+    dalloc(&c->perm, n);
+    size_t big = std::max<size_t>(n, (size_t)c->C) + 1;
+    dalloc(&c->deep, big);
+    dalloc(&c->ev_list, 2 * n + 2);
+    dalloc(&c->ev_off, big);
+    c->ev_cap = 2 * (int64_t)n + 1024;
+    dalloc(&c->ev_rand, 3 * (size_t)c->ev_cap);
+    dalloc(&c->partials, 4 * (big / 256 + 8));
+    dalloc(&c->scan_blocks, big / 2048 + 8);
+    c->cap = cap;
+}
+
+void alloc_grids(FlipCtx *c)
+{
+    for (int k = 0; k < 3; ++k) {
+        dalloc(&c->ug[k], c->F);
+        dalloc(&c->us[k], c->F);
+        FLIP_CHECK(hipMemset(c->ug[k], 0, c->F * sizeof(float)));
+        FLIP_CHECK(hipMemset(c->us[k], 0, c->F * sizeof(float)));
+    }
+    dalloc(&c->A, c->C);
+    dalloc(&c->L, c->C);
+    dalloc(&c->press, c->C);
+    dalloc(&c->div, c->C);
+    FLIP_CHECK(hipMemset(c->A, 0, c->C));
+    FLIP_CHECK(hipMemset(c->L, 0, c->C * sizeof(float)));
+    FLIP_CHECK(hipMemset(c->press, 0, c->C * sizeof(float)));
+    FLIP_CHECK(hipMemset(c->div, 0, c->C * sizeof(float)));
+    dalloc(&c->cell_count, c->C + 1);
+    dalloc(&c->cell_start, c->C + 1);
+    dalloc(&c->wall_start, c->C + 1);
+    dalloc(&c->scan_tmp, c->C + 1);
+    dalloc(&c->fl, c->C);
+    dalloc(&c->P, c->C);
+    dalloc(&c->r, c->C);
+    dalloc(&c->z, c->C);
+    dalloc(&c->q, c->C);
+    dalloc(&c->sv[0], c->C);
+    dalloc(&c->sv[1], c->C);
+    FLIP_CHECK(hipMemset(c->sv[0], 0, c->C * sizeof(float)));
+    FLIP_CHECK(hipMemset(c->sv[1], 0, c->C * sizeof(float)));
+    FLIP_CHECK(hipMemset(c->z, 0, c->C * sizeof(float)));
+    dalloc(&c->scal, 32);
+    dalloc(&c->iflags, 16);
+    dalloc(&c->tickets, 8);
+    FLIP_CHECK(hipMemset(c->tickets, 0, 8 * sizeof(unsigned)));
+    dalloc(&c->ev_count, 4);
+    FLIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(int32_t)));
+}
+
+void free_all(FlipCtx *c)
+{
+    free_particles(c);
+    for (int k = 0; k < 3; ++k) {
+        dfree(c->ug[k]);
+        dfree(c->us[k]);
+    }
+    dfree(c->A);
+    dfree(c->L);
+    dfree(c->press);
+    dfree(c->div);
+    dfree(c->cell_count);
+    dfree(c->cell_start);
+    dfree(c->wall_start);
+    dfree(c->scan_tmp);
+    dfree(c->fl);
+    dfree(c->P);
+    dfree(c->r);
+    dfree(c->z);
+    dfree(c->q);
+    dfree(c->sv[0]);
+    dfree(c->sv[1]);
+    dfree(c->scal);
+    dfree(c->iflags);
+    dfree(c->tickets);
+    dfree(c->ev_count);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    c->h_pinned = nullptr;
+    for (auto e : c->tev) (void)hipEventDestroy(e);
+    c->tev.clear();
+    hipEvent_t evs[4] = {c->ev_a, c->ev_b, c->ev_s0, c->ev_s1};
+    for (auto e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+float grid_cell_of(int gn, float p)  // sorter.cpp:35-37 on the host
+{
+    return (float)(int)fmax(0.0, fmin((double)(gn - 1), (double)((float)gn * p)));
+}
+
+void build_wall_bins(FlipCtx *c, const std::vector<float4> &pos, const std::vector<int32_t> &meta,
+                     const std::vector<float4> &nrm_id)
+{
+    int n = c->n;
+    std::vector<std::pair<int64_t, int32_t>> walls;  // (cell, id)
+    for (size_t i = 0; i < pos.size(); ++i) {
+        if ((meta[i] & 3) != FLIP_WALL) continue;
+        int ci = (int)grid_cell_of(n, pos[i].x), cj = (int)grid_cell_of(n, pos[i].y), ck = (int)grid_cell_of(n, pos[i].z);
+        walls.push_back({((int64_t)ci * n + cj) * n + ck, (int32_t)i});
+    }
+    std::sort(walls.begin(), walls.end());
+    std::vector<int32_t> start(c->C + 1, 0);
+    std::vector<float4> wp(walls.size()), wn(walls.size());
+    for (size_t t = 0; t < walls.size(); ++t) {
+        start[walls[t].first + 1]++;
+        wp[t] = pos[walls[t].second];
+        wn[t] = nrm_id[(uint32_t)meta[walls[t].second] >> 2];
+    }
+    for (long k = 0; k < c->C; ++k) start[k + 1] += start[k];
+    c->nwall = (int64_t)walls.size();
+    dalloc(&c->wall_pos, walls.size());
+    dalloc(&c->wall_nrm, walls.size());
+    FLIP_CHECK(hipMemcpy(c->wall_start, start.data(), (c->C + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!walls.empty()) {
+        FLIP_CHECK(hipMemcpy(c->wall_pos, wp.data(), wp.size() * sizeof(float4), hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpy(c->wall_nrm, wn.data(), wn.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
+}
+
+void ensure_sorted(FlipCtx *c)
+{
+    if (!c->sorted) {
+        k_launch_sort(c);
+        c->sorted = true;
+    }
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    FLIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+}  // namespace
+
+int flip_internal_fail(int code, const char *msg) { return fail(code, msg); }
+
+flip_hip_error::flip_hip_error(hipError_t e, const char *expr, const char *file, int line) : err(e)
+{
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", expr, hipGetErrorString(e), file, line);
+    what = buf;
+}
+
+void GlibcRand::seed(unsigned sd)
+{
+    if (sd == 0) sd = 1;
+    int32_t w = (int32_t)sd;
+    s[0] = w;
+    for (int i = 1; i < 31; ++i) {
+        long hi = w / 127773, lo = w % 127773;
+        long t = 16807 * lo - 2836 * hi;
+        if (t < 0) t += 2147483647;
+        w = (int32_t)t;
+        s[i] = w;
+    }
+    f = 3;
+    r = 0;
+    for (int k = 0; k < 310; ++k) (void)next();
+}
+
+int GlibcRand::next()
+{
+    uint32_t v = (uint32_t)s[f] + (uint32_t)s[r];
+    s[f] = (int32_t)v;
+    f = (f + 1) % 31;
+    r = (r + 1) % 31;
+    return (int)(v >> 1);
+}
+
+void t_begin(FlipCtx *c, const char *name)
+{
+    if (!c->profiling) return;
+    hipEvent_t e;
+    FLIP_CHECK(hipEventCreate(&e));
+    FLIP_CHECK(hipEventRecord(e, c->stream));
+    c->tnames.push_back(name);
+    c->tev.push_back(e);
+}
+
+void t_end(FlipCtx *c)
+{
+    if (!c->profiling) return;
+    hipEvent_t e;
+    FLIP_CHECK(hipEventCreate(&e));
+    FLIP_CHECK(hipEventRecord(e, c->stream));
+    c->tev.push_back(e);
+}
+
+static void timing_reset(FlipCtx *c)
+{
+    for (auto e : c->tev) (void)hipEventDestroy(e);
+    c->tev.clear();
+    c->tnames.clear();
+    c->tms.clear();
+}
+
+static void timing_collect(FlipCtx *c)
+{
+    if (!c->profiling) return;
+    FLIP_CHECK(hipStreamSynchronize(c->stream));
+    c->tms.clear();
+    for (size_t t = 0; t + 1 < c->tev.size(); t += 2) c->tms.push_back(elapsed(c->tev[t], c->tev[t + 1]));
+}
+
+static void run_stage(FlipCtx *c, int stage, flip_step_stats *st)
+{
+    switch (stage) {
+    case FLIP_STAGE_DENSITY:
+        k_launch_sort(c);
+        c->sorted = true;
+        k_launch_density(c);
+        break;
+    case FLIP_STAGE_EXT_FORCE:
+        k_launch_ext_force(c);
+        break;
+    case FLIP_STAGE_P2G:
+        ensure_sorted(c);
+        k_launch_flags(c);
+        k_launch_p2g(c);
+        break;
+    case FLIP_STAGE_PROJECT: {
+        k_launch_divergence(c);
+        if (c->profiling || st) {
+            if (!c->ev_s0) {
+                FLIP_CHECK(hipEventCreate(&c->ev_s0));
+                FLIP_CHECK(hipEventCreate(&c->ev_s1));
+            }
+            FLIP_CHECK(hipEventRecord(c->ev_s0, c->stream));
+        }
+        int rc = k_solve(c, st);
+        if (c->ev_s0 && (c->profiling || st)) {
+            FLIP_CHECK(hipEventRecord(c->ev_s1, c->stream));
+            FLIP_CHECK(hipEventSynchronize(c->ev_s1));
+            if (st) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
+        }
+        if (rc == FLIP_E_NUMERIC) throw flip_arg_error{FLIP_E_NUMERIC, "pressure solve produced NaN"};
+        k_launch_gradient(c);
+        break;
+    }
+    case FLIP_STAGE_EXTRAP:
+        k_launch_extrapolate(c);
+        break;
+    case FLIP_STAGE_G2P:
+        k_launch_g2p(c);
+        break;
+    case FLIP_STAGE_ADVECT: {
+        k_launch_advect(c);
+        c->sorted = false;
+        int m = k_stuck_and_reposition(c);  // leaves the particles sorted
+        c->sorted = true;
+        if (st) st->n_reposition = m;
+        break;
+    }
+    case FLIP_STAGE_CORRECT: {
+        ensure_sorted(c);
+        int ev = k_correct(c);
+        c->sorted = false;
+        if (st) st->n_jitter = ev;
+        break;
+    }
+    default:
+        throw flip_arg_error{FLIP_E_ARG, "unknown stage"};
+    }
+}
+
+static void choose_dt(FlipCtx *c)
+{
+    if (c->prm.dt_policy == 0) {
+        c->dt = c->prm.dt;
+        return;
+    }
+    double umax = k_max_speed(c);
+    double dt = c->prm.dt;
+    if (umax > 0.0) {
+        double cfl = c->prm.cfl / (c->n * umax);
+        if (cfl < dt) dt = cfl;
+    }
+    c->dt = dt;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char *flip_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int flip_gpu_default_params(flip_params *p, int n)
+{
+    if (!p) return fail(FLIP_E_ARG, "null params");
+    memset(p, 0, sizeof(*p));
+    p->n = n;
+    p->dt = 0.6e-2;
+    p->dt_policy = 0;
+    p->cfl = 0.6;
+    p->alpha = 0.95;
+    p->gravity = 9.8;
+    p->density = 0.5;
+    p->subcell = 1;
+    p->tol_policy = 0;
+    p->tol = 1e-5;
+    p->precond = FLIP_PRECOND_REFERENCE;
+    p->max_iters = 0;
+    p->correction = 1;
+    p->reposition = 1;
+    p->rand_seed = 1;
+    p->refine = 0;
+    p->refine_tol = 1e-7;
+    return FLIP_OK;
+}
+
+int flip_gpu_create(const flip_params *p, flip_ctx **out)
+{
+    return guarded([&]() -> int {
+        require(p && out, "null argument");
+        require(p->n >= 2 && p->n <= 1024, "n out of range [2, 1024]");
+        require(p->precond >= 0 && p->precond <= 2, "unknown preconditioner");
+        int ndev = 0;
+        FLIP_CHECK(hipGetDeviceCount(&ndev));
+        require(ndev > 0, "no HIP device", FLIP_E_HIP);
+        FlipCtx *c = new FlipCtx();
+        try {
+            c->prm = *p;
+            c->n = p->n;
+            c->C = (long)p->n * p->n * p->n;
+            c->F = (long)(p->n + 1) * p->n * p->n;
+            c->dt = p->dt;
+            c->rng.seed(p->rand_seed);
+            FLIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            alloc_grids(c);
+            alloc_particles(c, 1);
+        } catch (...) {
+            free_all(c);
+            delete c;
+            throw;
+        }
+        *out = reinterpret_cast<flip_ctx *>(c);
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_destroy(flip_ctx *ctx)
+{
+    if (!ctx) return FLIP_OK;
+    FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return FLIP_OK;
+}
+
+int flip_gpu_synchronize(flip_ctx *ctx)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FLIP_CHECK(hipStreamSynchronize(reinterpret_cast<FlipCtx *>(ctx)->stream));
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_upload_particles(flip_ctx *ctx, int64_t np, const float *pos, const float *vel, const float *nrm,
+                              const int8_t *type, const float *mass, const float *dens)
+{
+    return guarded([&]() -> int {
+        require(ctx && np >= 0, "bad argument");
+        require(np == 0 || (pos && type), "pos and type are required");
+        require(np < (int64_t(1) << 29), "too many particles (limit 2^29)");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        std::vector<float4> p4(np), v4(np), n4(np);
+        std::vector<int32_t> mt(np);
+        for (int64_t i = 0; i < np; ++i) {
+            int t = type[i];
+            require(t == FLIP_FLUID || t == FLIP_WALL, "particle type must be FLUID or WALL");
+            float m = mass ? mass[i] : 1.0f;
+            require(m >= 0.0f, "negative mass");
+            float w = t == FLIP_WALL ? -m : m;
+            if (t == FLIP_WALL && m == 0.0f) w = -0.0f;
+            p4[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w);
+            v4[i] = make_float4(vel ? vel[3 * i] : 0.0f, vel ? vel[3 * i + 1] : 0.0f, vel ? vel[3 * i + 2] : 0.0f,
+                                dens ? dens[i] : 10.0f);
+            n4[i] = make_float4(nrm ? nrm[3 * i] : 0.0f, nrm ? nrm[3 * i + 1] : 0.0f, nrm ? nrm[3 * i + 2] : 0.0f,
+                                0.0f);
+            mt[i] = (int32_t)((uint32_t)i << 2 | (uint32_t)t);
+        }
+        if (np > c->cap || np == 0) alloc_particles(c, std::max<int64_t>(np, 1));
+        c->np = np;
+        c->cur = 0;
+        if (np > 0) {
+            FLIP_CHECK(hipMemcpy(c->pos[0], p4.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+            FLIP_CHECK(hipMemcpy(c->vel[0], v4.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+            FLIP_CHECK(hipMemcpy(c->meta[0], mt.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
+            FLIP_CHECK(hipMemcpy(c->nrm_id, n4.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+        }
+        build_wall_bins(c, p4, mt, n4);
+        c->sorted = false;
+        c->uploaded = true;
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_download_particles(flip_ctx *ctx, float *pos, float *vel, float *nrm, int8_t *type, float *dens)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        int64_t np = c->np;
+        std::vector<float4> p4(np), v4(np), n4(np);
+        std::vector<int32_t> mt(np);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        if (np > 0) {
+            FLIP_CHECK(hipMemcpy(p4.data(), c->P4(), np * sizeof(float4), hipMemcpyDeviceToHost));
+            FLIP_CHECK(hipMemcpy(v4.data(), c->V4(), np * sizeof(float4), hipMemcpyDeviceToHost));
+            FLIP_CHECK(hipMemcpy(mt.data(), c->M(), np * sizeof(int32_t), hipMemcpyDeviceToHost));
+            FLIP_CHECK(hipMemcpy(n4.data(), c->nrm_id, np * sizeof(float4), hipMemcpyDeviceToHost));
+        }
+        for (int64_t s = 0; s < np; ++s) {
+            int64_t id = (uint32_t)mt[s] >> 2;
+            require(id < np, "corrupt particle index", FLIP_E_STATE);
+            if (pos) {
+                pos[3 * id] = p4[s].x;
+                pos[3 * id + 1] = p4[s].y;
+                pos[3 * id + 2] = p4[s].z;
+            }
+            if (vel) {
+                vel[3 * id] = v4[s].x;
+                vel[3 * id + 1] = v4[s].y;
+                vel[3 * id + 2] = v4[s].z;
+            }
+            if (dens) dens[id] = v4[s].w;
+            if (type) type[id] = (int8_t)(mt[s] & 3);
+        }
+        if (nrm)
+            for (int64_t i = 0; i < np; ++i) {
+                nrm[3 * i] = n4[i].x;
+                nrm[3 * i + 1] = n4[i].y;
+                nrm[3 * i + 2] = n4[i].z;
+            }
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_num_particles(flip_ctx *ctx, int64_t *np)
+{
+    if (!ctx || !np) return fail(FLIP_E_ARG, "null argument");
+    *np = reinterpret_cast<FlipCtx *>(ctx)->np;
+    return FLIP_OK;
+}
+
+int flip_gpu_set_max_dens(flip_ctx *ctx, float v)
+{
+    if (!ctx) return fail(FLIP_E_ARG, "null context");
+    reinterpret_cast<FlipCtx *>(ctx)->max_dens = v;
+    return FLIP_OK;
+}
+
+int flip_gpu_get_max_dens(flip_ctx *ctx, float *v)
+{
+    if (!ctx || !v) return fail(FLIP_E_ARG, "null argument");
+    *v = reinterpret_cast<FlipCtx *>(ctx)->max_dens;
+    return FLIP_OK;
+}
+
+int flip_gpu_srand(flip_ctx *ctx, unsigned seed)
+{
+    if (!ctx) return fail(FLIP_E_ARG, "null context");
+    reinterpret_cast<FlipCtx *>(ctx)->rng.seed(seed);
+    return FLIP_OK;
+}
+
+int flip_gpu_set_dt(flip_ctx *ctx, double dt)
+{
+    if (!ctx) return fail(FLIP_E_ARG, "null context");
+    reinterpret_cast<FlipCtx *>(ctx)->dt = dt;
+    return FLIP_OK;
+}
+
+static void *grid_ptr(FlipCtx *c, int which, size_t &bytes)
+{
+    switch (which) {
+    case FLIP_GRID_U0: case FLIP_GRID_U1: case FLIP_GRID_U2:
+        bytes = c->F * sizeof(float);
+        return c->ug[which];
+    case FLIP_GRID_US0: case FLIP_GRID_US1: case FLIP_GRID_US2:
+        bytes = c->F * sizeof(float);
+        return c->us[which - FLIP_GRID_US0];
+    case FLIP_GRID_A:
+        bytes = c->C;
+        return c->A;
+    case FLIP_GRID_L:
+        bytes = c->C * sizeof(float);
+        return c->L;
+    case FLIP_GRID_PRESS:
+        bytes = c->C * sizeof(float);
+        return c->press;
+    case FLIP_GRID_DIV:
+        bytes = c->C * sizeof(float);
+        return c->div;
+    }
+    throw flip_arg_error{FLIP_E_ARG, "unknown grid id"};
+}
+
+int flip_gpu_get_grid(flip_ctx *ctx, int which, void *out)
+{
+    return guarded([&]() -> int {
+        require(ctx && out, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        size_t bytes = 0;
+        void *p = grid_ptr(c, which, bytes);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        FLIP_CHECK(hipMemcpy(out, p, bytes, hipMemcpyDeviceToHost));
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_set_grid(flip_ctx *ctx, int which, const void *in)
+{
+    return guarded([&]() -> int {
+        require(ctx && in, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        size_t bytes = 0;
+        void *p = grid_ptr(c, which, bytes);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        FLIP_CHECK(hipMemcpy(p, in, bytes, hipMemcpyHostToDevice));
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_stage(flip_ctx *ctx, int stage, flip_step_stats *st)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        if (st) memset(st, 0, sizeof(*st));
+        run_stage(c, stage, st);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        FLIP_CHECK(hipGetLastError());
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        flip_step_stats local;
+        if (!st) st = &local;
+        memset(st, 0, sizeof(*st));
+        timing_reset(c);
+        if (!c->ev_a) {
+            FLIP_CHECK(hipEventCreate(&c->ev_a));
+            FLIP_CHECK(hipEventCreate(&c->ev_b));
+        }
+        FLIP_CHECK(hipEventRecord(c->ev_a, c->stream));
+        choose_dt(c);
+        st->dt = c->dt;
+        // flip3D.cpp:1221-1222 sort + computeDensity
+        k_launch_sort(c);
+        c->sorted = true;
+        k_launch_density(c);
+        k_launch_ext_force(c);  // :1230
+        // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
+        k_launch_flags(c);
+        k_launch_p2g(c);
+        run_stage(c, FLIP_STAGE_PROJECT, st);
+        k_launch_extrapolate(c);
+        k_launch_g2p(c);
+        run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
+        if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
+        FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));
+        FLIP_CHECK(hipEventSynchronize(c->ev_b));
+        FLIP_CHECK(hipGetLastError());
+        st->ms_step = elapsed(c->ev_a, c->ev_b);
+        timing_collect(c);
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_map_p2g(flip_ctx *ctx, float *u0, float *u1, float *u2)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        run_stage(c, FLIP_STAGE_P2G, nullptr);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        float *o[3] = {u0, u1, u2};
+        for (int k = 0; k < 3; ++k)
+            if (o[k]) FLIP_CHECK(hipMemcpy(o[k], c->us[k], c->F * sizeof(float), hipMemcpyDeviceToHost));
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_map_g2p(flip_ctx *ctx, const float *u0, const float *u1, const float *u2, int64_t m, const float *pos,
+                     float *vel)
+{
+    return guarded([&]() -> int {
+        require(ctx && u0 && u1 && u2 && (m == 0 || (pos && vel)) && m >= 0, "bad argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        float *g[3] = {nullptr, nullptr, nullptr};
+        float4 *p = nullptr, *o = nullptr;
+        try {
+            const float *src[3] = {u0, u1, u2};
+            for (int k = 0; k < 3; ++k) {
+                dalloc(&g[k], c->F);
+                FLIP_CHECK(hipMemcpy(g[k], src[k], c->F * sizeof(float), hipMemcpyHostToDevice));
+            }
+            std::vector<float4> hp(m), ho(m);
+            for (int64_t i = 0; i < m; ++i) hp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0f);
+            dalloc(&p, m);
+            dalloc(&o, m);
+            if (m > 0) FLIP_CHECK(hipMemcpy(p, hp.data(), m * sizeof(float4), hipMemcpyHostToDevice));
+            k_launch_g2p_points(c, g[0], g[1], g[2], m, p, o);
+            FLIP_CHECK(hipStreamSynchronize(c->stream));
+            if (m > 0) FLIP_CHECK(hipMemcpy(ho.data(), o, m * sizeof(float4), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < m; ++i) {
+                vel[3 * i] = ho[i].x;
+                vel[3 * i + 1] = ho[i].y;
+                vel[3 * i + 2] = ho[i].z;
+            }
+        } catch (...) {
+            for (int k = 0; k < 3; ++k) dfree(g[k]);
+            dfree(p);
+            dfree(o);
+            throw;
+        }
+        for (int k = 0; k < 3; ++k) dfree(g[k]);
+        dfree(p);
+        dfree(o);
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_solve(flip_ctx *ctx, const int8_t *A, const float *L, float *x, float *b, int subcell,
+                   flip_step_stats *st)
+{
+    return guarded([&]() -> int {
+        require(ctx && A && L && x && b, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        flip_step_stats local;
+        if (!st) st = &local;
+        memset(st, 0, sizeof(*st));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        FLIP_CHECK(hipMemcpy(c->A, A, c->C, hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpy(c->L, L, c->C * sizeof(float), hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpy(c->press, x, c->C * sizeof(float), hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpy(c->div, b, c->C * sizeof(float), hipMemcpyHostToDevice));
+        int save = c->prm.subcell;
+        c->prm.subcell = subcell;
+        int rc = 0;
+        try {
+            rc = k_solve(c, st);
+        } catch (...) {
+            c->prm.subcell = save;
+            throw;
+        }
+        c->prm.subcell = save;
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        // the reference's op() leaves non-FLUID x at 0 after the first iteration
+        FLIP_CHECK(hipMemcpy(x, c->press, c->C * sizeof(float), hipMemcpyDeviceToHost));
+        FLIP_CHECK(hipMemcpy(b, c->div, c->C * sizeof(float), hipMemcpyDeviceToHost));
+        if (rc == FLIP_E_NUMERIC) return fail(FLIP_E_NUMERIC, "pressure solve produced NaN");
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_sorted_cells(flip_ctx *ctx, int32_t *counts, int32_t *ids)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        k_launch_sort(c);
+        c->sorted = true;
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        if (counts) {
+            std::vector<int32_t> st(c->C + 1);
+            FLIP_CHECK(hipMemcpy(st.data(), c->cell_start, (c->C + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (long k = 0; k < c->C; ++k) counts[k] = st[k + 1] - st[k];
+        }
+        if (ids && c->np > 0) {
+            std::vector<int32_t> mt(c->np);
+            FLIP_CHECK(hipMemcpy(mt.data(), c->M(), c->np * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (int64_t s = 0; s < c->np; ++s) ids[s] = (int32_t)((uint32_t)mt[s] >> 2);
+        }
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_kernel_times(flip_ctx *ctx, char *names, int names_cap, double *ms, int ms_cap)
+{
+    if (!ctx) return fail(FLIP_E_ARG, "null context");
+    FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+    std::string all;
+    int cnt = (int)c->tms.size();
+    for (int i = 0; i < cnt; ++i) {
+        if (i) all += ";";
+        all += c->tnames[i];
+        if (ms && i < ms_cap) ms[i] = c->tms[i];
+    }
+    if (names && names_cap > 0) {
+        strncpy(names, all.c_str(), names_cap - 1);
+        names[names_cap - 1] = 0;
+    }
+    return cnt;
+}
+
+int flip_gpu_set_profiling(flip_ctx *ctx, int on)
+{
+    if (!ctx) return fail(FLIP_E_ARG, "null context");
+    reinterpret_cast<FlipCtx *>(ctx)->profiling = on != 0;
+    return FLIP_OK;
+}
+
+int flip_gpu_selftest_math(int64_t m, const float *x, const float *y, float *out)
+{
+    return guarded([&]() -> int {
+        require(m >= 0 && (m == 0 || (x && y && out)), "bad argument");
+        float *dx = nullptr, *dy = nullptr, *dout = nullptr;
+        dalloc(&dx, m);
+        dalloc(&dy, m);
+        dalloc(&dout, 4 * m);
+        FLIP_CHECK(hipMemcpy(dx, x, m * sizeof(float), hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpy(dy, y, m * sizeof(float), hipMemcpyHostToDevice));
+        k_launch_selftest(m, dx, dy, dout, nullptr);
+        FLIP_CHECK(hipDeviceSynchronize());
+        FLIP_CHECK(hipMemcpy(out, dout, 4 * m * sizeof(float), hipMemcpyDeviceToHost));
+        dfree(dx);
+        dfree(dy);
+        dfree(dout);
+        return FLIP_OK;
+    });
+}
+
+}  // extern "C"

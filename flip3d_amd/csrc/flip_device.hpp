@@ -1,0 +1,111 @@
+// flip_device.hpp - device arithmetic shared by the FLIP kernels (gfx950).
+//
+// Every helper reproduces the reference's rounding exactly: the library is
+// built with -ffp-contract=off (no FMA fusion), float division and sqrtf are
+// IEEE correctly rounded (HIP default), and double temporaries appear exactly
+// where the reference's double literals put them (src/common.h:10 FLOAT=float).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flipdev {
+
+constexpr int AIR = 0;
+constexpr int FLUID = 1;
+constexpr int WALL = 2;
+
+// Particle metadata word: creation index << 2 | type.
+__device__ __forceinline__ int meta_type(int32_t m) { return m & 3; }
+__device__ __forceinline__ int32_t meta_id(int32_t m) { return (int32_t)((uint32_t)m >> 2); }
+
+__device__ __forceinline__ long cidx(int n, int i, int j, int k) { return ((long)i * n + j) * n + k; }
+// face grids: u0 (n+1,n,n), u1 (n,n+1,n), u2 (n,n,n+1)
+__device__ __forceinline__ long fidx0(int n, int i, int j, int k) { return ((long)i * n + j) * n + k; }
+__device__ __forceinline__ long fidx1(int n, int i, int j, int k) { return ((long)i * (n + 1) + j) * n + k; }
+__device__ __forceinline__ long fidx2(int n, int i, int j, int k) { return ((long)i * n + j) * (n + 1) + k; }
+
+// kernel.cpp:4-6 smooth_kernel: (float)fmax(1.0 - (double)(r2/(h*h)), 0.0).
+// For float q >= 0 the double 1.0 - q is exact whenever the float result can
+// differ from 1.0f, so one float subtraction rounds to the same value.
+__device__ __forceinline__ float smooth_kernel(float r2, float h)
+{
+    float q = r2 / (h * h);
+    float v = 1.0f - q;
+    return v > 0.0f ? v : 0.0f;
+}
+
+// kernel.cpp:8-10 sharp_kernel: (float)fmax((double)(h*h)/fmax((double)r2,1e-5) - 1.0, 0.0).
+// r2 >= h*h gives exactly 0 (quotient <= 1); NaN r2 takes the full path like fmax does.
+__device__ __forceinline__ float sharp_kernel_hh(float r2, float hh)
+{
+    if (r2 >= hh) return 0.0f;
+    double v = (double)hh / fmax((double)r2, 1.0e-5) - 1.0;
+    return (float)fmax(v, 0.0);
+}
+
+// utility.cpp:20-26 length2: float (a*a + b*b) + c*c of a - b
+__device__ __forceinline__ float length2(float ax, float ay, float az, float bx, float by, float bz)
+{
+    float x = ax - bx, y = ay - by, z = az - bz;
+    return x * x + y * y + z * z;
+}
+
+// glibc 2.35 hypotf: sqrt((double)x*x + (double)y*y) rounded to float, with
+// INFINITY when either operand is infinite.
+__device__ __forceinline__ float hypotf_glibc(float x, float y)
+{
+    if (__builtin_isinf(x) || __builtin_isinf(y)) return __builtin_inff();
+    double s = (double)x * (double)x + (double)y * (double)y;
+    return (float)__dsqrt_rn(s);
+}
+
+// utility.cpp:16-18 length: hypotf(hypotf(dx,dy),dz)
+__device__ __forceinline__ float length3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return hypotf_glibc(hypotf_glibc(ax - bx, ay - by), az - bz);
+}
+
+// sorter.cpp:35-37 cell key (int)fmax(0, fmin(gn-1, gn*p)); NaN maps to gn-1.
+__device__ __forceinline__ int cell_of(int gn, float p)
+{
+    float v = (float)gn * p;
+    v = fminf(v, (float)(gn - 1));
+    v = fmaxf(v, 0.0f);
+    return (int)v;
+}
+
+// flip3D.cpp:82,752,903: (int)fmin(N-1, fmax(0, p*N)); NaN maps to 0.
+__device__ __forceinline__ int cell_of_b(int gn, float p)
+{
+    float v = p * (float)gn;
+    v = fmaxf(v, 0.0f);
+    v = fminf(v, (float)(gn - 1));
+    return (int)v;
+}
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Deterministic block sum (fixed tree order) of one double per thread.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < BLOCK / 64; ++i) r += lds[i];
+    }
+    __syncthreads();
+    return r;
+}
+
+}  // namespace flipdev

@@ -1,0 +1,135 @@
+// flip_internal.hpp - host-side context and kernel launch entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/flip3d_gpu.h"
+
+#define FLIP_CHECK(x)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) throw flip_hip_error(e_, #x, __FILE__, __LINE__); \
+    } while (0)
+
+struct flip_hip_error {
+    hipError_t err;
+    std::string what;
+    flip_hip_error(hipError_t e, const char *expr, const char *file, int line);
+};
+
+struct flip_arg_error {
+    int code;
+    std::string what;
+};
+
+// glibc random_r TYPE_3 generator (additive lagged, taps 31/3), seeded like
+// srandom_r; the reference's rand() stream with seed 1.
+struct GlibcRand {
+    int32_t s[31];
+    int f = 3, r = 0;
+    void seed(unsigned seed);
+    int next();
+};
+
+// Device buffers of one context.  Particles live in cell-sorted order and are
+// re-permuted after every sort; meta carries creation index << 2 | type.
+struct FlipCtx {
+    flip_params prm;
+    int n = 0;
+    long C = 0, F = 0;  // cells n^3, face-grid size (n+1) n^2
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+
+    // particles (double-buffered for the permutation gather)
+    int64_t np = 0, cap = 0;
+    int cur = 0;
+    float4 *pos[2] = {nullptr, nullptr};  // x, y, z, m
+    float4 *vel[2] = {nullptr, nullptr};  // ux, uy, uz, dens
+    int32_t *meta[2] = {nullptr, nullptr};
+    float4 *nrm_id = nullptr;   // wall normals by creation index
+    int32_t *inv = nullptr;     // creation index -> storage slot
+    float4 *spos = nullptr;     // P2G positions: clamp(n p, 0, n), m (far away for non-fluid)
+    float4 *tmp0 = nullptr, *tmp1 = nullptr;  // corrector pseudo-position / velocity
+    int32_t *pkey = nullptr, *pslot = nullptr, *perm = nullptr;
+
+    // cells
+    int32_t *cell_count = nullptr, *cell_start = nullptr;
+    int32_t *wall_start = nullptr;  // static wall bins
+    float4 *wall_pos = nullptr, *wall_nrm = nullptr;
+    int64_t nwall = 0;
+
+    // grids
+    float *ug[3] = {nullptr, nullptr, nullptr}, *us[3] = {nullptr, nullptr, nullptr};
+    int8_t *A = nullptr;
+    float *L = nullptr, *press = nullptr, *div = nullptr;
+
+    // solver
+    int32_t *fl = nullptr;  // fluid cell list (ascending cell index)
+    int nfl = 0;
+    double *P = nullptr;
+    float *r = nullptr, *z = nullptr, *q = nullptr, *sv[2] = {nullptr, nullptr};
+    double *partials = nullptr;  // per-block partial sums
+    double *scal = nullptr;      // device scalars (see pcg kernels)
+    int32_t *iflags = nullptr;   // device int flags / counters
+    unsigned *tickets = nullptr;
+    int last_iters = 0;
+
+    // scan scratch
+    int32_t *scan_tmp = nullptr, *scan_blocks = nullptr;
+    int32_t *deep = nullptr;
+
+    // event buffers (host-drawn rand() consumers)
+    int32_t *ev_list = nullptr;  // (slot) of particles with events
+    int32_t *ev_count = nullptr;
+    float *ev_rand = nullptr;    // rand()%101 values, 3 per event
+    int32_t *ev_off = nullptr;   // per listed particle: offset into ev_rand
+    int64_t ev_cap = 0;
+    int32_t *h_pinned = nullptr;  // pinned host scalars
+
+    float max_dens = 1.0f;
+    double dt = 6e-3;
+    GlibcRand rng;
+    bool uploaded = false;
+    bool sorted = false;  // storage order matches the current positions
+
+    // timing
+    std::vector<std::string> tnames;
+    std::vector<hipEvent_t> tev;
+    std::vector<double> tms;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
+
+    float4 *P4() const { return pos[cur]; }
+    float4 *V4() const { return vel[cur]; }
+    int32_t *M() const { return meta[cur]; }
+};
+
+// ---- kernels (launch wrappers; each enqueues on ctx->stream) ----
+void k_launch_sort(FlipCtx *c);                 // counting sort + per-cell id order + permute
+void k_launch_density(FlipCtx *c);
+void k_launch_ext_force(FlipCtx *c);
+void k_launch_flags(FlipCtx *c);                // A and L from the sorted buckets
+void k_launch_p2g(FlipCtx *c);                  // P2G + save_grid + enforce_boundary
+void k_launch_divergence(FlipCtx *c);           // div on FLUID cells
+void k_launch_gradient(FlipCtx *c);             // ghost-fluid gradient + enforce_boundary
+void k_launch_extrapolate(FlipCtx *c);          // extrapolate_velocity + subtract_grid
+void k_launch_g2p(FlipCtx *c);                  // FLIP/PIC blend
+void k_launch_advect(FlipCtx *c);               // advect + clamp + wall push-out
+int k_stuck_and_reposition(FlipCtx *c);         // returns repositioned count (host syncs)
+int k_correct(FlipCtx *c);                      // corrector; returns jitter events
+void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n);  // exclusive scan, out[n] = total
+int k_build_fluid_list(FlipCtx *c);             // returns nfl (host sync)
+int k_solve(FlipCtx *c, flip_step_stats *st);   // PCG on (A, L, press, div)
+void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const float *g2, int64_t m,
+                         const float4 *p, float4 *out);
+void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s);
+double k_max_speed(FlipCtx *c);
+
+// records the message for flip_gpu_last_error and returns code
+int flip_internal_fail(int code, const char *msg);
+
+// timing helpers
+void t_begin(FlipCtx *c, const char *name);
+void t_end(FlipCtx *c);

@@ -1,0 +1,886 @@
+// k_part.hip - particle-side kernels of the FLIP step on cell-sorted storage.
+//
+// Particles are stored sorted by (cell, creation index), so the particles of
+// the cells (si, sj, klo..khi) form ONE contiguous range
+// [cell_start[c(si,sj,klo)], cell_start[c(si,sj,khi)+1]) whose order is exactly
+// the reference's window order (sorter.cpp:42-64: si -> sj -> sk, then bucket
+// order).  A 3x3x3 window is 9 ranges, a 2x4x4 P2G window 8 ranges.
+// pos.w carries the mass with the sign bit set for WALL particles, so
+// neighbour loops need no type load.
+#include "flip_device.hpp"
+#include "flip_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+using namespace flipdev;
+
+namespace {
+
+constexpr int TB = 256;
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__device__ __forceinline__ bool is_wall_w(float w) { return __builtin_signbit(w); }
+__device__ __forceinline__ float mass_of(float w) { return __builtin_fabsf(w); }
+
+// ------------------------------------------------------------- density
+// flip3D.cpp:166-190 computeDensity (+ add_ExtForce when gdt != 0 is fused
+// by the step driver through a separate call).
+__global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 *__restrict__ pos,
+                                                float4 *__restrict__ vel, const int32_t *__restrict__ cs, float h,
+                                                float max_dens)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) {
+        vel[i].w = 1.0f;
+        return;
+    }
+    int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    float wsum = 0.0f;
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            for (int a = b; a < e; ++a) {
+                float4 q = pos[a];
+                if (is_wall_w(q.w)) continue;
+                float d2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
+                float w = q.w * smooth_kernel(d2, h);
+                wsum += w;
+            }
+        }
+    }
+    vel[i].w = wsum / max_dens;
+}
+
+// flip3D.cpp:870-875 add_ExtForce: u_y = (float)((double)u_y + gdt), gdt = -dt*g
+__global__ void k_ext_force(int64_t np, float4 *vel, double gdt)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    vel[i].y = (float)((double)vel[i].y + gdt);
+}
+
+// --------------------------------------------------------- flags / level set
+// sorter.cpp:70-93 markWater + levelset (flip3D.cpp:824-826 recomputes the
+// same L): WALL if the bucket holds a wall particle (L = 1.0), else
+// L = (float)(0.2*n0 - accm) with accm the float sum of dens in bucket order.
+__global__ void k_flags(long C, const int32_t *__restrict__ cs, const float4 *__restrict__ pos,
+                        const float4 *__restrict__ vel, int8_t *A, float *L, double c0)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    int b = cs[c], e = cs[c + 1];
+    float accm = 0.0f;
+    bool wall = false;
+    for (int a = b; a < e; ++a) {
+        if (is_wall_w(pos[a].w)) {
+            wall = true;
+            break;
+        }
+        accm += vel[a].w;
+    }
+    if (wall) {
+        A[c] = WALL;
+        L[c] = 1.0f;
+    } else {
+        float l = (float)(c0 - (double)accm);
+        L[c] = l;
+        A[c] = l < 0.0f ? FLUID : AIR;
+    }
+}
+
+// ------------------------------------------------------------------- P2G
+// P2G input: clamp(n*p, 0, n) per axis (mapper.cpp:34-36) for FLUID particles;
+// others are placed infinitely far so every weight test rejects them.
+__global__ void k_p2g_prep(int64_t np, int n, const float4 *__restrict__ pos, float4 *spos)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) {
+        const float inf = __builtin_inff();
+        spos[i] = make_float4(inf, inf, inf, 0.0f);
+        return;
+    }
+    float g = (float)n;
+    float x = fmaxf(fminf(g * p.x, g), 0.0f);
+    float y = fmaxf(fminf(g * p.y, g), 0.0f);
+    float z = fmaxf(fminf(g * p.z, g), 0.0f);
+    spos[i] = make_float4(x, y, z, p.w);
+}
+
+__device__ __forceinline__ int wall_chk(int8_t a) { return a == WALL ? 1 : -1; }
+
+// mapper.cpp:17-88 one face component per launch.  COMP selects the face
+// family: X faces (i, j+.5, k+.5) over window [i-1,i]x[j-2,j+1]x[k-2,k+1],
+// and the analogous Y/Z windows.  Writes u_save = raw value (save_grid,
+// flip3D.cpp:971) and u = value with enforce_boundary (flip3D.cpp:783-805).
+template <int COMP>
+__global__ __launch_bounds__(TB) void k_p2g(int n, const float4 *__restrict__ spos, const float4 *__restrict__ vel,
+                                            const int32_t *__restrict__ cs, const int8_t *__restrict__ A,
+                                            float *__restrict__ ug, float *__restrict__ us, float hh)
+{
+    const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
+    long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= (long)ni * nj * nk) return;
+    int k = (int)(f % nk);
+    int j = (int)((f / nk) % nj);
+    int i = (int)(f / ((long)nk * nj));
+    float fx = COMP == 0 ? (float)i : (float)(i + 0.5);
+    float fy = COMP == 1 ? (float)j : (float)(j + 0.5);
+    float fz = COMP == 2 ? (float)k : (float)(k + 0.5);
+    const int wi = COMP == 0 ? 1 : 2, wj = COMP == 1 ? 1 : 2, wk = COMP == 2 ? 1 : 2;
+    int klo = max(k - wk, 0), khi = min(k + wk - 1, n - 1);
+    float sumw = 0.0f, sumx = 0.0f;
+    if (klo <= khi) {
+        for (int si = i - wi; si <= i + wi - 1; ++si) {
+            if (si < 0 || si > n - 1) continue;
+            for (int sj = j - wj; sj <= j + wj - 1; ++sj) {
+                if (sj < 0 || sj > n - 1) continue;
+                int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+                for (int a = b; a < e; ++a) {
+                    float4 q = spos[a];
+                    float r2 = length2(q.x, q.y, q.z, fx, fy, fz);
+                    if (!(r2 < hh)) continue;  // weight is exactly 0 (sharp_kernel_hh)
+                    float w = q.w * sharp_kernel_hh(r2, hh);
+                    float4 v = vel[a];
+                    float uc = COMP == 0 ? v.x : (COMP == 1 ? v.y : v.z);
+                    sumx += w * uc;
+                    sumw += w;
+                }
+            }
+        }
+    }
+    float val = sumw != 0.0f ? sumx / sumw : 0.0f;
+    us[f] = val;
+    int a0 = COMP == 0 ? i : (COMP == 1 ? j : k);
+    int na = COMP == 0 ? ni : (COMP == 1 ? nj : nk);
+    bool zero = a0 == 0 || a0 == na - 1;
+    if (!zero) {
+        long c1 = cidx(n, i, j, k);
+        long c0 = COMP == 0 ? cidx(n, i - 1, j, k) : (COMP == 1 ? cidx(n, i, j - 1, k) : cidx(n, i, j, k - 1));
+        zero = wall_chk(A[c1]) * wall_chk(A[c0]) < 0;
+    }
+    ug[f] = zero ? 0.0f : val;
+}
+
+// ------------------------------------------------------------------- G2P
+// mapper.cpp:96-106 linear(), including its k = min(z, h-2) clamp; a read
+// past the last k layer (grid[1] rows, mapper.cpp:102) returns 0.
+template <int COMP>
+__device__ __forceinline__ float grid_at(const float *q, int n, int i, int j, int k, int d)
+{
+    if (k >= d) return 0.0f;
+    long f = COMP == 0 ? fidx0(n, i, j, k) : (COMP == 1 ? fidx1(n, i, j, k) : fidx2(n, i, j, k));
+    return q[f];
+}
+
+template <int COMP>
+__device__ __forceinline__ float linear(const float *__restrict__ q, int n, float x, float y, float z)
+{
+    const int w = n + (COMP == 0), h = n + (COMP == 1), d = n + (COMP == 2);
+    x = fmaxf(0.0f, fminf((float)w, x));
+    y = fmaxf(0.0f, fminf((float)h, y));
+    z = fmaxf(0.0f, fminf((float)d, z));
+    int i = (int)(x > (float)(w - 2) ? (float)(w - 2) : x);
+    int j = (int)(y > (float)(h - 2) ? (float)(h - 2) : y);
+    int k = (int)(z > (float)(h - 2) ? (float)(h - 2) : z);
+    float ix = (float)(i + 1) - x, xi = x - (float)i;
+    float jy = (float)(j + 1) - y, yj = y - (float)j;
+    float kz = (float)(k + 1) - z, zk = z - (float)k;
+    float lo = (ix * grid_at<COMP>(q, n, i, j, k, d) + xi * grid_at<COMP>(q, n, i + 1, j, k, d)) * jy +
+               (ix * grid_at<COMP>(q, n, i, j + 1, k, d) + xi * grid_at<COMP>(q, n, i + 1, j + 1, k, d)) * yj;
+    float hi = (ix * grid_at<COMP>(q, n, i, j, k + 1, d) + xi * grid_at<COMP>(q, n, i + 1, j, k + 1, d)) * jy +
+               (ix * grid_at<COMP>(q, n, i, j + 1, k + 1, d) + xi * grid_at<COMP>(q, n, i + 1, j + 1, k + 1, d)) * yj;
+    return kz * lo + zk * hi;
+}
+
+// mapper.cpp:108-112 fetchVelocity; gn*p - 0.5 in double rounds like the
+// float subtraction (one rounding of an exact value).
+__device__ __forceinline__ float3 fetch_velocity(const float *g0, const float *g1, const float *g2, int n, float px,
+                                                 float py, float pz)
+{
+    float gx = (float)n * px, gy = (float)n * py, gz = (float)n * pz;
+    float3 u;
+    u.x = linear<0>(g0, n, gx, gy - 0.5f, gz - 0.5f);
+    u.y = linear<1>(g1, n, gx - 0.5f, gy, gz - 0.5f);
+    u.z = linear<2>(g2, n, gx - 0.5f, gy - 0.5f, gz);
+    return u;
+}
+
+// flip3D.cpp:1071-1097 FLIP/PIC: t = G2P(du) + u; u = (1-a)*G2P(u_grid) + a*t
+// in double, rounded to float.  Wall velocities are not read by the step and
+// are left as they are.
+__global__ __launch_bounds__(TB) void k_g2p_flip(int64_t np, int n, const float4 *__restrict__ pos,
+                                                 float4 *__restrict__ vel, const float *__restrict__ u0,
+                                                 const float *__restrict__ u1, const float *__restrict__ u2,
+                                                 const float *__restrict__ d0, const float *__restrict__ d1,
+                                                 const float *__restrict__ d2, double c_pic, double c_flip)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) return;
+    float4 v = vel[i];
+    float3 du = fetch_velocity(d0, d1, d2, n, p.x, p.y, p.z);
+    float tx = du.x + v.x, ty = du.y + v.y, tz = du.z + v.z;
+    float3 pic = fetch_velocity(u0, u1, u2, n, p.x, p.y, p.z);
+    v.x = (float)(c_pic * (double)pic.x + c_flip * (double)tx);
+    v.y = (float)(c_pic * (double)pic.y + c_flip * (double)ty);
+    v.z = (float)(c_pic * (double)pic.z + c_flip * (double)tz);
+    vel[i] = v;
+}
+
+__global__ void k_g2p_points(int64_t m, int n, const float4 *__restrict__ p, const float *__restrict__ u0,
+                             const float *__restrict__ u1, const float *__restrict__ u2, float4 *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    float4 q = p[i];
+    float3 u = fetch_velocity(u0, u1, u2, n, q.x, q.y, q.z);
+    out[i] = make_float4(u.x, u.y, u.z, 0.0f);
+}
+
+// ---------------------------------------------------------------- advect
+// flip3D.cpp:877-930: p += dt*G2P(u) (double), clamp to [1/N, 1-1/N], then
+// push out of wall particles within 1.5*DENSITY/N, in window order, updating
+// p in place and projecting out the normal velocity.  Wall particles never
+// move, so the static wall bins are the reference's post-advect wall buckets.
+__global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                               const float *__restrict__ u0, const float *__restrict__ u1,
+                                               const float *__restrict__ u2, double dt, float rlo, double re,
+                                               const int32_t *__restrict__ wst, const float4 *__restrict__ wpos,
+                                               const float4 *__restrict__ wnrm)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) return;
+    float4 v = vel[i];
+    float3 u = fetch_velocity(u0, u1, u2, n, p.x, p.y, p.z);
+    p.x = (float)((double)p.x + dt * (double)u.x);
+    p.y = (float)((double)p.y + dt * (double)u.y);
+    p.z = (float)((double)p.z + dt * (double)u.z);
+    double lo = (double)rlo, hi = 1.0 - (double)rlo;
+    p.x = (float)fmax(lo, fmin(hi, (double)p.x));
+    p.y = (float)fmax(lo, fmin(hi, (double)p.y));
+    p.z = (float)fmax(lo, fmin(hi, (double)p.z));
+    int ci = cell_of_b(n, p.x), cj = cell_of_b(n, p.y), ck = cell_of_b(n, p.z);
+    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            int b = wst[cidx(n, si, sj, klo)], e = wst[cidx(n, si, sj, khi) + 1];
+            for (int a = b; a < e; ++a) {
+                float4 q = wpos[a];
+                float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+                if ((double)dist < re) {
+                    float4 nr = wnrm[a];
+                    float nx = nr.x, ny = nr.y, nz = nr.z;
+                    if (nx == 0.0f && ny == 0.0f && nz == 0.0f && dist != 0.0f) {
+                        nx = (p.x - q.x) / dist;
+                        ny = (p.y - q.y) / dist;
+                        nz = (p.z - q.z) / dist;
+                    }
+                    double sh = re - (double)dist;
+                    p.x = (float)((double)p.x + sh * (double)nx);
+                    p.y = (float)((double)p.y + sh * (double)ny);
+                    p.z = (float)((double)p.z + sh * (double)nz);
+                    float dot = v.x * nx + v.y * ny + v.z * nz;
+                    v.x -= dot * nx;
+                    v.y -= dot * ny;
+                    v.z -= dot * nz;
+                }
+            }
+        }
+    }
+    pos[i] = p;
+    vel[i] = v;
+}
+
+// flip3D.cpp:933-955 stuck flags: in a WALL cell, or thin (dens < 0.04) next
+// to a wall in y.  Appends creation indices (unordered; the host sorts them).
+__global__ void k_stuck(int64_t np, int n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                        const int32_t *__restrict__ meta, const int8_t *__restrict__ A, int32_t *list, int32_t *count)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) return;
+    bool rem = A[cidx(n, cell_of_b(n, p.x), cell_of_b(n, p.y), cell_of_b(n, p.z))] == WALL;
+    float g = (float)n;
+    int ii = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.x * g)));
+    int jj = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.y * g)));
+    int kk = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.z * g)));
+    int jm = jj - 1 > 0 ? jj - 1 : 0, jp = jj + 1 < n - 1 ? jj + 1 : n - 1;
+    if ((double)vel[i].w < 0.04 && (A[cidx(n, ii, jm, kk)] == WALL || A[cidx(n, ii, jp, kk)] == WALL)) rem = true;
+    if (rem) {
+        int32_t at = atomicAdd(count, 1);
+        list[at] = meta_id(meta[i]);
+    }
+}
+
+// flip3D.cpp:414-421 deep-water cells: FLUID with every in-range neighbour FLUID
+__global__ void k_deep(long C, int n, const int8_t *__restrict__ A, int32_t *flag)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
+    bool d = A[c] == FLUID;
+    if (d && i > 0 && A[c - (long)n * n] != FLUID) d = false;
+    if (d && i < n - 1 && A[c + (long)n * n] != FLUID) d = false;
+    if (d && j > 0 && A[c - n] != FLUID) d = false;
+    if (d && j < n - 1 && A[c + n] != FLUID) d = false;
+    if (d && k > 0 && A[c - 1] != FLUID) d = false;
+    if (d && k < n - 1 && A[c + 1] != FLUID) d = false;
+    flag[c] = d ? 1 : 0;
+}
+
+__global__ void k_deep_compact(long C, const int32_t *__restrict__ flag, const int32_t *__restrict__ at, int limit,
+                               int32_t *out)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    if (flag[c] && at[c] < limit) out[at[c]] = (int32_t)c;
+}
+
+// write positions of particles given by creation index
+__global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const float *__restrict__ xyz,
+                                const int32_t *__restrict__ inv, float4 *pos)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    int32_t s = inv[ids[t]];
+    float4 p = pos[s];
+    p.x = xyz[3 * t];
+    p.y = xyz[3 * t + 1];
+    p.z = xyz[3 * t + 2];
+    pos[s] = p;
+}
+
+// flip3D.cpp:445-450: resample each repositioned particle in list order with
+// re = h = 1/N (corrector.cpp:15-47), u starting from 0.  One block: weights
+// are evaluated in parallel, summed serially in window order, and each
+// particle's result is visible to the next (the reference's loop order).
+constexpr int RCH = 1024;
+__global__ __launch_bounds__(TB) void k_resample_list(int m, const int32_t *__restrict__ ids, const int32_t *inv,
+                                                      int n, const float4 *__restrict__ pos, float4 *vel,
+                                                      const int32_t *__restrict__ cs, float hh)
+{
+    __shared__ float sw[RCH], sx[RCH], sy[RCH], sz[RCH];
+    __shared__ int rb[9], re_[9];
+    __shared__ float acc[4];
+    for (int t = 0; t < m; ++t) {
+        int32_t slot = inv[ids[t]];
+        float4 p = pos[slot];
+        int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+        int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+        if (threadIdx.x < 9) {
+            int si = ci - 1 + threadIdx.x / 3, sj = cj - 1 + threadIdx.x % 3;
+            int b = 0, e = 0;
+            if (si >= 0 && si < n && sj >= 0 && sj < n) {
+                b = cs[cidx(n, si, sj, klo)];
+                e = cs[cidx(n, si, sj, khi) + 1];
+            }
+            rb[threadIdx.x] = b;
+            re_[threadIdx.x] = e;
+        }
+        if (threadIdx.x == 0) acc[0] = acc[1] = acc[2] = acc[3] = 0.0f;
+        __syncthreads();
+        int total = 0;
+        for (int r = 0; r < 9; ++r) total += re_[r] - rb[r];
+        for (int base = 0; base < total; base += RCH) {
+            for (int e = threadIdx.x; e < RCH && base + e < total; e += blockDim.x) {
+                int g = base + e, r = 0;
+                while (g >= re_[r] - rb[r]) {
+                    g -= re_[r] - rb[r];
+                    ++r;
+                }
+                int a = rb[r] + g;
+                float4 q = pos[a];
+                float w = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+                if (!is_wall_w(q.w)) {
+                    float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
+                    w = q.w * sharp_kernel_hh(d2, hh);
+                    float4 u = vel[a];
+                    wx = w * u.x;
+                    wy = w * u.y;
+                    wz = w * u.z;
+                } else {
+                    w = -1.0f;  // marks "not a FLUID neighbour"
+                }
+                sw[e] = w;
+                sx[e] = wx;
+                sy[e] = wy;
+                sz[e] = wz;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int lim = min(RCH, total - base);
+                float ax = acc[0], ay = acc[1], az = acc[2], aw = acc[3];
+                for (int e = 0; e < lim; ++e) {
+                    if (sw[e] < 0.0f) continue;
+                    ax += sx[e];
+                    ay += sy[e];
+                    az += sz[e];
+                    aw += sw[e];
+                }
+                acc[0] = ax;
+                acc[1] = ay;
+                acc[2] = az;
+                acc[3] = aw;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            float4 v = vel[slot];
+            if (acc[3] != 0.0f) {
+                v.x = acc[0] / acc[3];
+                v.y = acc[1] / acc[3];
+                v.z = acc[2] / acc[3];
+            } else {
+                v.x = v.y = v.z = 0.0f;
+            }
+            vel[slot] = v;
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- corrector
+// corrector.cpp:55-88 spring for one particle.  RAND == false: stops at the
+// first random-jitter event and reports it; RAND == true: consumes the
+// host-drawn rand()%101 values in window order.
+template <bool RAND>
+__device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__restrict__ pos,
+                                          const int32_t *__restrict__ meta, const float4 *__restrict__ nrm_id,
+                                          const int32_t *__restrict__ cs, float dt, float re, const float *rv,
+                                          float4 *tmp0)
+{
+    float4 p = pos[i];
+    float g = (float)n;
+    float cx = g * p.x, cy = g * p.y, cz = g * p.z;
+    // max(0, min(cs, cs*p)) with the reference's macros (utility.h:16-17)
+    float x = g > cx ? cx : g;
+    x = 0.0f > x ? 0.0f : x;
+    float y = g > cy ? cy : g;
+    y = 0.0f > y ? 0.0f : y;
+    float z = g > cz ? cz : g;
+    z = 0.0f > z ? 0.0f : z;
+    int ci = (int)x, cj = (int)y, ck = (int)z;
+    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    int events = 0;
+    const double rre = 0.1 * (double)re;
+    const double cj_ = 0.01 * (double)re / (double)dt;
+    const double cw = 0.05 * (double)re / (double)dt;
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            if (klo > khi) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            for (int a = b; a < e; ++a) {
+                if (a == i) continue;
+                float4 q = pos[a];
+                float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+                float w = (float)(50.0 * (double)mass_of(q.w) * (double)smooth_kernel(dist * dist, re));
+                if ((double)dist > rre) {
+                    sx += w * (p.x - q.x) / dist * re;
+                    sy += w * (p.y - q.y) / dist * re;
+                    sz += w * (p.z - q.z) / dist * re;
+                } else if (!is_wall_w(q.w)) {
+                    if (!RAND) return -1;
+                    sx = (float)((double)sx + cj_ * (double)rv[3 * events + 0] / 100.0);
+                    sy = (float)((double)sy + cj_ * (double)rv[3 * events + 1] / 100.0);
+                    sz = (float)((double)sz + cj_ * (double)rv[3 * events + 2] / 100.0);
+                    events++;
+                } else {
+                    float4 nr = nrm_id[meta_id(meta[a])];
+                    sx = (float)((double)sx + cw * (double)nr.x);
+                    sy = (float)((double)sy + cw * (double)nr.y);
+                    sz = (float)((double)sz + cw * (double)nr.z);
+                }
+            }
+        }
+    }
+    tmp0[i] = make_float4(p.x + dt * sx, p.y + dt * sy, p.z + dt * sz, 0.0f);
+    return events;
+}
+
+__global__ __launch_bounds__(TB) void k_spring(int64_t np, int n, const float4 *__restrict__ pos,
+                                               const int32_t *__restrict__ meta, const float4 *__restrict__ nrm_id,
+                                               const int32_t *__restrict__ cs, float dt, float re, float4 *tmp0,
+                                               int32_t *list, int32_t *count)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    if (is_wall_w(pos[i].w)) return;
+    int r = spring_one<false>(i, n, pos, meta, nrm_id, cs, dt, re, nullptr, tmp0);
+    if (r < 0) {
+        int32_t at = atomicAdd(count, 1);
+        list[2 * at] = meta_id(meta[i]);
+        list[2 * at + 1] = (int32_t)i;
+    }
+}
+
+// count the jitter events of listed particles (for the host's rand() draws)
+__global__ void k_spring_count(int m, const int32_t *__restrict__ list, int n, const float4 *__restrict__ pos,
+                               const int32_t *__restrict__ cs, float re, int32_t *events)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    int64_t i = list[t];
+    float4 p = pos[i];
+    float g = (float)n;
+    float cx = g * p.x, cy = g * p.y, cz = g * p.z;
+    float x = g > cx ? cx : g;
+    x = 0.0f > x ? 0.0f : x;
+    float y = g > cy ? cy : g;
+    y = 0.0f > y ? 0.0f : y;
+    float z = g > cz ? cz : g;
+    z = 0.0f > z ? 0.0f : z;
+    int ci = (int)x, cj = (int)y, ck = (int)z;
+    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    const double rre = 0.1 * (double)re;
+    int ev = 0;
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1 || klo > khi) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            for (int a = b; a < e; ++a) {
+                if (a == i) continue;
+                float4 q = pos[a];
+                float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+                if (!((double)dist > rre) && !is_wall_w(q.w)) ev++;
+            }
+        }
+    }
+    events[t] = ev;
+}
+
+__global__ void k_spring_fix(int m, const int32_t *__restrict__ list, const int32_t *__restrict__ off, int n,
+                             const float4 *__restrict__ pos, const int32_t *__restrict__ meta,
+                             const float4 *__restrict__ nrm_id, const int32_t *__restrict__ cs, float dt, float re,
+                             const float *__restrict__ rv, float4 *tmp0)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    spring_one<true>(list[t], n, pos, meta, nrm_id, cs, dt, re, rv + off[t], tmp0);
+}
+
+// corrector.cpp:15-47 resample at the pseudo-moved position with the old
+// positions and velocities (re = DENSITY/N); keeps u when no weight.
+__global__ __launch_bounds__(TB) void k_resample(int64_t np, int n, const float4 *__restrict__ pos,
+                                                 const float4 *__restrict__ vel, const int32_t *__restrict__ cs,
+                                                 const float4 *__restrict__ tmp0, float4 *tmp1, float hh)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    if (is_wall_w(pos[i].w)) return;
+    float4 p = tmp0[i];
+    float4 v0 = vel[i];
+    int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            for (int a = b; a < e; ++a) {
+                float4 q = pos[a];
+                if (is_wall_w(q.w)) continue;
+                float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
+                if (!(d2 < hh)) continue;  // weight exactly 0
+                float w = q.w * sharp_kernel_hh(d2, hh);
+                float4 u = vel[a];
+                ax += w * u.x;
+                ay += w * u.y;
+                az += w * u.z;
+                wsum += w;
+            }
+        }
+    }
+    if (wsum != 0.0f) tmp1[i] = make_float4(ax / wsum, ay / wsum, az / wsum, 0.0f);
+    else tmp1[i] = make_float4(v0.x, v0.y, v0.z, 0.0f);
+}
+
+// corrector.cpp:103-114 update
+__global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const float4 *__restrict__ tmp0,
+                                 const float4 *__restrict__ tmp1)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) return;
+    float4 a = tmp0[i], b = tmp1[i];
+    pos[i] = make_float4(a.x, a.y, a.z, p.w);
+    float4 v = vel[i];
+    vel[i] = make_float4(b.x, b.y, b.z, v.w);
+}
+
+__global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float4 *__restrict__ vel, double *out,
+                         unsigned *ticket, double *partials)
+{
+    __shared__ double lds[TB / 64];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i < np && !is_wall_w(pos[i].w)) {
+        float4 v = vel[i];
+        m = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < TB / 64; ++w) r = fmax(r, lds[w]);
+        partials[blockIdx.x] = r;
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double mm = 0.0;
+            for (unsigned b = 0; b < gridDim.x; ++b) mm = fmax(mm, ((volatile double *)partials)[b]);
+            *out = mm;
+            *ticket = 0;
+        }
+    }
+}
+
+__global__ void k_selftest(int64_t m, const float *x, const float *y, float *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    float a = x[i], b = y[i];
+    out[4 * i + 0] = a / b;
+    out[4 * i + 1] = sqrtf(a);
+    out[4 * i + 2] = hypotf_glibc(a, b);
+    out[4 * i + 3] = (float)((double)a / (double)b);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+void k_launch_density(FlipCtx *c)
+{
+    t_begin(c, "density");
+    if (c->np > 0)
+        hipLaunchKernelGGL(k_density, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(),
+                           c->cell_start, (float)(4.0 * c->prm.density / c->n), c->max_dens);
+    t_end(c);
+}
+
+void k_launch_ext_force(FlipCtx *c)
+{
+    t_begin(c, "ext_force");
+    if (c->np > 0)
+        hipLaunchKernelGGL(k_ext_force, nblk(c->np, TB), TB, 0, c->stream, c->np, c->V4(), -c->dt * c->prm.gravity);
+    t_end(c);
+}
+
+void k_launch_flags(FlipCtx *c)
+{
+    float d = (float)c->prm.density;
+    float n0 = (float)(1.0 / (d * d * d));
+    t_begin(c, "flags");
+    hipLaunchKernelGGL(k_flags, nblk(c->C, TB), TB, 0, c->stream, c->C, c->cell_start, c->P4(), c->V4(), c->A,
+                       c->L, 0.2 * (double)n0);
+    t_end(c);
+}
+
+void k_launch_p2g(FlipCtx *c)
+{
+    const float re = (float)1.4;  // RE, mapper.cpp:14
+    const float hh = re * re;
+    t_begin(c, "p2g_prep");
+    if (c->np > 0) hipLaunchKernelGGL(k_p2g_prep, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->spos);
+    t_end(c);
+    t_begin(c, "p2g");
+    unsigned nb = nblk(c->F, TB);
+    hipLaunchKernelGGL(k_p2g<0>, nb, TB, 0, c->stream, c->n, c->spos, c->V4(), c->cell_start, c->A, c->ug[0],
+                       c->us[0], hh);
+    hipLaunchKernelGGL(k_p2g<1>, nb, TB, 0, c->stream, c->n, c->spos, c->V4(), c->cell_start, c->A, c->ug[1],
+                       c->us[1], hh);
+    hipLaunchKernelGGL(k_p2g<2>, nb, TB, 0, c->stream, c->n, c->spos, c->V4(), c->cell_start, c->A, c->ug[2],
+                       c->us[2], hh);
+    t_end(c);
+}
+
+void k_launch_g2p(FlipCtx *c)
+{
+    t_begin(c, "g2p_flip");
+    double a = c->prm.alpha;
+    if (c->np > 0)
+        hipLaunchKernelGGL(k_g2p_flip, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(), c->ug[0],
+                           c->ug[1], c->ug[2], c->us[0], c->us[1], c->us[2], 1.0 - a, a);
+    t_end(c);
+}
+
+void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const float *g2, int64_t m, const float4 *p,
+                         float4 *out)
+{
+    if (m > 0) hipLaunchKernelGGL(k_g2p_points, nblk(m, TB), TB, 0, c->stream, m, c->n, p, g0, g1, g2, out);
+}
+
+void k_launch_advect(FlipCtx *c)
+{
+    t_begin(c, "advect");
+    if (c->np > 0)
+        hipLaunchKernelGGL(k_advect, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(), c->ug[0],
+                           c->ug[1], c->ug[2], c->dt, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n,
+                           c->wall_start, c->wall_pos, c->wall_nrm);
+    t_end(c);
+}
+
+void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s)
+{
+    if (m > 0) hipLaunchKernelGGL(k_selftest, nblk(m, TB), TB, 0, s, m, x, y, out);
+}
+
+double k_max_speed(FlipCtx *c)
+{
+    if (c->np == 0) return 0.0;
+    unsigned nb = nblk(c->np, TB);
+    hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), c->scal + 15, c->tickets + 3,
+                       c->partials);
+    double h = 0.0;
+    FLIP_CHECK(hipMemcpyAsync(&h, c->scal + 15, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FLIP_CHECK(hipStreamSynchronize(c->stream));
+    return sqrt(h);
+}
+
+// Stuck detection and repositioning (flip3D.cpp:932-968, 405-451).  The
+// device flags and lists; the host sorts the list, shuffles the deep-water
+// cells and draws positions with the reference's rand() order; the device
+// re-sorts and resamples.  Returns the number of particles moved.  Always
+// leaves the particles sorted on their current positions.
+int k_stuck_and_reposition(FlipCtx *c)
+{
+    int n = c->n;
+    int m = 0;
+    std::vector<int32_t> ids;
+    if (c->prm.reposition && c->np > 0) {
+        t_begin(c, "stuck");
+        FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, sizeof(int32_t), c->stream));
+        hipLaunchKernelGGL(k_stuck, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->V4(), c->M(), c->A,
+                           c->ev_list, c->ev_count);
+        t_end(c);
+        FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        m = c->h_pinned[0];
+    }
+    int moved = 0;
+    if (m > 0) {
+        ids.resize(m);
+        FLIP_CHECK(hipMemcpy(ids.data(), c->ev_list, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+        std::sort(ids.begin(), ids.end());
+        // deep-water cells in scan order (flip3D.cpp:411-430)
+        hipLaunchKernelGGL(k_deep, nblk(c->C, TB), TB, 0, c->stream, c->C, n, c->A, c->cell_count);
+        k_launch_scan(c, c->cell_count, c->scan_tmp, c->C);
+        hipLaunchKernelGGL(k_deep_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->cell_count, c->scan_tmp, m,
+                           c->ev_off);
+        int32_t nd = 0;
+        FLIP_CHECK(hipMemcpyAsync(&nd, c->scan_tmp + c->C, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        if (nd > 0) {
+            int take = std::min(nd, m);
+            std::vector<int32_t> deep(take), waters(m);
+            FLIP_CHECK(hipMemcpy(deep.data(), c->ev_off, sizeof(int32_t) * take, hipMemcpyDeviceToHost));
+            for (int t = 0; t < m; ++t) waters[t] = deep[t % take];
+            // libstdc++ random_shuffle: swap(a[i], a[rand() % (i+1)]) for i >= 1
+            for (int i = 1; i < m; ++i) {
+                int j = c->rng.next() % (i + 1);
+                if (i != j) std::swap(waters[i], waters[j]);
+            }
+            float h = (float)(1.0 / n);
+            std::vector<float> xyz(3 * (size_t)m);
+            for (int t = 0; t < m; ++t) {
+                int wi = waters[t] / (n * n), wj = (waters[t] / n) % n, wk = waters[t] % n;
+                int r0 = c->rng.next() % 101;
+                xyz[3 * t + 0] = (float)((double)h * (wi + 0.25 + 0.5 * r0 / 100));
+                int r1 = c->rng.next() % 101;
+                xyz[3 * t + 1] = (float)((double)h * (wj + 0.25 + 0.5 * r1 / 100));
+                int r2 = c->rng.next() % 101;
+                xyz[3 * t + 2] = (float)((double)h * (wk + 0.25 + 0.5 * r2 / 100));
+            }
+            FLIP_CHECK(hipMemcpyAsync(c->ev_list, ids.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
+            FLIP_CHECK(hipMemcpyAsync(c->ev_rand, xyz.data(), sizeof(float) * 3 * m, hipMemcpyHostToDevice,
+                                      c->stream));
+            hipLaunchKernelGGL(k_set_pos_by_id, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_rand, c->inv,
+                               c->P4());
+            k_launch_sort(c);
+            t_begin(c, "reposition_resample");
+            hipLaunchKernelGGL(k_resample_list, 1, TB, 0, c->stream, m, c->ev_list, c->inv, n, c->P4(), c->V4(),
+                               c->cell_start, h * h);
+            t_end(c);
+            FLIP_CHECK(hipStreamSynchronize(c->stream));
+            moved = m;
+            return moved;
+        }
+    }
+    k_launch_sort(c);
+    return moved;
+}
+
+// corrector::correct (corrector.cpp:49-115); particles must be sorted.
+int k_correct(FlipCtx *c)
+{
+    if (c->np == 0) return 0;
+    int n = c->n;
+    float dt = (float)c->dt;
+    float re = (float)(c->prm.density / n);
+    t_begin(c, "spring");
+    FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_spring, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->M(), c->nrm_id,
+                       c->cell_start, dt, re, c->tmp0, c->ev_list, c->ev_count);
+    t_end(c);
+    FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    FLIP_CHECK(hipStreamSynchronize(c->stream));
+    int m = c->h_pinned[0];
+    int events = 0;
+    if (m > 0) {
+        // order the listed particles by creation index, count their events, and
+        // draw 3 rand()%101 per event in that order (corrector.cpp:74-76 at one thread)
+        std::vector<int32_t> pairs(2 * (size_t)m), slots(m), ev(m);
+        FLIP_CHECK(hipMemcpy(pairs.data(), c->ev_list, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost));
+        std::vector<std::pair<int32_t, int32_t>> byid(m);
+        for (int t = 0; t < m; ++t) byid[t] = {pairs[2 * t], pairs[2 * t + 1]};
+        std::sort(byid.begin(), byid.end());
+        for (int t = 0; t < m; ++t) slots[t] = byid[t].second;
+        FLIP_CHECK(hipMemcpy(c->ev_list, slots.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_spring_count, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, n, c->P4(), c->cell_start,
+                           re, c->ev_off);
+        FLIP_CHECK(hipMemcpyAsync(ev.data(), c->ev_off, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        std::vector<int32_t> off(m);
+        for (int t = 0; t < m; ++t) {
+            off[t] = 3 * events;
+            events += ev[t];
+        }
+        std::vector<float> rv(3 * (size_t)events + 1);
+        for (int e = 0; e < 3 * events; ++e) rv[e] = (float)(c->rng.next() % 101);
+        if ((int64_t)rv.size() > c->ev_cap * 3) throw flip_arg_error{FLIP_E_STATE, "too many corrector jitter events"};
+        FLIP_CHECK(hipMemcpy(c->ev_rand, rv.data(), sizeof(float) * rv.size(), hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpy(c->ev_off, off.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_spring_fix, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_off, n, c->P4(), c->M(),
+                           c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
+    }
+    t_begin(c, "resample");
+    hipLaunchKernelGGL(k_resample, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->V4(), c->cell_start,
+                       c->tmp0, c->tmp1, re * re);
+    t_end(c);
+    t_begin(c, "correct_update");
+    hipLaunchKernelGGL(k_correct_update, nblk(c->np, TB), TB, 0, c->stream, c->np, c->P4(), c->V4(), c->tmp0,
+                       c->tmp1);
+    t_end(c);
+    return events;
+}

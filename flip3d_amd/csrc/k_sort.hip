@@ -1,0 +1,212 @@
+// k_sort.hip - cell binning (sorter::sort, src/sorter.cpp:22-40) as a counting
+// sort, plus the exclusive scan used by the sort and by ordered compactions.
+//
+// The reference keeps, per cell, the particles in particle-vector (creation)
+// order.  Here particles are stored cell-sorted; every sort
+//   1. computes each particle's cell key and claims a slot with an atomic
+//      counter (arbitrary order inside the cell),
+//   2. scans the per-cell counts into cell_start,
+//   3. scatters particle indices to cell_start[key] + slot,
+//   4. restores creation order inside each cell (insertion sort by index;
+//      cells hold ~8 particles), and
+//   5. permutes all particle arrays into the new order (coalesced gather).
+// The result is bit-identical to the reference's buckets.
+#include "flip_device.hpp"
+#include "flip_internal.hpp"
+
+using namespace flipdev;
+
+namespace {
+
+constexpr int SB = 256;     // threads per block
+constexpr int SI = 8;       // items per thread in the scan
+constexpr int TILE = SB * SI;
+
+__global__ void k_fill_i32(int32_t *a, long n, int32_t v)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of per-thread values over the block; returns the prefix and
+// writes the block total to *total.
+__device__ __forceinline__ int block_excl_scan(int v, int *lds, int *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = wave_incl_scan(v);
+    if (lane == 63) lds[w] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < SB / 64; ++i) {
+        if (i < w) off += lds[i];
+        tot += lds[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+__global__ __launch_bounds__(SB) void k_scan_reduce(const int32_t *__restrict__ in, long n, int32_t *bsum)
+{
+    __shared__ int lds[SB / 64];
+    long base = (long)blockIdx.x * TILE;
+    int s = 0;
+#pragma unroll
+    for (int t = 0; t < SI; ++t) {
+        long i = base + (long)t * SB + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    int tot;
+    (void)block_excl_scan(s, lds, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of nb block sums in place; out_total <- sum
+__global__ __launch_bounds__(SB) void k_scan_top(int32_t *bsum, int nb, int32_t *out_total)
+{
+    __shared__ int lds[SB / 64];
+    int carry = 0;
+    for (int base = 0; base < nb; base += SB) {
+        int i = base + threadIdx.x;
+        int v = i < nb ? bsum[i] : 0;
+        int tot;
+        int ex = block_excl_scan(v, lds, &tot);
+        if (i < nb) bsum[i] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(SB) void k_scan_apply(const int32_t *__restrict__ in, long n,
+                                                   const int32_t *__restrict__ bsum, int32_t *out)
+{
+    __shared__ int lds[SB / 64];
+    long base = (long)blockIdx.x * TILE + (long)threadIdx.x * SI;
+    int v[SI];
+    int s = 0;
+#pragma unroll
+    for (int t = 0; t < SI; ++t) {
+        v[t] = base + t < n ? in[base + t] : 0;
+        s += v[t];
+    }
+    int tot;
+    int ex = block_excl_scan(s, lds, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int t = 0; t < SI; ++t) {
+        if (base + t < n) out[base + t] = ex;
+        ex += v[t];
+    }
+}
+
+// 1. cell key and slot claim
+__global__ void k_sort_count(const float4 *__restrict__ pos, int64_t np, int n, int32_t *key, int32_t *slot,
+                             int32_t *count)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    int32_t c = (int32_t)cidx(n, cell_of(n, p.x), cell_of(n, p.y), cell_of(n, p.z));
+    key[i] = c;
+    slot[i] = atomicAdd(&count[c], 1);
+}
+
+// 3. scatter source indices to their bucket slots
+__global__ void k_sort_scatter(int64_t np, const int32_t *__restrict__ key, const int32_t *__restrict__ slot,
+                               const int32_t *__restrict__ start, const int32_t *__restrict__ meta, int32_t *perm,
+                               int32_t *pid)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    int32_t d = start[key[i]] + slot[i];
+    perm[d] = (int32_t)i;
+    pid[d] = meta_id(meta[i]);
+}
+
+// 4. creation order inside every bucket
+__global__ void k_sort_order(long C, const int32_t *__restrict__ start, int32_t *perm, int32_t *pid)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    int b = start[c], e = start[c + 1];
+    for (int a = b + 1; a < e; ++a) {
+        int32_t kid = pid[a], kp = perm[a];
+        int t = a - 1;
+        while (t >= b && pid[t] > kid) {
+            pid[t + 1] = pid[t];
+            perm[t + 1] = perm[t];
+            --t;
+        }
+        pid[t + 1] = kid;
+        perm[t + 1] = kp;
+    }
+}
+
+// 5. permute particle state
+__global__ void k_sort_gather(int64_t np, const int32_t *__restrict__ perm, const float4 *__restrict__ pin,
+                              const float4 *__restrict__ vin, const int32_t *__restrict__ min, float4 *pout,
+                              float4 *vout, int32_t *mout, int32_t *inv)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= np) return;
+    int32_t s = perm[j];
+    pout[j] = pin[s];
+    vout[j] = vin[s];
+    int32_t m = min[s];
+    mout[j] = m;
+    inv[meta_id(m)] = (int32_t)j;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n)
+{
+    if (n <= 0) {
+        hipLaunchKernelGGL(k_fill_i32, 1, 1, 0, c->stream, out, 1L, 0);
+        return;
+    }
+    int nb = (int)((n + TILE - 1) / TILE);
+    hipLaunchKernelGGL(k_scan_reduce, nb, SB, 0, c->stream, in, n, c->scan_blocks);
+    hipLaunchKernelGGL(k_scan_top, 1, SB, 0, c->stream, c->scan_blocks, nb, out + n);
+    hipLaunchKernelGGL(k_scan_apply, nb, SB, 0, c->stream, in, n, c->scan_blocks, out);
+}
+
+void k_launch_sort(FlipCtx *c)
+{
+    int64_t np = c->np;
+    t_begin(c, "sort_count");
+    hipLaunchKernelGGL(k_fill_i32, nblk(c->C, 256), 256, 0, c->stream, c->cell_count, c->C, 0);
+    if (np > 0)
+        hipLaunchKernelGGL(k_sort_count, nblk(np, 256), 256, 0, c->stream, c->P4(), np, c->n, c->pkey, c->pslot,
+                           c->cell_count);
+    t_end(c);
+    t_begin(c, "sort_scan");
+    k_launch_scan(c, c->cell_count, c->cell_start, c->C);
+    t_end(c);
+    if (np == 0) return;
+    t_begin(c, "sort_scatter");
+    int32_t *pid = c->deep;  // scratch sized max(C, np) + 1: creation index per bucket slot
+    hipLaunchKernelGGL(k_sort_scatter, nblk(np, 256), 256, 0, c->stream, np, c->pkey, c->pslot, c->cell_start,
+                       c->M(), c->perm, pid);
+    hipLaunchKernelGGL(k_sort_order, nblk(c->C, 256), 256, 0, c->stream, c->C, c->cell_start, c->perm, pid);
+    t_end(c);
+    t_begin(c, "sort_gather");
+    int nx = c->cur ^ 1;
+    hipLaunchKernelGGL(k_sort_gather, nblk(np, 256), 256, 0, c->stream, np, c->perm, c->pos[c->cur],
+                       c->vel[c->cur], c->meta[c->cur], c->pos[nx], c->vel[nx], c->meta[nx], c->inv);
+    c->cur = nx;
+    t_end(c);
+}

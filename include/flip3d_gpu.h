@@ -1,0 +1,188 @@
+/*
+ * flip3d_gpu.h - C ABI of the MI355X (gfx950) FLIP step library.
+ *
+ * Drop-in boundary for the per-step path of mmmovania/flip3d.  The reference
+ * exposes plain C++ namespace functions (src/flip3D.h:9-12, src/mapper.h:11-14,
+ * src/solver.h:9-12, src/sorter.h:11-29, src/corrector.h:10-13); each entry
+ * below names the reference interface it replaces.  All state is
+ * device-resident behind an opaque context; calls are synchronous at return,
+ * like the reference's.  Every function returns 0 on success or a negative
+ * FLIP_E* code, with a message from flip_gpu_last_error(); nothing throws.
+ *
+ * Layout conventions (identical to the reference's alloc3D<T>(w,h,d) jagged
+ * arrays flattened in [i][j][k] order, k fastest, src/utility.h:40-51):
+ *   cell grids  A (int8), L, press, div : n*n*n
+ *   face grids  u0 : (n+1)*n*n, u1 : n*(n+1)*n, u2 : n*n*(n+1)
+ * Particles cross the boundary as structure-of-arrays in the reference's
+ * particle-vector order (src/flip3D.cpp:51); positions, velocities and
+ * normals are interleaved xyz triples.
+ */
+#ifndef FLIP3D_GPU_H
+#define FLIP3D_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLIP_OK 0
+#define FLIP_E_ARG (-1)      /* bad argument */
+#define FLIP_E_HIP (-2)      /* HIP runtime error (no device, out of memory, launch) */
+#define FLIP_E_STATE (-3)    /* call out of order (e.g. step before upload) */
+#define FLIP_E_NUMERIC (-4)  /* solver produced NaN/Inf or hit its iteration cap */
+
+/* cell flags, src/common.h:11-13 */
+#define FLIP_AIR 0
+#define FLIP_FLUID 1
+#define FLIP_WALL 2
+
+/* grid ids for flip_gpu_get_grid / flip_gpu_set_grid */
+#define FLIP_GRID_U0 0
+#define FLIP_GRID_U1 1
+#define FLIP_GRID_U2 2
+#define FLIP_GRID_US0 3 /* u_save (flip3D.cpp:46): raw P2G, then the FLIP delta */
+#define FLIP_GRID_US1 4
+#define FLIP_GRID_US2 5
+#define FLIP_GRID_A 6
+#define FLIP_GRID_L 7
+#define FLIP_GRID_PRESS 8
+#define FLIP_GRID_DIV 9
+
+/* preconditioners */
+#define FLIP_PRECOND_REFERENCE 0 /* solver.cpp:119-180 as written (value-test P_ref) */
+#define FLIP_PRECOND_IC0 1       /* same factorisation with couplings keyed on flags */
+#define FLIP_PRECOND_LEARNED 2   /* U-Net warm start + flexible PCG (reference precond inside) */
+
+/* Step parameters.  flip_gpu_default_params fills the reference's compile-time
+ * defaults (src/flip3D.cpp:25-39, mapper.cpp:14, corrector.cpp:13,
+ * solver.cpp:16). */
+typedef struct flip_params {
+    int n;              /* grid resolution N */
+    double dt;          /* DT, or the cap when dt_policy == 1 */
+    int dt_policy;      /* 0 fixed dt; 1 dt = min(dt, cfl / (N max|u_fluid|)) */
+    double cfl;         /* 0.6 */
+    double alpha;       /* ALPHA, FLIP/PIC blend 0.95 */
+    double gravity;     /* GRAVITY 9.8 */
+    double density;     /* DENSITY 0.5 (2x2x2 particles per cell) */
+    int subcell;        /* SUBCELL ghost-fluid boundary, 1 */
+    int tol_policy;     /* 0 reference stop sum r^2 <= 1e-2 n^3; 1 ||r|| <= tol ||b|| */
+    double tol;         /* relative tolerance for tol_policy 1 */
+    int precond;        /* FLIP_PRECOND_* */
+    int max_iters;      /* <= 0: n^3 as in solver.cpp:326 */
+    int correction;     /* !DISABLE_CORRECTION */
+    int reposition;     /* stuck-particle repositioning, flip3D.cpp:932-968 */
+    unsigned rand_seed; /* glibc-compatible rand() stream seed; the reference's is 1 */
+    int refine;         /* > 0: FP64-residual iterative refinement sweeps (config N=256) */
+    double refine_tol;  /* outer relative tolerance for refine > 0 */
+} flip_params;
+
+typedef struct flip_step_stats {
+    int iterations;      /* PCG iterations of this step's solve */
+    double error2;       /* final sum r^2 over fluid cells */
+    double bnorm2;       /* sum b^2 over fluid cells */
+    double error2_0;     /* initial sum r^2 */
+    double dt;           /* dt used */
+    int n_fluid_cells;
+    int n_reposition;    /* particles repositioned (flip3D.cpp:405-451) */
+    int n_jitter;        /* corrector random-jitter events (corrector.cpp:73-77) */
+    int n_precond_unit;  /* cells with P == 1.0, which enable value-test couplings */
+    int refine_sweeps;   /* outer refinement sweeps used */
+    double ms_step;      /* device time of the step, HIP events */
+    double ms_solve;     /* device time of the pressure solve */
+} flip_step_stats;
+
+typedef struct flip_ctx flip_ctx;
+
+const char *flip_gpu_last_error(void);
+int flip_gpu_default_params(flip_params *p, int n);
+
+/* Context lifetime.  Replaces flip3D::init's allocation half
+ * (src/flip3D.cpp:649-683) and the static solver scratch (solver.cpp:226). */
+int flip_gpu_create(const flip_params *p, flip_ctx **out);
+int flip_gpu_destroy(flip_ctx *ctx);
+int flip_gpu_synchronize(flip_ctx *ctx);
+
+/* Particle state in reference order.  pos/vel/nrm: 3*np floats; type: FLIP_FLUID
+ * or FLIP_WALL; mass, dens may be NULL (1.0, 10.0 as pushParticle sets,
+ * flip3D.cpp:641-644).  Replaces the loadState particle read (flip3D.cpp:540-547). */
+int flip_gpu_upload_particles(flip_ctx *ctx, int64_t np, const float *pos, const float *vel,
+                              const float *nrm, const int8_t *type, const float *mass,
+                              const float *dens);
+int flip_gpu_download_particles(flip_ctx *ctx, float *pos, float *vel, float *nrm, int8_t *type,
+                                float *dens);
+int flip_gpu_num_particles(flip_ctx *ctx, int64_t *np);
+int flip_gpu_set_max_dens(flip_ctx *ctx, float max_dens); /* flip3D.cpp:58 */
+int flip_gpu_get_max_dens(flip_ctx *ctx, float *max_dens);
+int flip_gpu_srand(flip_ctx *ctx, unsigned seed);          /* stdlib srand */
+int flip_gpu_set_dt(flip_ctx *ctx, double dt);             /* dt for stage calls */
+
+/* Grids (see FLIP_GRID_*): A is int8, all others float. */
+int flip_gpu_get_grid(flip_ctx *ctx, int which, void *out);
+int flip_gpu_set_grid(flip_ctx *ctx, int which, const void *in);
+
+/* One simulation step.  Replaces flip3D::simulateStep (flip3D.cpp:1194-1264),
+ * its timed region :1216-:1247 (no preview, mesh or checkpoint I/O). */
+int flip_gpu_step(flip_ctx *ctx, flip_step_stats *stats);
+
+/* Stages of the step, in order, for parity harnesses:
+ *   DENSITY    sorter::sort + computeDensity       flip3D.cpp:1221-1222
+ *   EXT_FORCE  add_ExtForce                        flip3D.cpp:1230
+ *   P2G        sort + mapper::mapP2G + markWater + save_grid + enforce_boundary
+ *                                                  flip3D.cpp:1059-1065
+ *   PROJECT    project + enforce_boundary          flip3D.cpp:1066-1067
+ *   EXTRAP     extrapolate_velocity + subtract_grid flip3D.cpp:1068-1069
+ *   G2P        FLIP/PIC blend via mapper::mapG2P   flip3D.cpp:1071-1097
+ *   ADVECT     advect_particle (push-out, stuck, reposition) flip3D.cpp:877-969
+ *   CORRECT    corrector::correct                  corrector.cpp:49-115 */
+#define FLIP_STAGE_DENSITY 0
+#define FLIP_STAGE_EXT_FORCE 1
+#define FLIP_STAGE_P2G 2
+#define FLIP_STAGE_PROJECT 3
+#define FLIP_STAGE_EXTRAP 4
+#define FLIP_STAGE_G2P 5
+#define FLIP_STAGE_ADVECT 6
+#define FLIP_STAGE_CORRECT 7
+int flip_gpu_stage(flip_ctx *ctx, int stage, flip_step_stats *stats);
+
+/* mapper::mapP2G (mapper.cpp:17-88) on the context's particles: sorts, then
+ * writes the three face grids (u0,u1,u2 layouts above); any pointer may be NULL. */
+int flip_gpu_map_p2g(flip_ctx *ctx, float *u0, float *u1, float *u2);
+/* mapper::mapG2P / fetchVelocity (mapper.cpp:90-112): trilinear samples of the
+ * given face grids at np positions. */
+int flip_gpu_map_g2p(flip_ctx *ctx, const float *u0, const float *u1, const float *u2, int64_t np,
+                     const float *pos, float *vel);
+/* solver::setSubcell + solver::solve (solver.cpp:220-241) on caller grids of the
+ * context's size n: x is the warm start and the result; b is negated in place as
+ * the reference does (solver.cpp:231). */
+int flip_gpu_solve(flip_ctx *ctx, const int8_t *A, const float *L, float *x, float *b, int subcell,
+                   flip_step_stats *stats);
+/* sorter::sort (sorter.cpp:22-40) result in reference bucket order: counts per
+ * cell (n^3) and particle indices (reference order) cell by cell (np). */
+int flip_gpu_sorted_cells(flip_ctx *ctx, int32_t *counts, int32_t *ids);
+
+/* Per-kernel device timings of the last step (ms), in launch-name order;
+ * names is filled with a ';'-separated list.  Returns the count. */
+int flip_gpu_kernel_times(flip_ctx *ctx, char *names, int names_cap, double *ms, int ms_cap);
+/* Enable/disable per-kernel event timing (adds events around each launch). */
+int flip_gpu_set_profiling(flip_ctx *ctx, int on);
+
+/* Scene generation (host, once; flip3D::init placement, flip3D.cpp:685-764).
+ * scene 0: the reference damBreakTest with rand() jitter (consumes the
+ * context's rand stream); scene 1: the benchmark scene (box x<0.4,y<0.6 plus
+ * sphere r=0.12 at (0.6,0.7,0.5), SplitMix64 jitter U(+-0.25dx)).  Uploads the
+ * particles and max_dens into the context; returns the particle count via np. */
+#define FLIP_SCENE_REFERENCE 0
+#define FLIP_SCENE_BENCHMARK 1
+int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np);
+
+/* Device math self-check for the ops whose rounding the parity argument
+ * relies on: out[4*i+0] = x/y, +1 = sqrtf(x), +2 = (float)sqrt((double)x*x+(double)y*y),
+ * +3 = (float)((double)x / (double)y). */
+int flip_gpu_selftest_math(int64_t m, const float *x, const float *y, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
