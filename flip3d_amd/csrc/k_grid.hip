@@ -127,7 +127,7 @@ void k_launch_divergence(FlipCtx *c)
 {
     t_begin(c, "divergence");
     hipLaunchKernelGGL(k_divergence, nblk(c->C, TB), TB, 0, c->stream, c->n, c->A, c->ug[0], c->ug[1], c->ug[2],
-                       (float)(1.0 / c->n), c->div, c->press, 1);
+                       (float)(1.0 / c->n), c->div, c->press, 0);
     t_end(c);
 }
 

@@ -318,6 +318,10 @@ __global__ void k_stuck(int64_t np, int n, const float4 *__restrict__ pos, const
     int ii = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.x * g)));
     int jj = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.y * g)));
     int kk = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.z * g)));
+    // for n < 5 the reference indexes out of range; keep the window inside
+    ii = min(max(ii, 0), n - 1);
+    jj = min(max(jj, 0), n - 1);
+    kk = min(max(kk, 0), n - 1);
     int jm = jj - 1 > 0 ? jj - 1 : 0, jp = jj + 1 < n - 1 ? jj + 1 : n - 1;
     if ((double)vel[i].w < 0.04 && (A[cidx(n, ii, jm, kk)] == WALL || A[cidx(n, ii, jp, kk)] == WALL)) rem = true;
     if (rem) {

@@ -191,10 +191,15 @@ __global__ void k_negate(long C, float *b)
     if (c < C) b[c] = -b[c];
 }
 
-__global__ void k_fluid_flag(long C, const int8_t *__restrict__ A, int32_t *f)
+// FLUID flags for the compaction; also zeroes x on non-FLUID cells, which the
+// reference's first op(x) does (solver.cpp:205-212) and nothing reads earlier
+__global__ void k_fluid_flag(long C, const int8_t *__restrict__ A, int32_t *f, float *x)
 {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < C) f[c] = A[c] == FLUID ? 1 : 0;
+    if (c >= C) return;
+    bool fl = A[c] == FLUID;
+    f[c] = fl ? 1 : 0;
+    if (!fl && x) x[c] = 0.0f;
 }
 
 __global__ void k_fluid_compact(long C, const int8_t *__restrict__ A, const int32_t *__restrict__ at, int32_t *fl)
@@ -488,7 +493,7 @@ __global__ void k_fill_f64(long n, double *a, double v)
 // ordered list of FLUID cells; returns the count (host sync)
 int k_build_fluid_list(FlipCtx *c)
 {
-    hipLaunchKernelGGL(k_fluid_flag, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->cell_count);
+    hipLaunchKernelGGL(k_fluid_flag, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->cell_count, c->press);
     k_launch_scan(c, c->cell_count, c->scan_tmp, c->C);
     hipLaunchKernelGGL(k_fluid_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->scan_tmp, c->fl);
     FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->scan_tmp + c->C, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));

@@ -675,7 +675,12 @@ static void conj_grad(const orc_params *prm, const int8_t *A, const real *L, rea
     int it = 0;
     double err = error2_0;
     if (st) { st->error2_0 = error2_0; st->bnorm2 = bnorm2; }
-    if (prm->tol_policy == 1 && error2_0 <= eps) cap = 0;
+    if (prm->tol_policy == 1 && error2_0 <= eps) {
+        /* no iteration: apply the masking of the loop's first op(x) anyway */
+        cap = 0;
+        for (long c = 0; c < C; ++c)
+            if (A[c] != FLUID) x[c] = 0.0;
+    }
     for (long k = 0; k < cap; ++k) {
         compute_ax(prm, A, L, s, z);
         double alpha = a / product(n, A, z, s);
