@@ -138,6 +138,8 @@ void alloc_grids(FlipCtx *c)
     FLIP_CHECK(hipMemset(c->sv[0], 0, c->C * sizeof(float)));
     FLIP_CHECK(hipMemset(c->sv[1], 0, c->C * sizeof(float)));
     FLIP_CHECK(hipMemset(c->z, 0, c->C * sizeof(float)));
+    dalloc(&c->pcg_part, 8 * 512);
+    dalloc(&c->qv, c->C);
     dalloc(&c->scal, 32);
     dalloc(&c->iflags, 16);
     dalloc(&c->tickets, 8);
@@ -168,6 +170,8 @@ void free_all(FlipCtx *c)
     dfree(c->q);
     dfree(c->sv[0]);
     dfree(c->sv[1]);
+    dfree(c->pcg_part);
+    dfree(c->qv);
     dfree(c->scal);
     dfree(c->iflags);
     dfree(c->tickets);

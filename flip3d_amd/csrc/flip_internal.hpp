@@ -72,6 +72,8 @@ struct FlipCtx {
     double *P = nullptr;
     float *r = nullptr, *z = nullptr, *q = nullptr, *sv[2] = {nullptr, nullptr};
     double *partials = nullptr;  // per-block partial sums
+    double *pcg_part = nullptr;  // PCG per-block partials (8 * 512)
+    double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
     double *scal = nullptr;      // device scalars (see pcg kernels)
     int32_t *iflags = nullptr;   // device int flags / counters
     unsigned *tickets = nullptr;

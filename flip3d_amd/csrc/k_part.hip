@@ -134,13 +134,19 @@ __global__ __launch_bounds__(TB) void k_p2g(int n, const float4 *__restrict__ sp
     float fx = COMP == 0 ? (float)i : (float)(i + 0.5);
     float fy = COMP == 1 ? (float)j : (float)(j + 0.5);
     float fz = COMP == 2 ? (float)k : (float)(k + 0.5);
-    const int wi = COMP == 0 ? 1 : 2, wj = COMP == 1 ? 1 : 2, wk = COMP == 2 ? 1 : 2;
-    int klo = max(k - wk, 0), khi = min(k + wk - 1, n - 1);
+    // The reference window is [i-2, i+1] on the two transverse axes, but a
+    // particle in the first row (cell i-2, clamped coordinate < i-1) is at
+    // least 1.5 from the face centre i+0.5, so r2 >= 2.25 > RE^2 and its weight
+    // is exactly 0: those rows are skipped.  Along the normal axis the window
+    // is [i-1, i] as in the reference.
+    const int ilo_off = COMP == 0 ? 1 : 1, jlo_off = 1, klo_off = 1;
+    const int ihi_off = COMP == 0 ? 0 : 1, jhi_off = COMP == 1 ? 0 : 1, khi_off = COMP == 2 ? 0 : 1;
+    int klo = max(k - klo_off, 0), khi = min(k + khi_off, n - 1);
     float sumw = 0.0f, sumx = 0.0f;
     if (klo <= khi) {
-        for (int si = i - wi; si <= i + wi - 1; ++si) {
+        for (int si = i - ilo_off; si <= i + ihi_off; ++si) {
             if (si < 0 || si > n - 1) continue;
-            for (int sj = j - wj; sj <= j + wj - 1; ++sj) {
+            for (int sj = j - jlo_off; sj <= j + jhi_off; ++sj) {
                 if (sj < 0 || sj > n - 1) continue;
                 int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
                 for (int a = b; a < e; ++a) {
@@ -372,15 +378,73 @@ __global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const fl
 // re = h = 1/N (corrector.cpp:15-47), u starting from 0.  One block: weights
 // are evaluated in parallel, summed serially in window order, and each
 // particle's result is visible to the next (the reference's loop order).
+// A listed particle with no other listed particle in its 3x3x3 window reads
+// no velocity another listed particle writes, and (the window relation being
+// symmetric) none reads its own, so those resample in parallel here, one
+// thread each with the reference's serial sum; dep[t] marks the others, which
+// k_resample_list then does in list order.
+__global__ __launch_bounds__(TB) void k_resample_list_indep(int m, const int32_t *__restrict__ ids,
+                                                            const int32_t *__restrict__ inv, int n,
+                                                            const float4 *__restrict__ pos, float4 *vel,
+                                                            const int32_t *__restrict__ cs, float hh, int32_t *dep)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    int32_t slot = inv[ids[t]];
+    float4 p = pos[slot];
+    int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+    bool d = false;
+    for (int u = 0; u < m && !d; ++u) {
+        if (u == t) continue;
+        float4 q = pos[inv[ids[u]]];
+        int qi = cell_of(n, q.x), qj = cell_of(n, q.y), qk = cell_of(n, q.z);
+        d = abs(qi - ci) <= 1 && abs(qj - cj) <= 1 && abs(qk - ck) <= 1;
+    }
+    dep[t] = d ? 1 : 0;
+    if (d) return;
+    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            for (int a = b; a < e; ++a) {
+                float4 q = pos[a];
+                if (is_wall_w(q.w)) continue;
+                float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
+                if (!(d2 < hh)) continue;
+                float w = q.w * sharp_kernel_hh(d2, hh);
+                float4 u = vel[a];
+                ax += w * u.x;
+                ay += w * u.y;
+                az += w * u.z;
+                wsum += w;
+            }
+        }
+    }
+    float4 v = vel[slot];
+    if (wsum != 0.0f) {
+        v.x = ax / wsum;
+        v.y = ay / wsum;
+        v.z = az / wsum;
+    } else {
+        v.x = v.y = v.z = 0.0f;
+    }
+    vel[slot] = v;
+}
+
 constexpr int RCH = 1024;
 __global__ __launch_bounds__(TB) void k_resample_list(int m, const int32_t *__restrict__ ids, const int32_t *inv,
                                                       int n, const float4 *__restrict__ pos, float4 *vel,
-                                                      const int32_t *__restrict__ cs, float hh)
+                                                      const int32_t *__restrict__ cs, float hh,
+                                                      const int32_t *__restrict__ dep)
 {
     __shared__ float sw[RCH], sx[RCH], sy[RCH], sz[RCH];
     __shared__ int rb[9], re_[9];
     __shared__ float acc[4];
     for (int t = 0; t < m; ++t) {
+        if (!dep[t]) continue;
         int32_t slot = inv[ids[t]];
         float4 p = pos[slot];
         int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
@@ -485,6 +549,11 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
     const double rre = 0.1 * (double)re;
     const double cj_ = 0.01 * (double)re / (double)dt;
     const double cw = 0.05 * (double)re / (double)dt;
+    // A neighbour at distance >= re has smooth weight exactly 0 and lies on the
+    // dist > 0.1 re branch, so it adds a signed zero: skipped.  The float
+    // squared distance is within a few ulp of dist*dist, so the 1e-4 margin
+    // keeps the test conservative.
+    const float far2 = re * re * 1.0001f;
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
@@ -494,6 +563,7 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
             for (int a = b; a < e; ++a) {
                 if (a == i) continue;
                 float4 q = pos[a];
+                if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) continue;
                 float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
                 float w = (float)(50.0 * (double)mass_of(q.w) * (double)smooth_kernel(dist * dist, re));
                 if ((double)dist > rre) {
@@ -554,6 +624,7 @@ __global__ void k_spring_count(int m, const int32_t *__restrict__ list, int n, c
     int ci = (int)x, cj = (int)y, ck = (int)z;
     int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
     const double rre = 0.1 * (double)re;
+    const float far2 = re * re * 1.0001f;  // as in spring_one
     int ev = 0;
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
@@ -563,6 +634,7 @@ __global__ void k_spring_count(int m, const int32_t *__restrict__ list, int n, c
             for (int a = b; a < e; ++a) {
                 if (a == i) continue;
                 float4 q = pos[a];
+                if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) continue;
                 float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
                 if (!((double)dist > rre) && !is_wall_w(q.w)) ev++;
             }
@@ -632,8 +704,10 @@ __global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const flo
     vel[i] = make_float4(b.x, b.y, b.z, v.w);
 }
 
-__global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float4 *__restrict__ vel, double *out,
-                         unsigned *ticket, double *partials)
+// max |u|^2 over FLUID particles: non-negative doubles order like their bit
+// patterns, so a 64-bit atomicMax of the bits is exact and order-free
+__global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                         unsigned long long *out)
 {
     __shared__ double lds[TB / 64];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -648,15 +722,7 @@ __global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float
     if (threadIdx.x == 0) {
         double r = 0.0;
         for (int w = 0; w < TB / 64; ++w) r = fmax(r, lds[w]);
-        partials[blockIdx.x] = r;
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double mm = 0.0;
-            for (unsigned b = 0; b < gridDim.x; ++b) mm = fmax(mm, ((volatile double *)partials)[b]);
-            *out = mm;
-            *ticket = 0;
-        }
+        if (r == r) atomicMax(out, (unsigned long long)__double_as_longlong(r));
     }
 }
 
@@ -754,8 +820,9 @@ double k_max_speed(FlipCtx *c)
 {
     if (c->np == 0) return 0.0;
     unsigned nb = nblk(c->np, TB);
-    hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), c->scal + 15, c->tickets + 3,
-                       c->partials);
+    unsigned long long *out = (unsigned long long *)(c->scal + 15);
+    FLIP_CHECK(hipMemsetAsync(out, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), out);
     double h = 0.0;
     FLIP_CHECK(hipMemcpyAsync(&h, c->scal + 15, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FLIP_CHECK(hipStreamSynchronize(c->stream));
@@ -823,8 +890,11 @@ int k_stuck_and_reposition(FlipCtx *c)
                                c->P4());
             k_launch_sort(c);
             t_begin(c, "reposition_resample");
+            int32_t *dep = c->ev_off;  // free again after the deep-cell list was read
+            hipLaunchKernelGGL(k_resample_list_indep, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->inv, n,
+                               c->P4(), c->V4(), c->cell_start, h * h, dep);
             hipLaunchKernelGGL(k_resample_list, 1, TB, 0, c->stream, m, c->ev_list, c->inv, n, c->P4(), c->V4(),
-                               c->cell_start, h * h);
+                               c->cell_start, h * h, dep);
             t_end(c);
             FLIP_CHECK(hipStreamSynchronize(c->stream));
             moved = m;

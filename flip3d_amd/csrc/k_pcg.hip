@@ -5,21 +5,25 @@
 // ghost-fluid fetch x_ref (solver.cpp:19-26) never reads a non-FLUID entry,
 // so only FLUID cells carry information.
 //
-// One PCG iteration (solver.cpp:326-344) is two kernels:
-//   pcg_a: s = z + (float)beta*s (recomputed for the 6 neighbours, ping-pong
-//          buffers), q = A s, partial sums of (float)(q*s); the last block to
-//          finish forms alpha = a / (q.s).
-//   pcg_b: x += (float)alpha*s; r += (float)(-alpha)*q; partial (float)(r*r);
-//          z = M r; partial (float)(z*r); the last block applies the stop
-//          test and forms beta = a2 / a.
-// Dot products accumulate the FLOAT products in double (solver.cpp:44-54) in a
-// fixed tree order, so results are deterministic run to run; they differ from
-// the reference's serial order only in the double's last bits, which the
-// float casts of alpha and beta absorb.
+// One PCG iteration k (solver.cpp:326-344) is two kernels over a grid of at
+// most GMAX blocks (grid-stride loops):
+//   pcg_a(k): every block sums the previous kernel's per-block partials of
+//             (float)(r*r) and (float)(z*r) in one fixed order (so all blocks
+//             agree bit for bit), applies the stop test, forms beta = a_k /
+//             a_{k-1}; then s = z + (float)beta*s (ping-pong buffers, the six
+//             neighbours recomputed), q = A s, partials of (float)(q*s).
+//   pcg_b(k): every block forms alpha = a_k / (q.s); x += (float)alpha*s;
+//             r += (float)(-alpha)*q; z = M r; partials of r.r and z.r.
+// Reductions cross kernel boundaries only, so no block needs a device-scope
+// fence (which on a multi-XCD part writes back the L2).  Products are FLOAT
+// and accumulated in double as in product() (solver.cpp:44-54); the fixed
+// tree order makes results deterministic run to run and differs from the
+// reference's serial order only in the double's last bits, which the float
+// casts of alpha and beta absorb.
 //
 // Preconditioner: with the reference's value test P_ref (solver.cpp:119-123)
 // every coupling term is a signed zero unless some P == 1.0 exactly, so M is
-// applied per cell; the signed zeros are kept so even the sign of a zero
+// applied per cell, keeping the signed zeros so even the sign of a zero
 // result matches.  If a P == 1.0 occurs (or for the flag-keyed IC(0) mode)
 // the triangular solves run as i+j+k hyperplane wavefronts; each cell's
 // formula and operand order are the reference's, so any topological order
@@ -35,71 +39,55 @@ using namespace flipdev;
 namespace {
 
 constexpr int TB = 256;
+constexpr int GMAX = 512;
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// device scalars
-enum { S_A = 0, S_ALPHA, S_BETA, S_ERR2, S_ERR2_0, S_BNORM2, S_EPS, S_ZS };
+// device scalars: a_k lives in S_A0 + (k & 1)
+enum { S_A0 = 0, S_A1, S_ALPHA, S_BETA, S_ERR2, S_ERR2_0, S_BNORM2, S_EPS };
 // device int flags
 enum { I_DONE = 0, I_ITERS, I_UNIT, I_CAPPED, I_TOLPOL, I_CAP };
 
-__device__ __forceinline__ double ld_coherent(const double *p)
+// Block-wide sum of one double per thread, fixed order, result in every thread.
+__device__ __forceinline__ double block_sum_all(double v)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ double lds[TB / 64];
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < TB / 64; ++i) r += lds[i];
+    return r;
 }
 
-// Two-sum last-block reduction.  Every block stores its partials; the last
-// block to take a ticket sums all partials in block order (fixed tree) and
-// returns true in every thread of that block with the totals in t0, t1.
-__device__ bool grid_reduce2(double v0, double v1, double *partials, unsigned *ticket, double &t0, double &t1)
+// Sum of cnt partials p[i*stride + off], identical in every block.
+__device__ __forceinline__ double sum_partials(const double *__restrict__ p, int cnt, int stride, int off)
 {
-    __shared__ double lds[2][TB / 64];
-    __shared__ bool last;
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-        lds[0][w] = v0;
-        lds[1][w] = v1;
-    }
-    __syncthreads();
+    double a = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += TB) a += p[i * stride + off];
+    return block_sum_all(a);
+}
+
+__device__ __forceinline__ void store_partials(double *p, int stride, double v0, double v1, double v2, int nv)
+{
+    double a = block_sum_all(v0);
+    double b = nv > 1 ? block_sum_all(v1) : 0.0;
+    double c = nv > 2 ? block_sum_all(v2) : 0.0;
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int i = 0; i < TB / 64; ++i) {
-            a += lds[0][i];
-            b += lds[1][i];
-        }
-        __hip_atomic_store(&partials[2 * blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[2 * blockIdx.x + 1], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
+        p[blockIdx.x * stride + 0] = a;
+        if (nv > 1) p[blockIdx.x * stride + 1] = b;
+        if (nv > 2) p[blockIdx.x * stride + 2] = c;
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double a = 0.0, b = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += TB) {
-        a += ld_coherent(&partials[2 * i]);
-        b += ld_coherent(&partials[2 * i + 1]);
-    }
-    a = wave_sum(a);
-    b = wave_sum(b);
-    __syncthreads();
-    if (lane == 0) {
-        lds[0][w] = a;
-        lds[1][w] = b;
-    }
-    __syncthreads();
-    a = 0.0;
-    b = 0.0;
-    for (int i = 0; i < TB / 64; ++i) {
-        a += lds[0][i];
-        b += lds[1][i];
-    }
-    t0 = a;
-    t1 = b;
-    if (threadIdx.x == 0) *ticket = 0;
-    return true;
+}
+
+// solver.cpp:331-337 stop test after `it` completed iterations
+__device__ __forceinline__ bool stop_test(double rr, int it, const int32_t *iflags, const double *scal, bool &capped)
+{
+    bool stop = iflags[I_TOLPOL] == 0 ? (double)(float)rr <= scal[S_EPS] : rr <= scal[S_EPS];
+    if (!(rr == rr)) stop = true;  // NaN: report instead of running to the cap
+    capped = !stop && it >= iflags[I_CAP];
+    return stop || capped;
 }
 
 struct Nb {
@@ -184,6 +172,7 @@ __device__ __forceinline__ float precond_cell(int n, long c, float r, double P, 
     return (float)(u * P);
 }
 
+
 // ---------------------------------------------------------------- setup
 __global__ void k_negate(long C, float *b)
 {
@@ -223,63 +212,83 @@ __global__ void k_build_diag(int nfl, const int32_t *__restrict__ fl, int n, con
     if (p == 1.0) atomicAdd(&iflags[I_UNIT], 1);
 }
 
-// r = b - A x (op with a = -1), z = M r, s = z; partial r.r, z.r, b.b
+// r = b - A x (op with a = -1), z = M r, s = z; partials r.r, z.r, b.b
 template <bool JACOBI>
 __global__ __launch_bounds__(TB) void k_pcg_init(int nfl, const int32_t *__restrict__ fl, int n,
                                                  const int8_t *__restrict__ A, const float *__restrict__ L,
                                                  int subcell, double h2, const float *__restrict__ x,
                                                  const float *__restrict__ b, const double *__restrict__ P, float *r,
-                                                 float *z, float *s, double *partials, unsigned *ticket,
-                                                 double *scal, double *bpartials, unsigned *bticket)
+                                                 float *z, float *s, double *part)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     double rr = 0.0, zr = 0.0, bb = 0.0;
-    if (t < nfl) {
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
         Nb nb;
         neighbours(n, c, nb, A);
         float xv[6];
 #pragma unroll
         for (int u = 0; u < 6; ++u) xv[u] = x[nb.c[u]];
-        float xs = x[c];
-        float ax = ax_row(nb, xv, xs, L[c], L, subcell, h2);
+        float ax = ax_row(nb, xv, x[c], L[c], L, subcell, h2);
         float bc = b[c];
         float rc = bc + -1.0f * ax;
         r[c] = rc;
-        rr = (double)(rc * rc);
-        bb = (double)(bc * bc);
+        rr += (double)(rc * rc);
+        bb += (double)(bc * bc);
         if (JACOBI) {
             float zc = precond_cell(n, c, rc, P[c], A);
             z[c] = zc;
             s[c] = zc;
-            zr = (double)(zc * rc);
+            zr += (double)(zc * rc);
         }
     }
-    double t0, t1;
-    if (grid_reduce2(rr, zr, partials, ticket, t0, t1)) {
-        if (threadIdx.x == 0) {
-            scal[S_ERR2_0] = t0;
-            scal[S_A] = t1;
-        }
-    }
-    double b0, b1;
-    if (grid_reduce2(bb, 0.0, bpartials, bticket, b0, b1)) {
-        if (threadIdx.x == 0) scal[S_BNORM2] = b0;
+    store_partials(part, 3, rr, zr, bb, 3);
+}
+
+// single block: initial scalars from the init partials
+__global__ __launch_bounds__(TB) void k_pcg_init_fin(int G, const double *__restrict__ part, double *scal)
+{
+    double rr = sum_partials(part, G, 3, 0);
+    double zr = sum_partials(part, G, 3, 1);
+    double bb = sum_partials(part, G, 3, 2);
+    if (threadIdx.x == 0) {
+        scal[S_ERR2_0] = rr;
+        scal[S_A0] = zr;
+        scal[S_BNORM2] = bb;
+        scal[S_ERR2] = rr;
     }
 }
 
-// iteration part a: s_new = z + beta*s_old (k > 0) or z (k == 0); q = A s_new
+// iteration part a (see the file comment)
 __global__ __launch_bounds__(TB) void k_pcg_a(int nfl, const int32_t *__restrict__ fl, int n,
                                               const int8_t *__restrict__ A, const float *__restrict__ L, int subcell,
                                               double h2, const float *__restrict__ z, const float *__restrict__ s_old,
-                                              float *s_new, float *q, double *partials, unsigned *ticket,
-                                              double *scal, const int32_t *__restrict__ iflags, int first)
+                                              float *s_new, float *q, const double *__restrict__ pb, double *pa,
+                                              double *scal, int32_t *iflags, int k, int G)
 {
     if (iflags[I_DONE]) return;
-    const float beta = (float)scal[S_BETA];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float beta = 0.0f;
+    if (k > 0) {
+        double rr = sum_partials(pb, G, 2, 0);
+        double zr = sum_partials(pb, G, 2, 1);
+        bool capped;
+        bool stop = stop_test(rr, k, iflags, scal, capped);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[S_ERR2] = iflags[I_TOLPOL] == 0 ? (double)(float)rr : rr;
+            iflags[I_ITERS] = k;
+            if (stop) {
+                iflags[I_DONE] = 1;
+                iflags[I_CAPPED] = capped ? 1 : 0;
+            } else {
+                scal[S_A0 + (k & 1)] = zr;
+            }
+        }
+        if (stop) return;
+        double bd = zr / scal[S_A0 + ((k - 1) & 1)];
+        if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_BETA] = bd;
+        beta = (float)bd;
+    }
     double qs = 0.0;
-    if (t < nfl) {
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
         Nb nb;
         neighbours(n, c, nb, A);
@@ -287,99 +296,76 @@ __global__ __launch_bounds__(TB) void k_pcg_a(int nfl, const int32_t *__restrict
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
             long m = nb.c[u];
-            sv[u] = nb.a[u] == FLUID ? (first ? z[m] : z[m] + beta * s_old[m]) : 0.0f;
+            sv[u] = nb.a[u] == FLUID ? (k == 0 ? z[m] : z[m] + beta * s_old[m]) : 0.0f;
         }
-        float ss = first ? z[c] : z[c] + beta * s_old[c];
+        float ss = k == 0 ? z[c] : z[c] + beta * s_old[c];
         float qc = ax_row(nb, sv, ss, L[c], L, subcell, h2);
         s_new[c] = ss;
         q[c] = qc;
-        qs = (double)(qc * ss);
+        qs += (double)(qc * ss);
     }
-    double t0, t1;
-    if (grid_reduce2(qs, 0.0, partials, ticket, t0, t1)) {
-        if (threadIdx.x == 0) {
-            scal[S_ZS] = t0;
-            scal[S_ALPHA] = scal[S_A] / t0;
-        }
-    }
+    store_partials(pa, 1, qs, 0.0, 0.0, 1);
 }
 
-// iteration part b: x, r updates, stop test, z = M r (JACOBI), beta
+// iteration part b: x, r updates and (JACOBI) z = M r
 template <bool JACOBI>
 __global__ __launch_bounds__(TB) void k_pcg_b(int nfl, const int32_t *__restrict__ fl, int n,
                                               const int8_t *__restrict__ A, float *x, float *r,
                                               const float *__restrict__ s, const float *__restrict__ q,
-                                              const double *__restrict__ P, float *z, double *partials,
-                                              unsigned *ticket, double *scal, int32_t *iflags)
+                                              const double *__restrict__ P, float *z, const double *__restrict__ pa,
+                                              double *pb, double *scal, const int32_t *__restrict__ iflags, int k,
+                                              int G)
 {
     if (iflags[I_DONE]) return;
-    const double alpha = scal[S_ALPHA];
+    double qs = sum_partials(pa, G, 1, 0);
+    const double alpha = scal[S_A0 + (k & 1)] / qs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_ALPHA] = alpha;
     const float a = (float)alpha, na = (float)-alpha;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     double rr = 0.0, zr = 0.0;
-    if (t < nfl) {
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
-        float sc = s[c];
-        x[c] = x[c] + a * sc;
+        x[c] = x[c] + a * s[c];
         float rc = r[c] + na * q[c];
         r[c] = rc;
-        rr = (double)(rc * rc);
+        rr += (double)(rc * rc);
         if (JACOBI) {
             float zc = precond_cell(n, c, rc, P[c], A);
             z[c] = zc;
-            zr = (double)(zc * rc);
+            zr += (double)(zc * rc);
         }
     }
-    double t0, t1;
-    if (grid_reduce2(rr, zr, partials, ticket, t0, t1)) {
-        if (threadIdx.x == 0) {
-            int it = iflags[I_ITERS] + 1;
-            iflags[I_ITERS] = it;
-            bool stop;
-            if (iflags[I_TOLPOL] == 0) {
-                float e2 = (float)t0;  // solver.cpp:331 FLOAT error2
-                scal[S_ERR2] = e2;
-                stop = (double)e2 <= scal[S_EPS];
-            } else {
-                scal[S_ERR2] = t0;
-                stop = t0 <= scal[S_EPS];
-            }
-            if (!(t0 == t0)) stop = true;  // NaN: report, do not spin
-            if (!stop && it >= iflags[I_CAP]) {
-                stop = true;
-                iflags[I_CAPPED] = 1;
-            }
-            if (stop) {
-                iflags[I_DONE] = 1;
-            } else if (JACOBI) {
-                scal[S_BETA] = t1 / scal[S_A];
-                scal[S_A] = t1;
-            }
-        }
-    }
+    store_partials(pb, 2, rr, zr, 0.0, 2);
 }
 
-// a2 = z.r and beta after a wavefront preconditioner
-__global__ __launch_bounds__(TB) void k_pcg_beta(int nfl, const int32_t *__restrict__ fl, const float *__restrict__ z,
-                                                 const float *__restrict__ r, double *partials, unsigned *ticket,
-                                                 double *scal, const int32_t *__restrict__ iflags, int init)
+// z.r partials after a wavefront preconditioner (into column `off` of stride `stride`)
+__global__ __launch_bounds__(TB) void k_pcg_zr(int nfl, const int32_t *__restrict__ fl, const float *__restrict__ z,
+                                               const float *__restrict__ r, double *part, int stride, int off,
+                                               const int32_t *__restrict__ iflags)
 {
-    if (!init && iflags[I_DONE]) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iflags[I_DONE]) return;
     double zr = 0.0;
-    if (t < nfl) {
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
-        zr = (double)(z[c] * r[c]);
+        zr += (double)(z[c] * r[c]);
     }
-    double t0, t1;
-    if (grid_reduce2(zr, 0.0, partials, ticket, t0, t1)) {
-        if (threadIdx.x == 0) {
-            if (init) {
-                scal[S_A] = t0;
-            } else {
-                scal[S_BETA] = t0 / scal[S_A];
-                scal[S_A] = t0;
-            }
+    double v = block_sum_all(zr);
+    if (threadIdx.x == 0) part[blockIdx.x * stride + off] = v;
+}
+
+// single block: stop test after k completed iterations (end of a batch)
+__global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb, double *scal, int32_t *iflags, int k,
+                                                  int G)
+{
+    if (iflags[I_DONE]) return;
+    double rr = sum_partials(pb, G, 2, 0);
+    bool capped;
+    bool stop = stop_test(rr, k, iflags, scal, capped);
+    if (threadIdx.x == 0) {
+        scal[S_ERR2] = iflags[I_TOLPOL] == 0 ? (double)(float)rr : rr;
+        iflags[I_ITERS] = k;
+        if (stop) {
+            iflags[I_DONE] = 1;
+            iflags[I_CAPPED] = capped ? 1 : 0;
         }
     }
 }
@@ -490,7 +476,7 @@ __global__ void k_fill_f64(long n, double *a, double v)
 
 }  // namespace
 
-// ordered list of FLUID cells; returns the count (host sync)
+// ordered list of FLUID cells (zeroes non-FLUID press); returns the count (host sync)
 int k_build_fluid_list(FlipCtx *c)
 {
     hipLaunchKernelGGL(k_fluid_flag, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->cell_count, c->press);
@@ -506,11 +492,10 @@ static void precond_wavefront(FlipCtx *c, int mode, const float *r, float *z)
 {
     int n = c->n;
     unsigned nb = nblk((int64_t)n * n, TB);
-    double *qv = (double *)c->tmp0;  // 2 * cap float4 >= n^3 doubles, see allocation
     for (int d = 0; d <= 3 * n - 3; ++d)
-        hipLaunchKernelGGL(k_ic_fwd_plane, nb, TB, 0, c->stream, n, d, mode, c->A, c->P, r, qv);
+        hipLaunchKernelGGL(k_ic_fwd_plane, nb, TB, 0, c->stream, n, d, mode, c->A, c->P, r, c->qv);
     for (int d = 3 * n - 3; d >= 0; --d)
-        hipLaunchKernelGGL(k_ic_bwd_plane, nb, TB, 0, c->stream, n, d, mode, c->A, c->P, qv, z);
+        hipLaunchKernelGGL(k_ic_bwd_plane, nb, TB, 0, c->stream, n, d, mode, c->A, c->P, c->qv, z);
 }
 
 // PCG on the context's A, L, press (x) and div (b, negated in place).
@@ -525,7 +510,6 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
     t_begin(c, "pcg_setup");
     hipLaunchKernelGGL(k_negate, nblk(C, TB), TB, 0, c->stream, C, c->div);  // solver.cpp:231 flip(b)
     int nfl = k_build_fluid_list(c);
-    // device scalars and flags
     int32_t hf[8] = {0, 0, 0, 0, c->prm.tol_policy, 0, 0, 0};
     long cap = c->prm.max_iters > 0 ? c->prm.max_iters : (long)n * n * n;
     hf[I_CAP] = (int32_t)std::min<long>(cap, 0x7fffffff);
@@ -542,6 +526,8 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
         return 0;
     }
     unsigned nb = nblk(nfl, TB);
+    int G = (int)std::min<unsigned>(nb, GMAX);
+    double *pi = c->pcg_part, *pa = c->pcg_part + 3 * GMAX, *pb = c->pcg_part + 4 * GMAX;
     bool jacobi = mode == 0;
     if (mode == 0) {
         hipLaunchKernelGGL(k_build_diag, nb, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, c->P, c->iflags);
@@ -553,25 +539,23 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
     }
     if (!jacobi) {
         hipLaunchKernelGGL(k_fill_f64, nblk(C, TB), TB, 0, c->stream, C, c->P, 0.0);
-        unsigned pb = nblk((int64_t)n * n, TB);
+        unsigned pb2 = nblk((int64_t)n * n, TB);
         for (int d = 0; d <= 3 * n - 3; ++d)
-            hipLaunchKernelGGL(k_ic_build_plane, pb, TB, 0, c->stream, n, d, mode, c->A, c->L, subcell, c->P);
+            hipLaunchKernelGGL(k_ic_build_plane, pb2, TB, 0, c->stream, n, d, mode, c->A, c->L, subcell, c->P);
     }
     float *s_cur = c->sv[0], *s_nxt = c->sv[1];
     if (jacobi) {
-        hipLaunchKernelGGL(k_pcg_init<true>, nb, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->press,
-                           c->div, c->P, c->r, c->z, s_cur, c->partials, c->tickets + 0, c->scal,
-                           c->partials + 2 * (size_t)nb, c->tickets + 1);
+        hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->press,
+                           c->div, c->P, c->r, c->z, s_cur, pi);
     } else {
-        hipLaunchKernelGGL(k_pcg_init<false>, nb, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2,
-                           c->press, c->div, c->P, c->r, c->z, s_cur, c->partials, c->tickets + 0, c->scal,
-                           c->partials + 2 * (size_t)nb, c->tickets + 1);
+        hipLaunchKernelGGL(k_pcg_init<false>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->press,
+                           c->div, c->P, c->r, c->z, s_cur, pi);
         precond_wavefront(c, mode, c->r, c->z);
         hipLaunchKernelGGL(k_copy_fluid, nb, TB, 0, c->stream, nfl, c->fl, c->z, s_cur);
-        hipLaunchKernelGGL(k_pcg_beta, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->r, c->partials, c->tickets + 2,
-                           c->scal, c->iflags, 1);
+        hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pi, 3, 1, c->iflags);
     }
-    // stop threshold (solver.cpp:323) or the relative one
+    hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal);
+    // stop threshold: solver.cpp:323, or the relative one
     double e0 = 0.0, b2 = 0.0;
     FLIP_CHECK(hipMemcpyAsync(&e0, c->scal + S_ERR2_0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FLIP_CHECK(hipMemcpyAsync(&b2, c->scal + S_BNORM2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -582,32 +566,41 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
 
     int iters = 0;
     bool done = c->prm.tol_policy == 1 && e0 <= eps;
-    t_begin(c, "pcg_iterations");
-    int batch = std::max(1, c->last_iters - 2);
+    // Iterations are enqueued in batches closed by a single-block stop check;
+    // kernels after convergence return at once.  With profiling on, every
+    // launch is timed and each batch is one iteration.
+    const bool prof = c->profiling;
+    int batch = prof ? 1 : std::max(1, c->last_iters - 2);
     int k = 0;
     while (!done) {
         for (int t = 0; t < batch; ++t, ++k) {
-            hipLaunchKernelGGL(k_pcg_a, nb, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->z, s_cur,
-                               s_nxt, c->q, c->partials, c->tickets + 0, c->scal, c->iflags, k == 0 ? 1 : 0);
+            t_begin(c, "pcg_a");
+            hipLaunchKernelGGL(k_pcg_a, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->z, s_cur,
+                               s_nxt, c->q, pb, pa, c->scal, c->iflags, k, G);
+            t_end(c);
             std::swap(s_cur, s_nxt);
+            t_begin(c, "pcg_b");
             if (jacobi) {
-                hipLaunchKernelGGL(k_pcg_b<true>, nb, TB, 0, c->stream, nfl, c->fl, n, c->A, c->press, c->r, s_cur,
-                                   c->q, c->P, c->z, c->partials, c->tickets + 1, c->scal, c->iflags);
+                hipLaunchKernelGGL(k_pcg_b<true>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->press, c->r, s_cur,
+                                   c->q, c->P, c->z, pa, pb, c->scal, c->iflags, k, G);
+                t_end(c);
             } else {
-                hipLaunchKernelGGL(k_pcg_b<false>, nb, TB, 0, c->stream, nfl, c->fl, n, c->A, c->press, c->r, s_cur,
-                                   c->q, c->P, c->z, c->partials, c->tickets + 1, c->scal, c->iflags);
+                hipLaunchKernelGGL(k_pcg_b<false>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->press, c->r, s_cur,
+                                   c->q, c->P, c->z, pa, pb, c->scal, c->iflags, k, G);
+                t_end(c);
+                t_begin(c, "precond_wavefront");
                 precond_wavefront(c, mode, c->r, c->z);
-                hipLaunchKernelGGL(k_pcg_beta, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->r, c->partials,
-                                   c->tickets + 2, c->scal, c->iflags, 0);
+                hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
+                t_end(c);
             }
         }
+        hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, k, G);
         FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->iflags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         FLIP_CHECK(hipStreamSynchronize(c->stream));
         done = c->h_pinned[I_DONE] != 0;
         iters = c->h_pinned[I_ITERS];
-        batch = 4;
+        batch = prof ? 1 : 4;
     }
-    t_end(c);
     c->last_iters = iters;
     int32_t capped = 0;
     double e2 = 0.0;
@@ -616,7 +609,7 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
     FLIP_CHECK(hipStreamSynchronize(c->stream));
     if (st) {
         st->iterations = iters;
-        st->error2 = iters > 0 ? e2 : e0;
+        st->error2 = e2;
         st->error2_0 = e0;
         st->bnorm2 = b2;
     }
