@@ -261,11 +261,12 @@ class FlipContext:
         _check(lib().flip_gpu_set_profiling(self.h, 1 if on else 0))
 
     def kernel_times(self):
-        buf = C.create_string_buffer(8192)
-        ms = np.zeros(512, np.float64)
-        cnt = _check(lib().flip_gpu_kernel_times(self.h, buf, 8192, _ptr(ms), 512))
+        cap = 1 << 16
+        buf = C.create_string_buffer(32 * cap)
+        ms = np.zeros(cap, np.float64)
+        cnt = _check(lib().flip_gpu_kernel_times(self.h, buf, 32 * cap, _ptr(ms), cap))
         names = buf.value.decode().split(";") if cnt else []
-        return list(zip(names, ms[:cnt].tolist()))
+        return list(zip(names, ms[:min(cnt, cap)].tolist()))
 
     def synchronize(self):
         _check(lib().flip_gpu_synchronize(self.h))

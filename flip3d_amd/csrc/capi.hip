@@ -133,12 +133,13 @@ void alloc_grids(FlipCtx *c)
     dalloc(&c->r, c->C);
     dalloc(&c->z, c->C);
     dalloc(&c->q, c->C);
-    dalloc(&c->sv[0], c->C);
-    dalloc(&c->sv[1], c->C);
-    FLIP_CHECK(hipMemset(c->sv[0], 0, c->C * sizeof(float)));
-    FLIP_CHECK(hipMemset(c->sv[1], 0, c->C * sizeof(float)));
+    dalloc(&c->zs[0], c->C);
+    dalloc(&c->zs[1], c->C);
+    FLIP_CHECK(hipMemset(c->zs[0], 0, c->C * sizeof(float2)));
+    FLIP_CHECK(hipMemset(c->zs[1], 0, c->C * sizeof(float2)));
     FLIP_CHECK(hipMemset(c->z, 0, c->C * sizeof(float)));
-    dalloc(&c->pcg_part, 8 * 512);
+    dalloc(&c->pcg_part, 8 * 1024);
+    dalloc(&c->flmeta, c->C);
     dalloc(&c->qv, c->C);
     dalloc(&c->scal, 32);
     dalloc(&c->iflags, 16);
@@ -168,10 +169,13 @@ void free_all(FlipCtx *c)
     dfree(c->r);
     dfree(c->z);
     dfree(c->q);
-    dfree(c->sv[0]);
-    dfree(c->sv[1]);
+    dfree(c->zs[0]);
+    dfree(c->zs[1]);
     dfree(c->pcg_part);
+    dfree(c->flmeta);
     dfree(c->qv);
+    if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
+    c->pcg_graph = nullptr;
     dfree(c->scal);
     dfree(c->iflags);
     dfree(c->tickets);
@@ -472,11 +476,13 @@ int flip_gpu_upload_particles(flip_ctx *ctx, int64_t np, const float *pos, const
         FLIP_CHECK(hipStreamSynchronize(c->stream));
         std::vector<float4> p4(np), v4(np), n4(np);
         std::vector<int32_t> mt(np);
+        bool unit = true;
         for (int64_t i = 0; i < np; ++i) {
             int t = type[i];
             require(t == FLIP_FLUID || t == FLIP_WALL, "particle type must be FLUID or WALL");
             float m = mass ? mass[i] : 1.0f;
             require(m >= 0.0f, "negative mass");
+            unit = unit && m == 1.0f;
             float w = t == FLIP_WALL ? -m : m;
             if (t == FLIP_WALL && m == 0.0f) w = -0.0f;
             p4[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w);
@@ -496,6 +502,7 @@ int flip_gpu_upload_particles(flip_ctx *ctx, int64_t np, const float *pos, const
             FLIP_CHECK(hipMemcpy(c->nrm_id, n4.data(), np * sizeof(float4), hipMemcpyHostToDevice));
         }
         build_wall_bins(c, p4, mt, n4);
+        c->unit_mass = unit;
         c->sorted = false;
         c->uploaded = true;
         return FLIP_OK;

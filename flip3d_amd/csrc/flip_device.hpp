@@ -83,6 +83,23 @@ __device__ __forceinline__ int cell_of_b(int gn, float p)
     return (int)v;
 }
 
+// Visit src[b..e) in order, four loads in flight: body(q, index) is called
+// for every element strictly in index order, so order-sensitive float sums
+// are unchanged; the early loads only hide memory latency.
+template <class T, class F>
+__device__ __forceinline__ void for_range4(const T *__restrict__ src, int b, int e, F &&body)
+{
+    int a = b;
+    for (; a + 3 < e; a += 4) {
+        T q0 = src[a], q1 = src[a + 1], q2 = src[a + 2], q3 = src[a + 3];
+        body(q0, a);
+        body(q1, a + 1);
+        body(q2, a + 2);
+        body(q3, a + 3);
+    }
+    for (; a < e; ++a) body(src[a], a);
+}
+
 // ------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_sum(double v)
 {

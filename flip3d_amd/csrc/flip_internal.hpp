@@ -70,9 +70,13 @@ struct FlipCtx {
     int32_t *fl = nullptr;  // fluid cell list (ascending cell index)
     int nfl = 0;
     double *P = nullptr;
-    float *r = nullptr, *z = nullptr, *q = nullptr, *sv[2] = {nullptr, nullptr};
+    float *r = nullptr, *z = nullptr, *q = nullptr;
+    float2 *zs[2] = {nullptr, nullptr};  // (z, s) pairs, ping-pong by iteration parity
     double *partials = nullptr;  // per-block partial sums
-    double *pcg_part = nullptr;  // PCG per-block partials (8 * 512)
+    double *pcg_part = nullptr;  // PCG per-block partials (8 * 1024)
+    uint32_t *flmeta = nullptr;  // per FLUID cell: neighbour kinds (2 bits x 6) + has-AIR bit
+    hipGraphExec_t pcg_graph = nullptr;  // captured batch of PCG iterations
+    int pcg_graph_subcell = -1;
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
     double *scal = nullptr;      // device scalars (see pcg kernels)
     int32_t *iflags = nullptr;   // device int flags / counters
@@ -96,6 +100,7 @@ struct FlipCtx {
     GlibcRand rng;
     bool uploaded = false;
     bool sorted = false;  // storage order matches the current positions
+    bool unit_mass = false;  // every particle has m == 1 (the reference's only mass)
 
     // timing
     std::vector<std::string> tnames;

@@ -45,13 +45,11 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
             if (sj < 0 || sj > n - 1) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for (int a = b; a < e; ++a) {
-                float4 q = pos[a];
-                if (is_wall_w(q.w)) continue;
-                float d2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
-                float w = q.w * smooth_kernel(d2, h);
+            for_range4(pos, b, e, [&](const float4 &q, int) {
+                if (is_wall_w(q.w)) return;
+                float w = q.w * smooth_kernel(length2(q.x, q.y, q.z, p.x, p.y, p.z), h);
                 wsum += w;
-            }
+            });
         }
     }
     vel[i].w = wsum / max_dens;
@@ -115,6 +113,107 @@ __global__ void k_p2g_prep(int64_t np, int n, const float4 *__restrict__ pos, fl
 }
 
 __device__ __forceinline__ int wall_chk(int8_t a) { return a == WALL ? 1 : -1; }
+
+// Unit-mass P2G input: per face component, (clamped x, y, z, u_c) in one
+// float4 (w = m * sharp = sharp when m == 1, which every reference particle has).
+__global__ void k_p2g_prep_unit(int64_t np, int n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                float4 *px, float4 *py, float4 *pz)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) {
+        const float inf = __builtin_inff();
+        float4 far = make_float4(inf, inf, inf, 0.0f);
+        px[i] = far;
+        py[i] = far;
+        pz[i] = far;
+        return;
+    }
+    float4 v = vel[i];
+    float g = (float)n;
+    float x = fmaxf(fminf(g * p.x, g), 0.0f);
+    float y = fmaxf(fminf(g * p.y, g), 0.0f);
+    float z = fmaxf(fminf(g * p.z, g), 0.0f);
+    px[i] = make_float4(x, y, z, v.x);
+    py[i] = make_float4(x, y, z, v.y);
+    pz[i] = make_float4(x, y, z, v.z);
+}
+
+// Faces of one component in 16x16 tiles of (a, k), one layer of the third
+// axis per block row (a = j, layer i for X and Z faces; a = i, layer j for Y
+// faces).  Each wave takes an 8x8 (a, k) patch: its windows' particle ranges
+// run along k, so neighbouring lanes read neighbouring memory.
+template <int COMP>
+__host__ __device__ __forceinline__ void p2g_dims(int n, int &layers, int &na, int &nk)
+{
+    layers = COMP == 1 ? n + 1 : (COMP == 0 ? n + 1 : n);
+    na = n;
+    nk = COMP == 2 ? n + 1 : n;
+}
+
+template <int COMP>
+__device__ __forceinline__ bool p2g_face_of(int n, int &i, int &j, int &k)
+{
+    int layers, na, nk;
+    p2g_dims<COMP>(n, layers, na, nk);
+    int tiles_k = (nk + 15) / 16;
+    int bx = blockIdx.x % tiles_k, by = blockIdx.x / tiles_k;
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int ta = by * 16 + (w >> 1) * 8 + (l >> 3);
+    int tk = bx * 16 + (w & 1) * 8 + (l & 7);
+    int layer = blockIdx.y;
+    if (ta >= na || tk >= nk || layer >= layers) return false;
+    if (COMP == 1) { j = layer; i = ta; }
+    else { i = layer; j = ta; }
+    k = tk;
+    return true;
+}
+
+// mapper.cpp:17-88 with unit masses and packed input (see k_p2g for the window)
+template <int COMP>
+__global__ __launch_bounds__(TB) void k_p2g_unit(int n, const float4 *__restrict__ pk,
+                                                 const int32_t *__restrict__ cs, const int8_t *__restrict__ A,
+                                                 float *__restrict__ ug, float *__restrict__ us, float hh)
+{
+    int i, j, k;
+    if (!p2g_face_of<COMP>(n, i, j, k)) return;
+    const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
+    const long f = ((long)i * nj + j) * nk + k;
+    float fx = COMP == 0 ? (float)i : (float)(i + 0.5);
+    float fy = COMP == 1 ? (float)j : (float)(j + 0.5);
+    float fz = COMP == 2 ? (float)k : (float)(k + 0.5);
+    const int ihi = COMP == 0 ? 0 : 1, jhi = COMP == 1 ? 0 : 1, khi_off = COMP == 2 ? 0 : 1;
+    int klo = max(k - 1, 0), khi = min(k + khi_off, n - 1);
+    float sumw = 0.0f, sumx = 0.0f;
+    if (klo <= khi) {
+        for (int si = i - 1; si <= i + ihi; ++si) {
+            if (si < 0 || si > n - 1) continue;
+            for (int sj = j - 1; sj <= j + jhi; ++sj) {
+                if (sj < 0 || sj > n - 1) continue;
+                int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+                for_range4(pk, b, e, [&](const float4 &q, int) {
+                    float r2 = length2(q.x, q.y, q.z, fx, fy, fz);
+                    if (!(r2 < hh)) return;
+                    float w = sharp_kernel_hh(r2, hh);
+                    sumx += w * q.w;
+                    sumw += w;
+                });
+            }
+        }
+    }
+    float val = sumw != 0.0f ? sumx / sumw : 0.0f;
+    us[f] = val;
+    int a0 = COMP == 0 ? i : (COMP == 1 ? j : k);
+    int na = COMP == 0 ? ni : (COMP == 1 ? nj : nk);
+    bool zero = a0 == 0 || a0 == na - 1;
+    if (!zero) {
+        long c1 = cidx(n, i, j, k);
+        long c0 = COMP == 0 ? cidx(n, i - 1, j, k) : (COMP == 1 ? cidx(n, i, j - 1, k) : cidx(n, i, j, k - 1));
+        zero = wall_chk(A[c1]) * wall_chk(A[c0]) < 0;
+    }
+    ug[f] = zero ? 0.0f : val;
+}
 
 // mapper.cpp:17-88 one face component per launch.  COMP selects the face
 // family: X faces (i, j+.5, k+.5) over window [i-1,i]x[j-2,j+1]x[k-2,k+1],
@@ -378,11 +477,11 @@ __global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const fl
 // re = h = 1/N (corrector.cpp:15-47), u starting from 0.  One block: weights
 // are evaluated in parallel, summed serially in window order, and each
 // particle's result is visible to the next (the reference's loop order).
-// A listed particle with no other listed particle in its 3x3x3 window reads
-// no velocity another listed particle writes, and (the window relation being
-// symmetric) none reads its own, so those resample in parallel here, one
-// thread each with the reference's serial sum; dep[t] marks the others, which
-// k_resample_list then does in list order.
+// A listed particle with no other listed particle inside its kernel support
+// (length2 < h^2) reads no velocity another listed particle writes, and (the
+// relation being symmetric) none reads its own, so those resample in parallel
+// here, one thread each with the reference's serial sum; dep[t] marks the
+// others, which k_resample_list then does in list order.
 __global__ __launch_bounds__(TB) void k_resample_list_indep(int m, const int32_t *__restrict__ ids,
                                                             const int32_t *__restrict__ inv, int n,
                                                             const float4 *__restrict__ pos, float4 *vel,
@@ -393,12 +492,13 @@ __global__ __launch_bounds__(TB) void k_resample_list_indep(int m, const int32_t
     int32_t slot = inv[ids[t]];
     float4 p = pos[slot];
     int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+    // the resample reads a neighbour's velocity only when length2 < hh (the
+    // weight test below); that relation is symmetric
     bool d = false;
     for (int u = 0; u < m && !d; ++u) {
         if (u == t) continue;
         float4 q = pos[inv[ids[u]]];
-        int qi = cell_of(n, q.x), qj = cell_of(n, q.y), qk = cell_of(n, q.z);
-        d = abs(qi - ci) <= 1 && abs(qj - cj) <= 1 && abs(qk - ck) <= 1;
+        d = length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh;
     }
     dep[t] = d ? 1 : 0;
     if (d) return;
@@ -560,10 +660,10 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
             if (sj < 0 || sj > n - 1) continue;
             if (klo > khi) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for (int a = b; a < e; ++a) {
-                if (a == i) continue;
-                float4 q = pos[a];
-                if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) continue;
+            bool bail = false;
+            for_range4(pos, b, e, [&](const float4 &q, int a) {
+                if (bail || a == i) return;
+                if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
                 float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
                 float w = (float)(50.0 * (double)mass_of(q.w) * (double)smooth_kernel(dist * dist, re));
                 if ((double)dist > rre) {
@@ -571,7 +671,10 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
                     sy += w * (p.y - q.y) / dist * re;
                     sz += w * (p.z - q.z) / dist * re;
                 } else if (!is_wall_w(q.w)) {
-                    if (!RAND) return -1;
+                    if (!RAND) {
+                        bail = true;
+                        return;
+                    }
                     sx = (float)((double)sx + cj_ * (double)rv[3 * events + 0] / 100.0);
                     sy = (float)((double)sy + cj_ * (double)rv[3 * events + 1] / 100.0);
                     sz = (float)((double)sz + cj_ * (double)rv[3 * events + 2] / 100.0);
@@ -582,7 +685,8 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
                     sy = (float)((double)sy + cw * (double)nr.y);
                     sz = (float)((double)sz + cw * (double)nr.z);
                 }
-            }
+            });
+            if (bail) return -1;
         }
     }
     tmp0[i] = make_float4(p.x + dt * sx, p.y + dt * sy, p.z + dt * sz, 0.0f);
@@ -672,18 +776,17 @@ __global__ __launch_bounds__(TB) void k_resample(int64_t np, int n, const float4
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
             if (sj < 0 || sj > n - 1) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for (int a = b; a < e; ++a) {
-                float4 q = pos[a];
-                if (is_wall_w(q.w)) continue;
+            for_range4(pos, b, e, [&](const float4 &q, int a) {
+                if (is_wall_w(q.w)) return;
                 float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
-                if (!(d2 < hh)) continue;  // weight exactly 0
+                if (!(d2 < hh)) return;  // weight exactly 0
                 float w = q.w * sharp_kernel_hh(d2, hh);
                 float4 u = vel[a];
                 ax += w * u.x;
                 ay += w * u.y;
                 az += w * u.z;
                 wsum += w;
-            }
+            });
         }
     }
     if (wsum != 0.0f) tmp1[i] = make_float4(ax / wsum, ay / wsum, az / wsum, 0.0f);
@@ -710,11 +813,11 @@ __global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float
                          unsigned long long *out)
 {
     __shared__ double lds[TB / 64];
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
-    if (i < np && !is_wall_w(pos[i].w)) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        if (is_wall_w(pos[i].w)) continue;
         float4 v = vel[i];
-        m = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
+        m = fmax(m, (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
@@ -771,6 +874,30 @@ void k_launch_p2g(FlipCtx *c)
 {
     const float re = (float)1.4;  // RE, mapper.cpp:14
     const float hh = re * re;
+    if (c->unit_mass) {
+        // packed (x, y, z, u_c) per component in spos / tmp0 / tmp1 (free until the corrector)
+        t_begin(c, "p2g_prep");
+        if (c->np > 0)
+            hipLaunchKernelGGL(k_p2g_prep_unit, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(),
+                               c->spos, c->tmp0, c->tmp1);
+        t_end(c);
+        t_begin(c, "p2g");
+        int layers, na, nk;
+        p2g_dims<0>(c->n, layers, na, nk);
+        dim3 g0(((na + 15) / 16) * ((nk + 15) / 16), layers);
+        hipLaunchKernelGGL(k_p2g_unit<0>, g0, TB, 0, c->stream, c->n, c->spos, c->cell_start, c->A, c->ug[0],
+                           c->us[0], hh);
+        p2g_dims<1>(c->n, layers, na, nk);
+        dim3 g1(((na + 15) / 16) * ((nk + 15) / 16), layers);
+        hipLaunchKernelGGL(k_p2g_unit<1>, g1, TB, 0, c->stream, c->n, c->tmp0, c->cell_start, c->A, c->ug[1],
+                           c->us[1], hh);
+        p2g_dims<2>(c->n, layers, na, nk);
+        dim3 g2(((na + 15) / 16) * ((nk + 15) / 16), layers);
+        hipLaunchKernelGGL(k_p2g_unit<2>, g2, TB, 0, c->stream, c->n, c->tmp1, c->cell_start, c->A, c->ug[2],
+                           c->us[2], hh);
+        t_end(c);
+        return;
+    }
     t_begin(c, "p2g_prep");
     if (c->np > 0) hipLaunchKernelGGL(k_p2g_prep, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->spos);
     t_end(c);
@@ -819,7 +946,7 @@ void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hi
 double k_max_speed(FlipCtx *c)
 {
     if (c->np == 0) return 0.0;
-    unsigned nb = nblk(c->np, TB);
+    unsigned nb = std::min(nblk(c->np, TB), 1024u);  // one atomic per block
     unsigned long long *out = (unsigned long long *)(c->scal + 15);
     FLIP_CHECK(hipMemsetAsync(out, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), out);

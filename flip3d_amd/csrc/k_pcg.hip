@@ -3,15 +3,18 @@
 // Vectors live on the full n^3 grid but every pass runs over the compacted
 // FLUID cell list: the reference's op() zeroes non-FLUID x, r, s, z and the
 // ghost-fluid fetch x_ref (solver.cpp:19-26) never reads a non-FLUID entry,
-// so only FLUID cells carry information.
+// so only FLUID cells carry information.  Each FLUID cell also carries a
+// metadata word with the kind of its six neighbours (FLUID / WALL / AIR /
+// outside the grid), so the stencil needs one load instead of seven flags.
 //
-// One PCG iteration k (solver.cpp:326-344) is two kernels over a grid of at
-// most GMAX blocks (grid-stride loops):
+// One PCG iteration k (solver.cpp:326-344) is two kernels over a fixed grid of
+// G blocks (grid-stride loops):
 //   pcg_a(k): every block sums the previous kernel's per-block partials of
 //             (float)(r*r) and (float)(z*r) in one fixed order (so all blocks
 //             agree bit for bit), applies the stop test, forms beta = a_k /
-//             a_{k-1}; then s = z + (float)beta*s (ping-pong buffers, the six
-//             neighbours recomputed), q = A s, partials of (float)(q*s).
+//             a_{k-1}; then s_k = z + (float)beta*s_{k-1} (the six neighbours
+//             recomputed; s_k lives in buffer k&1), q = A s_k, partials of
+//             (float)(q*s).
 //   pcg_b(k): every block forms alpha = a_k / (q.s); x += (float)alpha*s;
 //             r += (float)(-alpha)*q; z = M r; partials of r.r and z.r.
 // Reductions cross kernel boundaries only, so no block needs a device-scope
@@ -20,6 +23,10 @@
 // tree order makes results deterministic run to run and differs from the
 // reference's serial order only in the double's last bits, which the float
 // casts of alpha and beta absorb.
+//
+// The iteration index, buffer parity and FLUID count are read from device
+// memory, so a batch of iterations is captured once as a HIP graph and
+// replayed; kernels after convergence return at once.
 //
 // Preconditioner: with the reference's value test P_ref (solver.cpp:119-123)
 // every coupling term is a signed zero unless some P == 1.0 exactly, so M is
@@ -38,14 +45,20 @@ using namespace flipdev;
 
 namespace {
 
-constexpr int TB = 256;
-constexpr int GMAX = 512;
+constexpr int TB = 1024;   // one 16-wave block per CU for the iteration kernels
+constexpr int GMAX = 256;  // grid of the iteration kernels (partials per reduction)
+constexpr int BATCH = 8;    // iterations per captured graph
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // device scalars: a_k lives in S_A0 + (k & 1)
 enum { S_A0 = 0, S_A1, S_ALPHA, S_BETA, S_ERR2, S_ERR2_0, S_BNORM2, S_EPS };
-// device int flags
-enum { I_DONE = 0, I_ITERS, I_UNIT, I_CAPPED, I_TOLPOL, I_CAP };
+// device int flags; I_KA is the next pcg_a iteration, I_KB the next pcg_b one
+enum { I_DONE = 0, I_ITERS, I_UNIT, I_CAPPED, I_TOLPOL, I_CAP, I_NFL, I_KA, I_KB };
+
+// neighbour kinds in the metadata word, 2 bits each in x_ref order
+// (+i, -i, +j, -j, +k, -k), solver.cpp:19-26 / 161-163
+enum { NB_FLUID = 0, NB_WALL = 1, NB_AIR = 2, NB_OUT = 3 };
+constexpr uint32_t META_HAS_AIR = 1u << 12;
 
 // Block-wide sum of one double per thread, fixed order, result in every thread.
 __device__ __forceinline__ double block_sum_all(double v)
@@ -90,41 +103,39 @@ __device__ __forceinline__ bool stop_test(double rr, int it, const int32_t *ifla
     return stop || capped;
 }
 
-struct Nb {
-    long c[6];     // clamped neighbour cells in x_ref order: +i,-i,+j,-j,+k,-k
-    int8_t a[6];   // their flags
-};
-
-__device__ __forceinline__ void neighbours(int n, long c, Nb &nb, const int8_t *__restrict__ A)
+__device__ __forceinline__ long nb_offset(int u, int n)
 {
-    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
     long nn = (long)n * n;
-    nb.c[0] = i + 1 <= n - 1 ? c + nn : c;
-    nb.c[1] = i - 1 >= 0 ? c - nn : c;
-    nb.c[2] = j + 1 <= n - 1 ? c + n : c;
-    nb.c[3] = j - 1 >= 0 ? c - n : c;
-    nb.c[4] = k + 1 <= n - 1 ? c + 1 : c;
-    nb.c[5] = k - 1 >= 0 ? c - 1 : c;
-#pragma unroll
-    for (int t = 0; t < 6; ++t) nb.a[t] = A[nb.c[t]];
+    switch (u) {
+    case 0: return nn;
+    case 1: return -nn;
+    case 2: return n;
+    case 3: return -n;
+    case 4: return 1;
+    default: return -1;
+    }
 }
 
-// solver.cpp:19-26 x_ref for neighbour t of FLUID cell c with value xs
-__device__ __forceinline__ float x_ref(const Nb &nb, int t, float xn, float xs, float Lc, const float *__restrict__ L,
-                                       int subcell)
+// solver.cpp:19-26 x_ref of neighbour u (kind kd) for FLUID cell c, value xs
+__device__ __forceinline__ float x_ref(int kd, float xn, float xs, long c, long off, float Lc,
+                                       const float *__restrict__ L, int subcell)
 {
-    if (nb.a[t] == FLUID) return xn;
-    if (nb.a[t] == WALL) return xs;
-    return subcell ? (float)((double)L[nb.c[t]] / fmin(1.0e-6, (double)Lc) * (double)xs) : 0.0f;
+    if (kd == NB_FLUID) return xn;
+    if (kd != NB_AIR) return xs;  // WALL, or clamped to the cell itself
+    return subcell ? (float)((double)L[c + off] / fmin(1.0e-6, (double)Lc) * (double)xs) : 0.0f;
 }
 
-// solver.cpp:29-41 one row of compute_Ax
-__device__ __forceinline__ float ax_row(const Nb &nb, const float xv[6], float xs, float Lc, const float *L,
-                                        int subcell, double h2)
+// solver.cpp:29-41 one row of compute_Ax; xv[u] holds FLUID neighbours' values
+__device__ __forceinline__ float ax_row(uint32_t meta, const float xv[6], float xs, long c, int n,
+                                        const float *__restrict__ L, int subcell, double h2)
 {
     double acc = 6.0 * (double)xs;
+    float Lc = (meta & META_HAS_AIR) ? L[c] : 0.0f;
 #pragma unroll
-    for (int t = 0; t < 6; ++t) acc -= (double)x_ref(nb, t, xv[t], xs, Lc, L, subcell);
+    for (int u = 0; u < 6; ++u) {
+        int kd = (meta >> (2 * u)) & 3;
+        acc -= (double)x_ref(kd, xv[u], xs, c, nb_offset(u, n), Lc, L, subcell);
+    }
     return (float)(acc / h2);
 }
 
@@ -152,26 +163,27 @@ __device__ __forceinline__ float a_diag(int n, long c, const int8_t *__restrict_
     return diag;
 }
 
-// signed zero of A_ref(...) * 0.0f: -0 when both cells are in range and FLUID
-__device__ __forceinline__ double coupling_zero(bool both_fluid) { return both_fluid ? -0.0 : 0.0; }
-
-// per-cell preconditioner with every coupling a signed zero (no P == 1.0)
-__device__ __forceinline__ float precond_cell(int n, long c, float r, double P, const int8_t *__restrict__ A)
+// Per-cell preconditioner when no P == 1.0 exists: each coupling term
+// A_ref*P_ref(P)*P_ref(.) is -0 for an in-range FLUID neighbour and +0
+// otherwise (solver.cpp:147-176); the subtractions keep the signed zeros.
+__device__ __forceinline__ double coupling_zero(uint32_t meta, int u)
 {
-    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
-    long nn = (long)n * n;
-    double t = (double)r;
-    t = t - coupling_zero(i - 1 >= 0 && A[c - nn] == FLUID);
-    t = t - coupling_zero(j - 1 >= 0 && A[c - n] == FLUID);
-    t = t - coupling_zero(k - 1 >= 0 && A[c - 1] == FLUID);
-    double q = t * P;
-    double u = q;
-    u = u - coupling_zero(i + 1 <= n - 1 && A[c + nn] == FLUID);
-    u = u - coupling_zero(j + 1 <= n - 1 && A[c + n] == FLUID);
-    u = u - coupling_zero(k + 1 <= n - 1 && A[c + 1] == FLUID);
-    return (float)(u * P);
+    return ((meta >> (2 * u)) & 3) == NB_FLUID ? -0.0 : 0.0;
 }
 
+__device__ __forceinline__ float precond_cell(uint32_t meta, float r, double P)
+{
+    double t = (double)r;
+    t = t - coupling_zero(meta, 1);  // -i
+    t = t - coupling_zero(meta, 3);  // -j
+    t = t - coupling_zero(meta, 5);  // -k
+    double q = t * P;
+    double u = q;
+    u = u - coupling_zero(meta, 0);  // +i
+    u = u - coupling_zero(meta, 2);  // +j
+    u = u - coupling_zero(meta, 4);  // +k
+    return (float)(u * P);
+}
 
 // ---------------------------------------------------------------- setup
 __global__ void k_negate(long C, float *b)
@@ -191,10 +203,28 @@ __global__ void k_fluid_flag(long C, const int8_t *__restrict__ A, int32_t *f, f
     if (!fl && x) x[c] = 0.0f;
 }
 
-__global__ void k_fluid_compact(long C, const int8_t *__restrict__ A, const int32_t *__restrict__ at, int32_t *fl)
+// FLUID cell list and the neighbour metadata words
+__global__ void k_fluid_compact(long C, int n, const int8_t *__restrict__ A, const int32_t *__restrict__ at,
+                                int32_t *fl, uint32_t *meta)
 {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < C && A[c] == FLUID) fl[at[c]] = (int32_t)c;
+    if (c >= C || A[c] != FLUID) return;
+    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
+    const bool inr[6] = {i + 1 <= n - 1, i - 1 >= 0, j + 1 <= n - 1, j - 1 >= 0, k + 1 <= n - 1, k - 1 >= 0};
+    uint32_t m = 0;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        int kd = NB_OUT;
+        if (inr[u]) {
+            int8_t a = A[c + nb_offset(u, n)];
+            kd = a == FLUID ? NB_FLUID : (a == WALL ? NB_WALL : NB_AIR);
+        }
+        m |= (uint32_t)kd << (2 * u);
+        if (kd == NB_AIR) m |= META_HAS_AIR;
+    }
+    int t = at[c];
+    fl[t] = (int32_t)c;
+    meta[t] = m;
 }
 
 // per-cell factorisation diagonal (solver.cpp:125-139 with no active coupling)
@@ -214,30 +244,29 @@ __global__ void k_build_diag(int nfl, const int32_t *__restrict__ fl, int n, con
 
 // r = b - A x (op with a = -1), z = M r, s = z; partials r.r, z.r, b.b
 template <bool JACOBI>
-__global__ __launch_bounds__(TB) void k_pcg_init(int nfl, const int32_t *__restrict__ fl, int n,
-                                                 const int8_t *__restrict__ A, const float *__restrict__ L,
-                                                 int subcell, double h2, const float *__restrict__ x,
-                                                 const float *__restrict__ b, const double *__restrict__ P, float *r,
-                                                 float *z, float *s, double *part)
+__global__ __launch_bounds__(TB) void k_pcg_init(int nfl, const int32_t *__restrict__ fl,
+                                                 const uint32_t *__restrict__ meta, int n,
+                                                 const float *__restrict__ L, int subcell, double h2,
+                                                 const float *__restrict__ x, const float *__restrict__ b,
+                                                 const double *__restrict__ P, float *r, float2 *zs,
+                                                 double *part)
 {
     double rr = 0.0, zr = 0.0, bb = 0.0;
     for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
-        Nb nb;
-        neighbours(n, c, nb, A);
+        uint32_t m = meta[t];
         float xv[6];
 #pragma unroll
-        for (int u = 0; u < 6; ++u) xv[u] = x[nb.c[u]];
-        float ax = ax_row(nb, xv, x[c], L[c], L, subcell, h2);
+        for (int u = 0; u < 6; ++u) xv[u] = ((m >> (2 * u)) & 3) == NB_FLUID ? x[c + nb_offset(u, n)] : 0.0f;
+        float ax = ax_row(m, xv, x[c], c, n, L, subcell, h2);
         float bc = b[c];
         float rc = bc + -1.0f * ax;
         r[c] = rc;
         rr += (double)(rc * rc);
         bb += (double)(bc * bc);
         if (JACOBI) {
-            float zc = precond_cell(n, c, rc, P[c], A);
-            z[c] = zc;
-            s[c] = zc;
+            float zc = precond_cell(m, rc, P[c]);
+            zs[c].x = zc;  // s = z is formed by pcg_a(0)
             zr += (double)(zc * rc);
         }
     }
@@ -258,14 +287,29 @@ __global__ __launch_bounds__(TB) void k_pcg_init_fin(int G, const double *__rest
     }
 }
 
-// iteration part a (see the file comment)
-__global__ __launch_bounds__(TB) void k_pcg_a(int nfl, const int32_t *__restrict__ fl, int n,
-                                              const int8_t *__restrict__ A, const float *__restrict__ L, int subcell,
-                                              double h2, const float *__restrict__ z, const float *__restrict__ s_old,
-                                              float *s_new, float *q, const double *__restrict__ pb, double *pa,
-                                              double *scal, int32_t *iflags, int k, int G)
+// iteration part a (see the file comment).  Buffer B_p holds (z, s) pairs:
+// pcg_a(k) reads (z, s_{k-1}) from B_{(k+1)&1} and writes s_k into B_{k&1}.y;
+// pcg_b(k) reads s_k from B_{k&1}.y and writes the new z into B_{k&1}.x.
+__global__ __launch_bounds__(TB) void k_pcg_a(const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                              int n, const float *__restrict__ L, int subcell, double h2,
+                                              float2 *B0, float2 *B1, float *q, const double *__restrict__ pb,
+                                              double *pa, double *scal, int32_t *iflags, int G)
 {
     if (iflags[I_DONE]) return;
+    const int k = iflags[I_KA];
+    const int nfl = iflags[I_NFL];
+    const float2 *__restrict__ rd = (k & 1) ? B0 : B1;
+    float2 *wr = (k & 1) ? B1 : B0;
+    // the first item's loads go out before the partial-sum prologue
+    const int t0 = blockIdx.x * TB + threadIdx.x;
+    long c0 = 0;
+    uint32_t m0 = 0;
+    float2 v0 = make_float2(0.0f, 0.0f);
+    if (t0 < nfl) {
+        c0 = fl[t0];
+        m0 = meta[t0];
+        v0 = rd[c0];
+    }
     float beta = 0.0f;
     if (k > 0) {
         double rr = sum_partials(pb, G, 2, 0);
@@ -280,27 +324,42 @@ __global__ __launch_bounds__(TB) void k_pcg_a(int nfl, const int32_t *__restrict
                 iflags[I_CAPPED] = capped ? 1 : 0;
             } else {
                 scal[S_A0 + (k & 1)] = zr;
+                iflags[I_KB] = k;
             }
         }
         if (stop) return;
         double bd = zr / scal[S_A0 + ((k - 1) & 1)];
         if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_BETA] = bd;
         beta = (float)bd;
+    } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+        iflags[I_KB] = 0;
     }
     double qs = 0.0;
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        Nb nb;
-        neighbours(n, c, nb, A);
+    for (int t = t0; t < nfl; t += gridDim.x * TB) {
+        long c;
+        uint32_t m;
+        float2 v;
+        if (t == t0) {
+            c = c0;
+            m = m0;
+            v = v0;
+        } else {
+            c = fl[t];
+            m = meta[t];
+            v = rd[c];
+        }
         float sv[6];
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
-            long m = nb.c[u];
-            sv[u] = nb.a[u] == FLUID ? (k == 0 ? z[m] : z[m] + beta * s_old[m]) : 0.0f;
+            sv[u] = 0.0f;
+            if (((m >> (2 * u)) & 3) == NB_FLUID) {
+                float2 w = rd[c + nb_offset(u, n)];
+                sv[u] = k == 0 ? w.x : w.x + beta * w.y;
+            }
         }
-        float ss = k == 0 ? z[c] : z[c] + beta * s_old[c];
-        float qc = ax_row(nb, sv, ss, L[c], L, subcell, h2);
-        s_new[c] = ss;
+        float ss = k == 0 ? v.x : v.x + beta * v.y;
+        float qc = ax_row(m, sv, ss, c, n, L, subcell, h2);
+        wr[c].y = ss;
         q[c] = qc;
         qs += (double)(qc * ss);
     }
@@ -309,28 +368,57 @@ __global__ __launch_bounds__(TB) void k_pcg_a(int nfl, const int32_t *__restrict
 
 // iteration part b: x, r updates and (JACOBI) z = M r
 template <bool JACOBI>
-__global__ __launch_bounds__(TB) void k_pcg_b(int nfl, const int32_t *__restrict__ fl, int n,
-                                              const int8_t *__restrict__ A, float *x, float *r,
-                                              const float *__restrict__ s, const float *__restrict__ q,
-                                              const double *__restrict__ P, float *z, const double *__restrict__ pa,
-                                              double *pb, double *scal, const int32_t *__restrict__ iflags, int k,
-                                              int G)
+__global__ __launch_bounds__(TB) void k_pcg_b(const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                              float *x, float *r, float2 *B0, float2 *B1,
+                                              const float *__restrict__ q, const double *__restrict__ P,
+                                              const double *__restrict__ pa, double *pb, double *scal,
+                                              int32_t *iflags, int G)
 {
     if (iflags[I_DONE]) return;
+    const int k = iflags[I_KB];
+    const int nfl = iflags[I_NFL];
+    float2 *bk = (k & 1) ? B1 : B0;
+    const int t0 = blockIdx.x * TB + threadIdx.x;
+    long c0 = 0;
+    float x0 = 0.0f, sc0 = 0.0f, r0 = 0.0f, q0 = 0.0f;
+    if (t0 < nfl) {
+        c0 = fl[t0];
+        x0 = x[c0];
+        sc0 = bk[c0].y;
+        r0 = r[c0];
+        q0 = q[c0];
+    }
     double qs = sum_partials(pa, G, 1, 0);
     const double alpha = scal[S_A0 + (k & 1)] / qs;
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_ALPHA] = alpha;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[S_ALPHA] = alpha;
+        iflags[I_KA] = k + 1;
+    }
     const float a = (float)alpha, na = (float)-alpha;
     double rr = 0.0, zr = 0.0;
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        x[c] = x[c] + a * s[c];
-        float rc = r[c] + na * q[c];
+    for (int t = t0; t < nfl; t += gridDim.x * TB) {
+        long c;
+        float xc, sc, rc, qc;
+        if (t == t0) {
+            c = c0;
+            xc = x0;
+            sc = sc0;
+            rc = r0;
+            qc = q0;
+        } else {
+            c = fl[t];
+            xc = x[c];
+            sc = bk[c].y;
+            rc = r[c];
+            qc = q[c];
+        }
+        x[c] = xc + a * sc;
+        rc = rc + na * qc;
         r[c] = rc;
         rr += (double)(rc * rc);
         if (JACOBI) {
-            float zc = precond_cell(n, c, rc, P[c], A);
-            z[c] = zc;
+            float zc = precond_cell(meta[t], rc, P[c]);
+            bk[c].x = zc;
             zr += (double)(zc * rc);
         }
     }
@@ -352,11 +440,11 @@ __global__ __launch_bounds__(TB) void k_pcg_zr(int nfl, const int32_t *__restric
     if (threadIdx.x == 0) part[blockIdx.x * stride + off] = v;
 }
 
-// single block: stop test after k completed iterations (end of a batch)
-__global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb, double *scal, int32_t *iflags, int k,
-                                                  int G)
+// single block: stop test after the last pcg_b of a batch
+__global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb, double *scal, int32_t *iflags, int G)
 {
     if (iflags[I_DONE]) return;
+    const int k = iflags[I_KA];  // completed iterations
     double rr = sum_partials(pb, G, 2, 0);
     bool capped;
     bool stop = stop_test(rr, k, iflags, scal, capped);
@@ -370,10 +458,13 @@ __global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb,
     }
 }
 
-__global__ void k_copy_fluid(int nfl, const int32_t *__restrict__ fl, const float *__restrict__ a, float *b)
+// z from the wavefront preconditioner into the .x slots of a (z, s) buffer
+__global__ void k_copy_fluid_x(int nfl, const int32_t *__restrict__ fl, const float *__restrict__ z, float2 *B,
+                               const int32_t *__restrict__ iflags)
 {
+    if (iflags && iflags[I_DONE]) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nfl) b[fl[t]] = a[fl[t]];
+    if (t < nfl) B[fl[t]].x = z[fl[t]];
 }
 
 // ------------------------------------------------ wavefront preconditioner
@@ -425,10 +516,8 @@ __global__ void k_ic_build_plane(int n, int d, int mode, const int8_t *A, const 
     float l = aref(n, A, i - 1, j, k, i, j, k) * pref_d(mode, n, A, P, i - 1, j, k);
     float bo = aref(n, A, i, j - 1, k, i, j, k) * pref_d(mode, n, A, P, i, j - 1, k);
     float ba = aref(n, A, i, j, k - 1, i, j, k) * pref_d(mode, n, A, P, i, j, k - 1);
-    double left = l, bottom = bo, back = ba;
     double diag = (double)a_diag(n, c, A, L, subcell);
-    float sl = (float)left, sb = (float)bottom, sk = (float)back;
-    double e = diag - (double)(sl * sl) - (double)(sb * sb) - (double)(sk * sk);
+    double e = diag - (double)(l * l) - (double)(bo * bo) - (double)(ba * ba);
     if (e < 0.25 * diag) e = diag;
     P[c] = 1.0 / (double)sqrtf((float)e);
 }
@@ -474,14 +563,36 @@ __global__ void k_fill_f64(long n, double *a, double v)
     if (i < n) a[i] = v;
 }
 
+// grid of the iteration kernels for a context (fixed so graphs stay valid)
+int pcg_grid(const FlipCtx *c) { return (int)std::min<unsigned>(nblk(c->C, TB), GMAX); }
+
+void launch_pair(FlipCtx *c, bool jacobi, int G, double h2, int subcell, double *pa, double *pb)
+{
+    int n = c->n;
+    t_begin(c, "pcg_a");
+    hipLaunchKernelGGL(k_pcg_a, G, TB, 0, c->stream, c->fl, c->flmeta, n, c->L, subcell, h2, c->zs[0], c->zs[1],
+                       c->q, pb, pa, c->scal, c->iflags, G);
+    t_end(c);
+    t_begin(c, "pcg_b");
+    if (jacobi)
+        hipLaunchKernelGGL(k_pcg_b<true>, G, TB, 0, c->stream, c->fl, c->flmeta, c->press, c->r, c->zs[0], c->zs[1],
+                           c->q, c->P, pa, pb, c->scal, c->iflags, G);
+    else
+        hipLaunchKernelGGL(k_pcg_b<false>, G, TB, 0, c->stream, c->fl, c->flmeta, c->press, c->r, c->zs[0],
+                           c->zs[1], c->q, c->P, pa, pb, c->scal, c->iflags, G);
+    t_end(c);
+}
+
 }  // namespace
 
-// ordered list of FLUID cells (zeroes non-FLUID press); returns the count (host sync)
+// ordered list of FLUID cells and their neighbour metadata (zeroes non-FLUID
+// press); returns the count (host sync)
 int k_build_fluid_list(FlipCtx *c)
 {
     hipLaunchKernelGGL(k_fluid_flag, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->cell_count, c->press);
     k_launch_scan(c, c->cell_count, c->scan_tmp, c->C);
-    hipLaunchKernelGGL(k_fluid_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->scan_tmp, c->fl);
+    hipLaunchKernelGGL(k_fluid_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->n, c->A, c->scan_tmp, c->fl,
+                       c->flmeta);
     FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->scan_tmp + c->C, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     FLIP_CHECK(hipStreamSynchronize(c->stream));
     c->nfl = c->h_pinned[0];
@@ -510,9 +621,11 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
     t_begin(c, "pcg_setup");
     hipLaunchKernelGGL(k_negate, nblk(C, TB), TB, 0, c->stream, C, c->div);  // solver.cpp:231 flip(b)
     int nfl = k_build_fluid_list(c);
-    int32_t hf[8] = {0, 0, 0, 0, c->prm.tol_policy, 0, 0, 0};
     long cap = c->prm.max_iters > 0 ? c->prm.max_iters : (long)n * n * n;
+    int32_t hf[16] = {0};
+    hf[I_TOLPOL] = c->prm.tol_policy;
     hf[I_CAP] = (int32_t)std::min<long>(cap, 0x7fffffff);
+    hf[I_NFL] = nfl;
     FLIP_CHECK(hipMemcpyAsync(c->iflags, hf, sizeof(hf), hipMemcpyHostToDevice, c->stream));
     double hs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     FLIP_CHECK(hipMemcpyAsync(c->scal, hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
@@ -526,7 +639,7 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
         return 0;
     }
     unsigned nb = nblk(nfl, TB);
-    int G = (int)std::min<unsigned>(nb, GMAX);
+    const int G = pcg_grid(c);
     double *pi = c->pcg_part, *pa = c->pcg_part + 3 * GMAX, *pb = c->pcg_part + 4 * GMAX;
     bool jacobi = mode == 0;
     if (mode == 0) {
@@ -543,15 +656,15 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
         for (int d = 0; d <= 3 * n - 3; ++d)
             hipLaunchKernelGGL(k_ic_build_plane, pb2, TB, 0, c->stream, n, d, mode, c->A, c->L, subcell, c->P);
     }
-    float *s_cur = c->sv[0], *s_nxt = c->sv[1];
+    // the initial z goes to B_1.x, where pcg_a(0) reads it
     if (jacobi) {
-        hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->press,
-                           c->div, c->P, c->r, c->z, s_cur, pi);
+        hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
+                           c->press, c->div, c->P, c->r, c->zs[1], pi);
     } else {
-        hipLaunchKernelGGL(k_pcg_init<false>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->press,
-                           c->div, c->P, c->r, c->z, s_cur, pi);
+        hipLaunchKernelGGL(k_pcg_init<false>, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
+                           c->press, c->div, c->P, c->r, c->zs[1], pi);
         precond_wavefront(c, mode, c->r, c->z);
-        hipLaunchKernelGGL(k_copy_fluid, nb, TB, 0, c->stream, nfl, c->fl, c->z, s_cur);
+        hipLaunchKernelGGL(k_copy_fluid_x, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[1], nullptr);
         hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pi, 3, 1, c->iflags);
     }
     hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal);
@@ -564,42 +677,49 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
     FLIP_CHECK(hipMemcpyAsync(c->scal + S_EPS, &eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     t_end(c);
 
-    int iters = 0;
     bool done = c->prm.tol_policy == 1 && e0 <= eps;
-    // Iterations are enqueued in batches closed by a single-block stop check;
-    // kernels after convergence return at once.  With profiling on, every
-    // launch is timed and each batch is one iteration.
-    const bool prof = c->profiling;
-    int batch = prof ? 1 : std::max(1, c->last_iters - 2);
-    int k = 0;
-    while (!done) {
-        for (int t = 0; t < batch; ++t, ++k) {
-            t_begin(c, "pcg_a");
-            hipLaunchKernelGGL(k_pcg_a, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, h2, c->z, s_cur,
-                               s_nxt, c->q, pb, pa, c->scal, c->iflags, k, G);
-            t_end(c);
-            std::swap(s_cur, s_nxt);
-            t_begin(c, "pcg_b");
-            if (jacobi) {
-                hipLaunchKernelGGL(k_pcg_b<true>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->press, c->r, s_cur,
-                                   c->q, c->P, c->z, pa, pb, c->scal, c->iflags, k, G);
-                t_end(c);
-            } else {
-                hipLaunchKernelGGL(k_pcg_b<false>, G, TB, 0, c->stream, nfl, c->fl, n, c->A, c->press, c->r, s_cur,
-                                   c->q, c->P, c->z, pa, pb, c->scal, c->iflags, k, G);
-                t_end(c);
-                t_begin(c, "precond_wavefront");
-                precond_wavefront(c, mode, c->r, c->z);
-                hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
-                t_end(c);
-            }
+    // Jacobi iterations run as replays of a captured graph of BATCH
+    // iterations plus the stop check; profiling or the wavefront path launch
+    // kernel by kernel (one iteration per host check when profiling).
+    const bool use_graph = jacobi && !c->profiling;
+    if (use_graph && (!c->pcg_graph || c->pcg_graph_subcell != subcell)) {
+        if (c->pcg_graph) {
+            (void)hipGraphExecDestroy(c->pcg_graph);
+            c->pcg_graph = nullptr;
         }
-        hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, k, G);
+        hipGraph_t g;
+        FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int t = 0; t < BATCH; ++t) launch_pair(c, true, G, h2, subcell, pa, pb);
+        hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G);
+        FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
+        FLIP_CHECK(hipGraphInstantiate(&c->pcg_graph, g, nullptr, nullptr, 0));
+        FLIP_CHECK(hipGraphDestroy(g));
+        c->pcg_graph_subcell = subcell;
+    }
+    int iters = 0;
+    int hk = 0;  // host mirror of the iteration index (parity of the z buffer)
+    while (!done) {
+        if (use_graph) {
+            FLIP_CHECK(hipGraphLaunch(c->pcg_graph, c->stream));
+        } else {
+            int batch = c->profiling ? 1 : 4;
+            for (int t = 0; t < batch; ++t, ++hk) {
+                launch_pair(c, jacobi, G, h2, subcell, pa, pb);
+                if (!jacobi) {
+                    t_begin(c, "precond_wavefront");
+                    precond_wavefront(c, mode, c->r, c->z);
+                    hipLaunchKernelGGL(k_copy_fluid_x, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[hk & 1],
+                                       c->iflags);
+                    hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
+                    t_end(c);
+                }
+            }
+            hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G);
+        }
         FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->iflags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         FLIP_CHECK(hipStreamSynchronize(c->stream));
         done = c->h_pinned[I_DONE] != 0;
         iters = c->h_pinned[I_ITERS];
-        batch = prof ? 1 : 4;
     }
     c->last_iters = iters;
     int32_t capped = 0;

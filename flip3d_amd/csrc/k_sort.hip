@@ -110,16 +110,33 @@ __global__ __launch_bounds__(SB) void k_scan_apply(const int32_t *__restrict__ i
     }
 }
 
-// 1. cell key and slot claim
+// 1. cell key and slot claim.  Storage is sorted by the previous step's cells,
+// so neighbouring lanes mostly share a key: each run of equal keys within a
+// wave claims its slots with one atomic (slots in lane order within the run).
 __global__ void k_sort_count(const float4 *__restrict__ pos, int64_t np, int n, int32_t *key, int32_t *slot,
                              int32_t *count)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    float4 p = pos[i];
-    int32_t c = (int32_t)cidx(n, cell_of(n, p.x), cell_of(n, p.y), cell_of(n, p.z));
-    key[i] = c;
-    slot[i] = atomicAdd(&count[c], 1);
+    const int lane = threadIdx.x & 63;
+    int32_t c = -1;
+    if (i < np) {
+        float4 p = pos[i];
+        c = (int32_t)cidx(n, cell_of(n, p.x), cell_of(n, p.y), cell_of(n, p.z));
+    }
+    int32_t prev = __shfl_up(c, 1, 64);
+    bool head = lane == 0 || prev != c;
+    unsigned long long heads = __ballot(head);
+    unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    int my_head = 63 - __clzll(heads & upto);
+    unsigned long long after = my_head == 63 ? 0ull : (heads & ~((2ull << my_head) - 1));
+    int next_head = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && c >= 0) base = atomicAdd(&count[c], next_head - my_head);
+    base = __shfl(base, my_head, 64);
+    if (i < np) {
+        key[i] = c;
+        slot[i] = base + (lane - my_head);
+    }
 }
 
 // 3. scatter source indices to their bucket slots

@@ -204,3 +204,56 @@ def test_trajectory_parity(scene):
     finally:
         g.close()
         s.close()
+
+
+def test_pcg_graph_replays_match_eager_launches():
+    """The captured PCG batch (replayed many times per step, with eager work
+    between steps) against kernel-by-kernel launches: bit-identical fields."""
+    ga = fa.FlipContext(48, tol_policy=1, tol=1e-5, dt_policy=1)
+    gb = fa.FlipContext(48, tol_policy=1, tol=1e-5, dt_policy=1)
+    try:
+        ga.build_scene(fa.SCENE_BENCHMARK)
+        gb.build_scene(fa.SCENE_BENCHMARK)
+        gb.set_profiling(True)  # profiling launches every kernel directly
+        for k in range(6):
+            sa, sb = ga.step(), gb.step()
+            assert sa["iterations"] == sb["iterations"] and sa["iterations"] > 8
+            assert np.array_equal(ga.grid(fa.GRID_PRESS), gb.grid(fa.GRID_PRESS)), k
+            a, b = ga.download(), gb.download()
+            assert np.array_equal(a["p"], b["p"]) and np.array_equal(a["u"], b["u"]), k
+    finally:
+        ga.close()
+        gb.close()
+
+
+@pytest.mark.parametrize("precond", [fa.PRECOND_REFERENCE, fa.PRECOND_IC0])
+def test_solver_modes_match_oracle(precond):
+    """Stand-alone solve on a real N=16 system; IC(0) runs the hyperplane
+    wavefront sweeps (the reference-form path with a coupling-free factor
+    runs per cell)."""
+    d = gold("solve")
+    n = int(d["n"])
+    x0, b0 = d["x"].copy(), d["b"].copy()
+    xo, bo, so = po.solve(n, d["A"], d["L"], x0, b0, precond=precond, tol_policy=1, tol=1e-6)
+    with fa.FlipContext(n, precond=precond, tol_policy=1, tol=1e-6) as g:
+        x, b, st = g.solve(d["A"], d["L"], d["x"], d["b"])
+    assert st["iterations"] == so["iterations"] > 1
+    assert np.array_equal(b.reshape(-1), bo.reshape(-1))
+    assert rel_err(x, xo) <= REL_TOL
+
+
+def test_p2g_general_mass():
+    """Non-unit masses take the general P2G kernel (m * sharp weights)."""
+    d = gold("p2g")
+    gn = 12
+    p, u, t = d[f"p{gn}"], d[f"u{gn}"], d[f"t{gn}"]
+    m = (0.5 + np.random.RandomState(9).uniform(size=len(t))).astype(np.float32)
+    s = po.Sim(gn)
+    s.set_particles(p, u, np.zeros_like(p), t, dens=np.ones(len(t), np.float32), m=m)
+    s.call("sort")
+    s.call("map_p2g")
+    with fa.FlipContext(gn) as g:
+        g.upload(p, u, np.zeros_like(p), t, dens=np.ones(len(t), np.float32), mass=m)
+        grids = g.map_p2g()
+    for c, w in enumerate((po.G_U0, po.G_U1, po.G_U2)):
+        assert np.array_equal(grids[c], s.grid(w)), c
