@@ -24,6 +24,41 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-5
+METRIC = "FLIP steps/s at 128^3 (~4M particles), learned vs plain PCG, rel resid 1e-5"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_a_n128_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a copy of
+    known size); None when no summary for it exists."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return int(float(parts[2]))
+    return None
+
+
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime's copy
+    kernel: the known-size reference a counter run scales FETCH_SIZE and
+    WRITE_SIZE by."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    size = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for rc in (hip.hipMalloc(C.byref(a), C.c_size_t(size)), hip.hipMalloc(C.byref(b), C.c_size_t(size)),
+               hip.hipMemset(a, 1, C.c_size_t(size))):
+        if rc != 0:
+            raise RuntimeError(f"copy probe: HIP error {rc}")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(size), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("copy probe: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid):
@@ -71,6 +106,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precond", default="reference", choices=["reference", "ic0"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -86,11 +123,12 @@ def main():
     for _ in range(args.warmup):
         ctx.step()
     ctx.synchronize()
-    iters = []
+    iters, solve_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = ctx.step()
         iters.append(st["iterations"])
+        solve_ms.append(st["ms_solve"])
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     n_fluid = st["n_fluid_cells"]
@@ -116,8 +154,11 @@ def main():
         b = kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid)
         achieved = b / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                     "bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt}
+
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -125,7 +166,7 @@ def main():
 
     value = args.steps / elapsed
     out = {
-        "metric": "FLIP steps/s at 128^3 (~4M particles), plain PCG, rel resid 1e-5",
+        "metric": METRIC,
         "value": round(value, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -134,6 +175,7 @@ def main():
                    "n": n, "particles": n_particles, "precond": args.precond, "tol": TOL},
         "particles_per_s": round(value * n_particles, 1),
         "pcg_iterations_mean": round(sum(iters) / len(iters), 1),
+        "solve_ms_mean": round(sum(solve_ms) / len(solve_ms), 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "profile_step_ms": {k: round(v[0], 4) for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])},

@@ -845,11 +845,9 @@ __global__ void k_selftest(int64_t m, const float *x, const float *y, float *out
 // ------------------------------------------------------------- launchers
 void k_launch_density(FlipCtx *c)
 {
-    t_begin(c, "density");
     if (c->np > 0)
-        hipLaunchKernelGGL(k_density, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(),
-                           c->cell_start, (float)(4.0 * c->prm.density / c->n), c->max_dens);
-    t_end(c);
+        launch_timed(c, "density", k_density, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->cell_start,
+                     (float)(4.0 * c->prm.density / c->n), c->max_dens);
 }
 
 void k_launch_ext_force(FlipCtx *c)
@@ -864,10 +862,8 @@ void k_launch_flags(FlipCtx *c)
 {
     float d = (float)c->prm.density;
     float n0 = (float)(1.0 / (d * d * d));
-    t_begin(c, "flags");
-    hipLaunchKernelGGL(k_flags, nblk(c->C, TB), TB, 0, c->stream, c->C, c->cell_start, c->P4(), c->V4(), c->A,
-                       c->L, 0.2 * (double)n0);
-    t_end(c);
+    launch_timed(c, "flags", k_flags, nblk(c->C, TB), TB, c->C, c->cell_start, c->P4(), c->V4(), c->A, c->L,
+                 0.2 * (double)n0);
 }
 
 void k_launch_p2g(FlipCtx *c)
@@ -914,12 +910,10 @@ void k_launch_p2g(FlipCtx *c)
 
 void k_launch_g2p(FlipCtx *c)
 {
-    t_begin(c, "g2p_flip");
     double a = c->prm.alpha;
     if (c->np > 0)
-        hipLaunchKernelGGL(k_g2p_flip, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(), c->ug[0],
-                           c->ug[1], c->ug[2], c->us[0], c->us[1], c->us[2], 1.0 - a, a);
-    t_end(c);
+        launch_timed(c, "g2p_flip", k_g2p_flip, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0],
+                     c->ug[1], c->ug[2], c->us[0], c->us[1], c->us[2], 1.0 - a, a);
 }
 
 void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const float *g2, int64_t m, const float4 *p,
@@ -930,12 +924,10 @@ void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const flo
 
 void k_launch_advect(FlipCtx *c)
 {
-    t_begin(c, "advect");
     if (c->np > 0)
-        hipLaunchKernelGGL(k_advect, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(), c->ug[0],
-                           c->ug[1], c->ug[2], c->dt, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n,
-                           c->wall_start, c->wall_pos, c->wall_nrm);
-    t_end(c);
+        launch_timed(c, "advect", k_advect, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0], c->ug[1],
+                     c->ug[2], c->dt, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n, c->wall_start,
+                     c->wall_pos, c->wall_nrm);
 }
 
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s)
@@ -1039,11 +1031,9 @@ int k_correct(FlipCtx *c)
     int n = c->n;
     float dt = (float)c->dt;
     float re = (float)(c->prm.density / n);
-    t_begin(c, "spring");
     FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, sizeof(int32_t), c->stream));
-    hipLaunchKernelGGL(k_spring, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->M(), c->nrm_id,
-                       c->cell_start, dt, re, c->tmp0, c->ev_list, c->ev_count);
-    t_end(c);
+    launch_timed(c, "spring", k_spring, nblk(c->np, TB), TB, c->np, n, c->P4(), c->M(), c->nrm_id, c->cell_start, dt,
+                 re, c->tmp0, c->ev_list, c->ev_count);
     FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     FLIP_CHECK(hipStreamSynchronize(c->stream));
     int m = c->h_pinned[0];
@@ -1075,13 +1065,9 @@ int k_correct(FlipCtx *c)
         hipLaunchKernelGGL(k_spring_fix, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_off, n, c->P4(), c->M(),
                            c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
     }
-    t_begin(c, "resample");
-    hipLaunchKernelGGL(k_resample, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->V4(), c->cell_start,
-                       c->tmp0, c->tmp1, re * re);
-    t_end(c);
-    t_begin(c, "correct_update");
-    hipLaunchKernelGGL(k_correct_update, nblk(c->np, TB), TB, 0, c->stream, c->np, c->P4(), c->V4(), c->tmp0,
-                       c->tmp1);
-    t_end(c);
+    launch_timed(c, "resample", k_resample, nblk(c->np, TB), TB, c->np, n, c->P4(), c->V4(), c->cell_start, c->tmp0,
+                 c->tmp1, re * re);
+    launch_timed(c, "correct_update", k_correct_update, nblk(c->np, TB), TB, c->np, c->P4(), c->V4(), c->tmp0,
+                 c->tmp1);
     return events;
 }

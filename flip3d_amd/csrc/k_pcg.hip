@@ -569,18 +569,14 @@ int pcg_grid(const FlipCtx *c) { return (int)std::min<unsigned>(nblk(c->C, TB), 
 void launch_pair(FlipCtx *c, bool jacobi, int G, double h2, int subcell, double *pa, double *pb)
 {
     int n = c->n;
-    t_begin(c, "pcg_a");
-    hipLaunchKernelGGL(k_pcg_a, G, TB, 0, c->stream, c->fl, c->flmeta, n, c->L, subcell, h2, c->zs[0], c->zs[1],
-                       c->q, pb, pa, c->scal, c->iflags, G);
-    t_end(c);
-    t_begin(c, "pcg_b");
+    launch_timed(c, "pcg_a", k_pcg_a, G, TB, c->fl, c->flmeta, n, c->L, subcell, h2, c->zs[0], c->zs[1], c->q, pb,
+                 pa, c->scal, c->iflags, G);
     if (jacobi)
-        hipLaunchKernelGGL(k_pcg_b<true>, G, TB, 0, c->stream, c->fl, c->flmeta, c->press, c->r, c->zs[0], c->zs[1],
-                           c->q, c->P, pa, pb, c->scal, c->iflags, G);
+        launch_timed(c, "pcg_b", k_pcg_b<true>, G, TB, c->fl, c->flmeta, c->press, c->r, c->zs[0], c->zs[1], c->q,
+                     c->P, pa, pb, c->scal, c->iflags, G);
     else
-        hipLaunchKernelGGL(k_pcg_b<false>, G, TB, 0, c->stream, c->fl, c->flmeta, c->press, c->r, c->zs[0],
-                           c->zs[1], c->q, c->P, pa, pb, c->scal, c->iflags, G);
-    t_end(c);
+        launch_timed(c, "pcg_b", k_pcg_b<false>, G, TB, c->fl, c->flmeta, c->press, c->r, c->zs[0], c->zs[1], c->q,
+                     c->P, pa, pb, c->scal, c->iflags, G);
 }
 
 }  // namespace

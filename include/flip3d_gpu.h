@@ -165,7 +165,8 @@ int flip_gpu_sorted_cells(flip_ctx *ctx, int32_t *counts, int32_t *ids);
 /* Per-kernel device timings of the last step (ms), in launch-name order;
  * names is filled with a ';'-separated list.  Returns the count. */
 int flip_gpu_kernel_times(flip_ctx *ctx, char *names, int names_cap, double *ms, int ms_cap);
-/* Enable/disable per-kernel event timing (adds events around each launch). */
+/* Enable/disable per-kernel timing of the next steps: single-kernel stages record
+ * their dispatch begin/end timestamps, multi-kernel stages an event pair. */
 int flip_gpu_set_profiling(flip_ctx *ctx, int on);
 
 /* Scene generation (host, once; flip3D::init placement, flip3D.cpp:685-764).
