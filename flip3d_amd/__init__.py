@@ -28,6 +28,7 @@ EXPORTS = [
     "flip_gpu_set_dt", "flip_gpu_get_grid", "flip_gpu_set_grid", "flip_gpu_step", "flip_gpu_stage",
     "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
+    "flip_gpu_save_state", "flip_gpu_load_state",
 ]
 
 
@@ -92,6 +93,8 @@ def lib():
         L.flip_gpu_set_profiling.argtypes = [P, C.c_int]
         L.flip_gpu_build_scene.argtypes = [P, C.c_int, C.c_uint64, C.POINTER(C.c_int64)]
         L.flip_gpu_selftest_math.argtypes = [C.c_int64, P, P, P]
+        L.flip_gpu_save_state.argtypes = [P, C.c_char_p]
+        L.flip_gpu_load_state.argtypes = [P, C.c_char_p]
         _lib = L
     return _lib
 
@@ -270,3 +273,10 @@ class FlipContext:
 
     def synchronize(self):
         _check(lib().flip_gpu_synchronize(self.h))
+
+    def save_state(self, path):
+        """Checkpoint in the reference's render/save/<step>_scene.dat layout."""
+        _check(lib().flip_gpu_save_state(self.h, os.fsencode(path)))
+
+    def load_state(self, path):
+        _check(lib().flip_gpu_load_state(self.h, os.fsencode(path)))

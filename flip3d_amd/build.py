@@ -8,6 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libflip3d_amd.so")
+SHIM_SRC = os.path.join(HERE, "shim", "flip3d_shim.cpp")
+SHIM_OUT = os.path.join(HERE, "libflip3d_shim.so")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FLIP_OFFLOAD_ARCH", "gfx950")
@@ -46,9 +48,37 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, OUT)
+    build_shim()
+    build_shim_check()
     if verbose:
-        print("built", OUT)
+        print("built", OUT, SHIM_OUT)
     return OUT
+
+
+def build_shim():
+    """The reference-signature C++ shim: plain g++ over the C ABI, linked
+    against libflip3d_amd.so next to it."""
+    tmp = SHIM_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           SHIM_SRC, "-L", HERE, "-lflip3d_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("shim build failed:\n" + r.stderr)
+    os.replace(tmp, SHIM_OUT)
+    return SHIM_OUT
+
+
+def build_shim_check():
+    """tests/shim/shim_check: the GPU test program for the shim (built here,
+    run by the GPU test suite)."""
+    src = os.path.join(ROOT, "tests", "shim", "shim_check.cpp")
+    out = os.path.join(ROOT, "tests", "shim", "shim_check")
+    cmd = ["g++", "-O1", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-L", HERE,
+           "-lflip3d_shim", "-lflip3d_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../flip3d_amd", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("shim check build failed:\n" + r.stderr)
+    return out
 
 
 if __name__ == "__main__":

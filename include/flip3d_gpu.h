@@ -178,6 +178,19 @@ int flip_gpu_set_profiling(flip_ctx *ctx, int on);
 #define FLIP_SCENE_BENCHMARK 1
 int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np);
 
+/* Checkpoints in the reference's render/save/<step>_scene.dat layout
+ * (saveState flip3D.cpp:453-513, loadState :515-578): int count, float
+ * max_dens, int pourTime, float pourPos[2], float pourRad, count 72-byte
+ * particle records (src/common.h:34-45) in creation order, the pressure grid
+ * (n^3 floats, [i][j][k]), four wall-normal floats per cell.  Saving writes
+ * the emitter fields as the reference's defaults (-1, {0,0}, 0.12) and the
+ * wall-normal grid as compute_wall_normal (flip3D.cpp:75-164) builds it;
+ * loading takes particles, max_dens and pressure, checks the file size
+ * against n, and ignores the emitter fields and the wall-normal grid (wall
+ * normals travel in the particle records).  objects.dat is not read or written. */
+int flip_gpu_save_state(flip_ctx *ctx, const char *path);
+int flip_gpu_load_state(flip_ctx *ctx, const char *path);
+
 /* Device math self-check for the ops whose rounding the parity argument
  * relies on: out[4*i+0] = x/y, +1 = sqrtf(x), +2 = (float)sqrt((double)x*x+(double)y*y),
  * +3 = (float)((double)x / (double)y). */

@@ -338,4 +338,5 @@ def ref_lib():
     lib.ref_kat_solve.argtypes = [C.c_int, P, P, P, P, C.c_int]
     lib.ref_kat_kernels.argtypes = [C.c_int, P, P, P, P]
     lib.ref_kat_length.argtypes = [C.c_int, P, P, P, P]
+    lib.ref_save_state.argtypes = [C.c_int]
     return lib

@@ -138,6 +138,13 @@ void ref_step()
     ref_stage_advect();
     ref_stage_correct();
 }
+// saveState (flip3D.cpp:453-513) as step `at`: writes render/save/<at>_scene.dat
+// under the working directory
+void ref_save_state(int at)
+{
+    step = at;
+    saveState();
+}
 
 // ---- run-time-n module KATs (sorter / mapper / corrector / solver) ----
 static std::vector<particle *> make_particles(int np, const float *p, const float *uu, const float *nrm,

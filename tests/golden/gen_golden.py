@@ -25,7 +25,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-SECTIONS = ["kernels", "sort", "p2g", "g2p", "correct", "solve", "scene_ref", "traj_ref", "traj_bench"]
+SECTIONS = ["kernels", "sort", "p2g", "g2p", "correct", "solve", "scene_ref", "traj_ref", "traj_bench",
+            "checkpoint"]
 
 
 def digest(a):
@@ -286,6 +287,25 @@ def sec_traj_bench(out):
     p, u, nr, t, d = _ref_particles(R)
     np.savez_compressed(out + ".init.npz", p=p, n=nr, t=t, max_dens=np.float32(R.ref_get_max_dens()))
     _trajectory(R, 100, (1, 10), out)
+
+
+def sec_checkpoint(out):
+    """The reference dam-break scene after 5 steps, saved by saveState."""
+    R = ref()
+    with Quiet():
+        R.ref_init()
+        R.ref_srand(1)
+        for _ in range(5):
+            R.ref_step()
+        R.ref_save_state(5)
+    raw = open(os.path.join("render", "save", "5_scene.dat"), "rb").read()
+    from tests.checkpoint_format import checkpoint_fields, field_digests
+    f = checkpoint_fields(raw, R.ref_grid_n())
+    rec = {"n": np.int32(R.ref_grid_n()), "size": np.int64(f["size"]), "count": np.int32(f["count"]),
+           "max_dens": f["max_dens"], "pour_time": f["pour_time"], "pour": f["pour"]}
+    for k, v in field_digests(f).items():
+        rec["sha_" + k] = np.array(v)
+    np.savez_compressed(out, **rec)
 
 
 def run_section(name):
