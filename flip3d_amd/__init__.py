@@ -18,7 +18,7 @@ GRID_U0, GRID_U1, GRID_U2, GRID_US0, GRID_US1, GRID_US2, GRID_A, GRID_L, GRID_PR
 PRECOND_REFERENCE, PRECOND_IC0, PRECOND_LEARNED = 0, 1, 2
 (STAGE_DENSITY, STAGE_EXT_FORCE, STAGE_P2G, STAGE_PROJECT, STAGE_EXTRAP, STAGE_G2P, STAGE_ADVECT,
  STAGE_CORRECT) = range(8)
-SCENE_REFERENCE, SCENE_BENCHMARK = 0, 1
+SCENE_REFERENCE, SCENE_BENCHMARK, SCENE_RANDOM = 0, 1, 2
 BENCH_SEED = 20261203
 
 EXPORTS = [
@@ -29,6 +29,9 @@ EXPORTS = [
     "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
     "flip_gpu_save_state", "flip_gpu_load_state",
+    "flip_gpu_set_learned_weights", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
+    "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
+    "flip_unet_predict", "flip_unet_save", "flip_unet_load",
 ]
 
 
@@ -52,6 +55,7 @@ class Stats(C.Structure):
         ("error2_0", C.c_double), ("dt", C.c_double), ("n_fluid_cells", C.c_int),
         ("n_reposition", C.c_int), ("n_jitter", C.c_int), ("n_precond_unit", C.c_int),
         ("refine_sweeps", C.c_int), ("ms_step", C.c_double), ("ms_solve", C.c_double),
+        ("learned_iterations", C.c_int),
     ]
 
     def as_dict(self):
@@ -95,6 +99,19 @@ def lib():
         L.flip_gpu_selftest_math.argtypes = [C.c_int64, P, P, P]
         L.flip_gpu_save_state.argtypes = [P, C.c_char_p]
         L.flip_gpu_load_state.argtypes = [P, C.c_char_p]
+        L.flip_gpu_set_learned_weights.argtypes = [P, C.c_char_p]
+        L.flip_unet_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]
+        L.flip_unet_destroy.argtypes = [P]
+        L.flip_unet_num_params.argtypes = [P]
+        L.flip_unet_num_params.restype = C.c_int64
+        L.flip_unet_get_params.argtypes = [P, P]
+        L.flip_unet_set_params.argtypes = [P, P]
+        L.flip_unet_forward_features.argtypes = [P, C.c_int, P, P, P]
+        L.flip_unet_backward.argtypes = [P, C.c_int, P, P]
+        L.flip_unet_train_step.argtypes = [P, C.c_int, P, P, P, P, C.c_float, C.POINTER(C.c_double)]
+        L.flip_unet_predict.argtypes = [P, C.c_int, P, P, P, P, P]
+        L.flip_unet_save.argtypes = [P, C.c_char_p]
+        L.flip_unet_load.argtypes = [P, C.c_char_p]
         _lib = L
     return _lib
 
@@ -274,9 +291,96 @@ class FlipContext:
     def synchronize(self):
         _check(lib().flip_gpu_synchronize(self.h))
 
+    def set_learned_weights(self, path):
+        """Weights of the coarse-level network used by PRECOND_LEARNED."""
+        _check(lib().flip_gpu_set_learned_weights(self.h, os.fsencode(path)))
+
     def save_state(self, path):
         """Checkpoint in the reference's render/save/<step>_scene.dat layout."""
         _check(lib().flip_gpu_save_state(self.h, os.fsencode(path)))
 
     def load_state(self, path):
         _check(lib().flip_gpu_load_state(self.h, os.fsencode(path)))
+
+
+class UNet:
+    """The learned coarse-level solver (BF16 MFMA U-Net) for R^3 systems."""
+
+    def __init__(self, R, capacity=1, seed=1):
+        h = C.c_void_p()
+        _check(lib().flip_unet_create(R, capacity, seed, C.byref(h)))
+        self.h, self.R, self.capacity = h, R, capacity
+
+    def close(self):
+        if self.h:
+            lib().flip_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def num_params(self):
+        return int(lib().flip_unet_num_params(self.h))
+
+    def params(self):
+        w = np.empty(self.num_params, np.float32)
+        _check(lib().flip_unet_get_params(self.h, _ptr(w)))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.num_params
+        _check(lib().flip_unet_set_params(self.h, _ptr(w)))
+
+    def forward_features(self, F, mask):
+        F = np.ascontiguousarray(F, np.float32)
+        mask = np.ascontiguousarray(mask, np.float32)
+        S = mask.size // self.R ** 3
+        out = np.empty(mask.shape, np.float32)
+        _check(lib().flip_unet_forward_features(self.h, S, _ptr(F), _ptr(mask), _ptr(out)))
+        return out
+
+    def backward(self, gout):
+        gout = np.ascontiguousarray(gout, np.float32)
+        S = gout.size // self.R ** 3
+        g = np.empty(self.num_params, np.float32)
+        _check(lib().flip_unet_backward(self.h, S, _ptr(gout), _ptr(g)))
+        return g
+
+    @staticmethod
+    def _systems(A, L, b, x=None):
+        A = np.ascontiguousarray(A, np.int8)
+        L = np.ascontiguousarray(L, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        x = None if x is None else np.ascontiguousarray(x, np.float32)
+        return A, L, b, x
+
+    def train_step(self, A, L, b, x, lr):
+        A, L, b, x = self._systems(A, L, b, x)
+        S = A.size // self.R ** 3
+        loss = C.c_double()
+        _check(lib().flip_unet_train_step(self.h, S, _ptr(A), _ptr(L), _ptr(b), _ptr(x), lr, C.byref(loss)))
+        return loss.value
+
+    def predict(self, A, L, b):
+        A, L, b, _ = self._systems(A, L, b)
+        S = A.size // self.R ** 3
+        x = np.empty(b.shape, np.float32)
+        _check(lib().flip_unet_predict(self.h, S, _ptr(A), _ptr(L), _ptr(b), _ptr(x), None))
+        return x
+
+    def save(self, path):
+        _check(lib().flip_unet_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        _check(lib().flip_unet_load(self.h, os.fsencode(path)))

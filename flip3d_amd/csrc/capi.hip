@@ -176,6 +176,8 @@ void free_all(FlipCtx *c)
     dfree(c->qv);
     if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
     c->pcg_graph = nullptr;
+    learned_free(c->learned);
+    c->learned = nullptr;
     dfree(c->scal);
     dfree(c->iflags);
     dfree(c->tickets);
@@ -816,6 +818,23 @@ int flip_gpu_kernel_times(flip_ctx *ctx, char *names, int names_cap, double *ms,
         names[names_cap - 1] = 0;
     }
     return cnt;
+}
+
+int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path)
+{
+    return guarded([&]() -> int {
+        require(ctx && path, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->n % 16 == 0, "the learned preconditioner needs n to be a multiple of 16");
+        FILE *f = fopen(path, "rb");
+        require(f != nullptr, "cannot open U-Net weights file");
+        fclose(f);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        learned_free(c->learned);  // reloaded on the next learned solve
+        c->learned = nullptr;
+        c->learned_weights = path;
+        return FLIP_OK;
+    });
 }
 
 int flip_gpu_set_profiling(flip_ctx *ctx, int on)

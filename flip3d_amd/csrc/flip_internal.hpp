@@ -35,6 +35,9 @@ struct GlibcRand {
     int next();
 };
 
+struct LearnedSolver;  // learned two-grid FCG state (k_pcg.hip)
+void learned_free(LearnedSolver *ls);
+
 // Device buffers of one context.  Particles live in cell-sorted order and are
 // re-permuted after every sort; meta carries creation index << 2 | type.
 struct FlipCtx {
@@ -95,6 +98,9 @@ struct FlipCtx {
     int32_t *ev_off = nullptr;   // per listed particle: offset into ev_rand
     int64_t ev_cap = 0;
     int32_t *h_pinned = nullptr;  // pinned host scalars
+
+    std::string learned_weights;  // U-Net weights file for FLIP_PRECOND_LEARNED
+    LearnedSolver *learned = nullptr;
 
     float max_dens = 1.0f;
     double dt = 6e-3;

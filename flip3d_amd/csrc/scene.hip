@@ -263,6 +263,51 @@ extern "C" int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int
                 rng.next();
                 rng.next();
             });
+        } else if (scene == FLIP_SCENE_RANDOM) {
+            // 1-4 liquid boxes (edges U(0.1, 0.6)) or spheres (radius U(0.05, 0.2)),
+            // centres uniform inside the walls; draws from SplitMix64(seed) at
+            // indices 1e9 + i so the jitter stream below stays independent
+            uint64_t draw = 1000000000ull;
+            auto uni = [&](double a, double b) {
+                double uu = (double)(splitmix_at(seed, draw++) >> 11) * (1.0 / 9007199254740992.0);
+                return a + (b - a) * uu;
+            };
+            int nobj = 1 + (int)(uni(0.0, 1.0) * 4.0);
+            if (nobj > 4) nobj = 4;
+            for (int m = 0; m < nobj; ++m) {
+                Obj b{};
+                b.type = FLIP_FLUID;
+                double cx = uni(wt, 1.0 - wt), cy = uni(wt, 1.0 - wt), cz = uni(wt, 1.0 - wt);
+                if (uni(0.0, 1.0) < 0.5) {
+                    b.shape = 0;
+                    double e[3] = {uni(0.1, 0.6), uni(0.1, 0.6), uni(0.1, 0.6)};
+                    double c3[3] = {cx, cy, cz};
+                    for (int d = 0; d < 3; ++d) {
+                        b.lo[d] = (float)(c3[d] - 0.5 * e[d]);
+                        b.hi[d] = (float)(c3[d] + 0.5 * e[d]);
+                    }
+                } else {
+                    b.shape = 1;
+                    b.r = (float)uni(0.05, 0.2);
+                    b.c[0] = (float)cx;
+                    b.c[1] = (float)cy;
+                    b.c[2] = (float)cz;
+                }
+                o.push_back(b);
+            }
+            double dx = 1.0 / N;
+            uint64_t fi = 0;
+            place_fluid(s, o, N, dens, [&](Scene &sc, double x, double y, double z) {
+                double j3[3];
+                for (int ax = 0; ax < 3; ++ax) {
+                    uint64_t v = splitmix_at(seed, 3 * fi + (uint64_t)ax);
+                    double uu = (double)(v >> 11) * (1.0 / 9007199254740992.0);
+                    j3[ax] = (uu * 0.5 - 0.25) * dx;
+                }
+                sc.push((float)(x + j3[0]), (float)(y + j3[1]), (float)(z + j3[2]), FLIP_FLUID);
+                fi++;
+            });
+            place_walls(s, o, N, []() {});
         } else if (scene == FLIP_SCENE_BENCHMARK) {
             Obj a{}, b{};
             a.type = FLIP_FLUID;

@@ -53,7 +53,8 @@ extern "C" {
 /* preconditioners */
 #define FLIP_PRECOND_REFERENCE 0 /* solver.cpp:119-180 as written (value-test P_ref) */
 #define FLIP_PRECOND_IC0 1       /* same factorisation with couplings keyed on flags */
-#define FLIP_PRECOND_LEARNED 2   /* U-Net warm start + flexible PCG (reference precond inside) */
+#define FLIP_PRECOND_LEARNED 2   /* flexible CG, two-grid preconditioner with the U-Net as coarse solver;
+                                    the reference PCG then checks (and if needed finishes) the result */
 
 /* Step parameters.  flip_gpu_default_params fills the reference's compile-time
  * defaults (src/flip3D.cpp:25-39, mapper.cpp:14, corrector.cpp:13,
@@ -91,6 +92,7 @@ typedef struct flip_step_stats {
     int refine_sweeps;   /* outer refinement sweeps used */
     double ms_step;      /* device time of the step, HIP events */
     double ms_solve;     /* device time of the pressure solve */
+    int learned_iterations; /* FLIP_PRECOND_LEARNED: flexible-CG iterations before the plain-PCG check */
 } flip_step_stats;
 
 typedef struct flip_ctx flip_ctx;
@@ -176,6 +178,7 @@ int flip_gpu_set_profiling(flip_ctx *ctx, int on);
  * particles and max_dens into the context; returns the particle count via np. */
 #define FLIP_SCENE_REFERENCE 0
 #define FLIP_SCENE_BENCHMARK 1
+#define FLIP_SCENE_RANDOM 2 /* 1-4 liquid boxes (edges U(0.1,0.6)) / spheres (r U(0.05,0.2)), seeded */
 int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np);
 
 /* Checkpoints in the reference's render/save/<step>_scene.dat layout
@@ -190,6 +193,11 @@ int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np);
  * normals travel in the particle records).  objects.dat is not read or written. */
 int flip_gpu_save_state(flip_ctx *ctx, const char *path);
 int flip_gpu_load_state(flip_ctx *ctx, const char *path);
+
+/* Weights file (flip_unet_save layout) of the coarse-level network used by
+ * FLIP_PRECOND_LEARNED; the network runs on the (n/2)^3 coarse grid, so n
+ * must be a multiple of 16.  No counterpart in the reference. */
+int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path);
 
 /* Device math self-check for the ops whose rounding the parity argument
  * relies on: out[4*i+0] = x/y, +1 = sqrtf(x), +2 = (float)sqrt((double)x*x+(double)y*y),

@@ -20,7 +20,7 @@ def declared_symbols():
             continue
         text = open(os.path.join(INCLUDE, fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names.update(re.findall(r"\b(flip_gpu_\w+)\s*\(", text))
+        names.update(re.findall(r"\b(flip_(?:gpu|unet)_\w+)\s*\(", text))
     return names
 
 
