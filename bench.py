@@ -26,6 +26,7 @@ HBM_PEAK_GBS = 8000.0
 TOL = 1e-5
 METRIC = "FLIP steps/s at 128^3 (~4M particles), learned vs plain PCG, rel resid 1e-5"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_a_n128_traffic.txt")
+DEFAULT_WEIGHTS = os.path.join(ROOT, "flip3d_amd", "weights", "unet_coarse.bin")
 
 
 def measured_traffic(kernel):
@@ -104,7 +105,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--precond", default="reference", choices=["reference", "ic0"])
+    ap.add_argument("--precond", default="reference", choices=["reference", "ic0", "learned"])
+    ap.add_argument("--weights", default=DEFAULT_WEIGHTS,
+                    help="coarse-level network weights for --precond learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (counter calibration)")
@@ -116,8 +119,10 @@ def main():
 
     import flip3d_amd as fa
 
-    precond = {"reference": fa.PRECOND_REFERENCE, "ic0": fa.PRECOND_IC0}[args.precond]
+    precond = {"reference": fa.PRECOND_REFERENCE, "ic0": fa.PRECOND_IC0, "learned": fa.PRECOND_LEARNED}[args.precond]
     ctx = fa.FlipContext(args.n, tol_policy=1, tol=TOL, dt_policy=1, precond=precond)
+    if args.precond == "learned":
+        ctx.set_learned_weights(args.weights)
     n_particles = ctx.build_scene(fa.SCENE_BENCHMARK)
     ctx.srand(1)
     for _ in range(args.warmup):
