@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("subcell", C.c_int), ("tol_policy", C.c_int), ("tol", C.c_double), ("precond", C.c_int),
         ("max_iters", C.c_int), ("correction", C.c_int), ("reposition", C.c_int),
         ("rand_seed", C.c_uint), ("refine", C.c_int), ("refine_tol", C.c_double),
+        ("solve_fp64", C.c_int),
     ]
 
 

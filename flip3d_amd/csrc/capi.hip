@@ -174,6 +174,14 @@ void free_all(FlipCtx *c)
     dfree(c->pcg_part);
     dfree(c->flmeta);
     dfree(c->qv);
+    dfree(c->x64);
+    dfree(c->b64);
+    dfree(c->r64);
+    dfree(c->s64);
+    dfree(c->q64);
+    dfree(c->part64);
+    if (c->h_part64) (void)hipHostFree(c->h_part64);
+    c->h_part64 = nullptr;
     if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
     c->pcg_graph = nullptr;
     learned_free(c->learned);
@@ -415,6 +423,7 @@ int flip_gpu_default_params(flip_params *p, int n)
     p->rand_seed = 1;
     p->refine = 0;
     p->refine_tol = 1e-7;
+    p->solve_fp64 = 0;
     return FLIP_OK;
 }
 
@@ -424,6 +433,11 @@ int flip_gpu_create(const flip_params *p, flip_ctx **out)
         require(p && out, "null argument");
         require(p->n >= 2 && p->n <= 1024, "n out of range [2, 1024]");
         require(p->precond >= 0 && p->precond <= 2, "unknown preconditioner");
+        require(p->refine >= 0, "refine must be >= 0");
+        require(p->refine == 0 || (p->tol_policy == 1 && p->refine_tol > 0.0),
+                "iterative refinement needs tol_policy 1 and refine_tol > 0");
+        require(p->solve_fp64 == 0 || p->solve_fp64 == 1, "solve_fp64 must be 0 or 1");
+        require(p->solve_fp64 == 0 || p->tol_policy == 1, "the FP64 solve needs tol_policy 1");
         int ndev = 0;
         FLIP_CHECK(hipGetDeviceCount(&ndev));
         require(ndev > 0, "no HIP device", FLIP_E_HIP);

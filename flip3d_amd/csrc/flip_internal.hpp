@@ -35,7 +35,7 @@ struct GlibcRand {
     int next();
 };
 
-struct LearnedSolver;  // learned two-grid FCG state (k_pcg.hip)
+struct LearnedSolver;  // learned multilevel flexible-CG state (k_learned.hip)
 void learned_free(LearnedSolver *ls);
 
 // Device buffers of one context.  Particles live in cell-sorted order and are
@@ -82,6 +82,10 @@ struct FlipCtx {
     hipGraphExec_t pcg_graph = nullptr;  // captured batch of PCG iterations
     int pcg_graph_subcell = -1;
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
+    // FP64 solve / iterative refinement vectors (allocated on first use)
+    double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr, *s64 = nullptr, *q64 = nullptr;
+    double *part64 = nullptr;    // per-block partials of the FP64 kernels
+    double *h_part64 = nullptr;  // pinned host copy of part64
     double *scal = nullptr;      // device scalars (see pcg kernels)
     int32_t *iflags = nullptr;   // device int flags / counters
     unsigned *tickets = nullptr;
@@ -136,6 +140,10 @@ int k_correct(FlipCtx *c);                      // corrector; returns jitter eve
 void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n);  // exclusive scan, out[n] = total
 int k_build_fluid_list(FlipCtx *c);             // returns nfl (host sync)
 int k_solve(FlipCtx *c, flip_step_stats *st);   // PCG on (A, L, press, div)
+// FLIP_PRECOND_LEARNED: flexible CG from the current press and r (after the
+// PCG init); returns its iterations, *converged when ||r||^2 <= eps
+int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *converged);
+int learned_levels(int n);  // coarse levels of the learned preconditioner for side n
 void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const float *g2, int64_t m,
                          const float4 *p, float4 *out);
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s);

@@ -141,6 +141,87 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t *__restrict__ 
         }
 }
 
+// Split-K variant for small layers: one block per (16-voxel tile, 16-channel
+// output tile); its 4 waves take every 4th K step (the next step's operands
+// are loaded while the current MFMA runs) and are summed through LDS.  This
+// gives layers of a few hundred voxels (the bottom of the U-Net) 4 * NT
+// times the waves of k_conv3_fwd.
+// When U is given, the input is up(U) + X (nearest upsampling of the
+// (R/2)^3 tensor U, rounded to bf16 exactly as k_up_add rounds it), so the
+// sum is never stored.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict__ X, const uint16_t *__restrict__ U,
+                                                      const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
+                                                      uint16_t *Y, float *Yf, int R, long nvox, int relu)
+{
+    static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+    constexpr int CPT = CIN / 8;
+    constexpr int NCH = 27 * CPT;
+    constexpr int STEPS = (NCH + 3) / 4;
+    __shared__ f32x4 red[3][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, grp = lane >> 4;
+    const long base = (long)blockIdx.x * 16;
+    const int n0 = blockIdx.y * 16;
+    const long v = base + row;
+    const bool valid = v < nvox;
+    const Vox vo = vox_of(valid ? v : 0, R);
+    auto load_a = [&](int st) -> bf16x8 {
+        const int q = st * 4 + grp;
+        if (st >= STEPS || q >= NCH || !valid) return zero8();
+        const int tap = q / CPT, c0 = (q % CPT) * 8;
+        const long sv = vox_shift(vo, tap, R);
+        if (sv < 0) return zero8();
+        if (!U) return load8(X + sv * CIN + c0);
+        const int z = vo.z + tap / 9 - 1, y = vo.y + (tap / 3) % 3 - 1, x = vo.x + tap % 3 - 1, Rc = R >> 1;
+        const long p = (((long)vo.s * Rc + (z >> 1)) * Rc + (y >> 1)) * Rc + (x >> 1);
+        const uint4 xs = *reinterpret_cast<const uint4 *>(X + sv * CIN + c0);
+        const uint4 us = *reinterpret_cast<const uint4 *>(U + p * CIN + c0);
+        const uint32_t xw[4] = {xs.x, xs.y, xs.z, xs.w}, uw[4] = {us.x, us.y, us.z, us.w};
+        uint32_t o[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const uint16_t lo = f2bf(bf2f((uint16_t)(uw[h] & 0xffff)) + bf2f((uint16_t)(xw[h] & 0xffff)));
+            const uint16_t hi = f2bf(bf2f((uint16_t)(uw[h] >> 16)) + bf2f((uint16_t)(xw[h] >> 16)));
+            o[h] = (uint32_t)lo | ((uint32_t)hi << 16);
+        }
+        return __builtin_bit_cast(bf16x8, make_uint4(o[0], o[1], o[2], o[3]));
+    };
+    auto load_b = [&](int st) -> bf16x8 {
+        const int q = st * 4 + grp;
+        if (st >= STEPS || q >= NCH) return zero8();
+        return load8(Wt + ((long)(q / CPT) * COUT + n0 + row) * CIN + (q % CPT) * 8);
+    };
+    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 a = load_a(wave), b = load_b(wave);
+    for (int st = wave; st < STEPS; st += 4) {
+        const bf16x8 an = load_a(st + 4), bn = load_b(st + 4);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        a = an;
+        b = bn;
+    }
+    if (wave > 0) red[wave - 1][lane] = acc;
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        const f32x4 o = red[w][lane];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] += o[i];
+    }
+    const int co = n0 + row;
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        long vv = base + grp * 4 + i;
+        if (vv >= nvox) continue;
+        float val = acc[i] + bv;
+        if (relu && val < 0.0f) val *= UNET_LEAK;
+        if (Y) Y[vv * COUT + co] = f2bf(val);
+        if (Yf) Yf[vv * COUT + co] = val;
+    }
+}
+
 // ------------------------------------------------------- weight gradient
 // partial[(chunk*27 + tap)][co][ci] = sum over the chunk's voxels v of
 // dY[v][co] * X[v + tap][ci].  grid (chunks, 27), 4 waves per block; each
@@ -366,11 +447,11 @@ __global__ void k_head_back(const uint16_t *__restrict__ X, int C, const float *
 
 // Adam on fp32 master weights, then the bf16 working copy
 __global__ void k_adam(long n, float *w, const float *__restrict__ g, float *m, float *v, float lr, float b1, float b2,
-                       float eps, float c1, float c2, uint16_t *wbf)
+                       float eps, float c1, float c2, const float *__restrict__ gscale, uint16_t *wbf)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float gi = g[i];
+    float gi = g[i] * gscale[0];
     float mi = b1 * m[i] + (1.0f - b1) * gi;
     float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
@@ -378,6 +459,30 @@ __global__ void k_adam(long n, float *w, const float *__restrict__ g, float *m, 
     float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     w[i] = wi;
     if (wbf) wbf[i] = f2bf(wi);
+}
+
+// per-block sums of g^2 (gradient norm for clipping)
+__global__ void k_sumsq(long n, const float *__restrict__ g, double *partial)
+{
+    __shared__ double lds[4];
+    double a = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        a += (double)g[i] * (double)g[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// gscale = min(1, clip / |g|) from the per-block sums
+__global__ void k_clip_scale(const double *__restrict__ partial, int blocks, float clip, float *gscale)
+{
+    if (threadIdx.x != 0) return;
+    double a = 0.0;
+    for (int b = 0; b < blocks; ++b) a += partial[b];
+    double nrm = sqrt(a);
+    gscale[0] = (nrm > (double)clip && nrm == nrm) ? (float)((double)clip / nrm) : (nrm == nrm ? 1.0f : 0.0f);
 }
 
 __global__ void k_to_bf16(long n, const float *__restrict__ a, uint16_t *b)
@@ -400,6 +505,13 @@ void fwd_t(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y,
 {
     constexpr int MT = COUT >= 64 ? 2 : 4;
     long per_block = 4L * 16 * MT;
+    const long waves = (nvox + 16 * MT - 1) / (16 * MT);
+    if (waves < 2048) {
+        // too few voxel tiles to fill the chip: split K and the output channels
+        hipLaunchKernelGGL((k_conv3_fwd_sk<CIN, COUT>), dim3(nb(nvox, 16), COUT / 16), dim3(256), 0, s, X,
+                           (const uint16_t *)nullptr, W, bias, Y, Yf, R, nvox, relu);
+        return;
+    }
     hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, MT>), dim3(nb(nvox, (int)per_block)), dim3(256), 0, s, X, W, bias, Y,
                        Yf, R, nvox, relu);
 }
@@ -423,6 +535,20 @@ void unet_conv3_fwd(int cin, int cout, const uint16_t *X, const uint16_t *W, con
         return;                                                     \
     }
     CASE(8, 16) CASE(16, 16) CASE(16, 32) CASE(32, 32) CASE(32, 64) CASE(64, 64) CASE(64, 32) CASE(32, 16)
+#undef CASE
+    throw flip_arg_error{FLIP_E_ARG, "unsupported conv channel pair"};
+}
+
+void unet_conv3_fwd_up(int cin, int cout, const uint16_t *U, const uint16_t *X, const uint16_t *W, const float *bias,
+                       uint16_t *Y, int R, long nvox, hipStream_t s)
+{
+#define CASE(a, b)                                                                                                   \
+    if (cin == a && cout == b) {                                                                                     \
+        hipLaunchKernelGGL((k_conv3_fwd_sk<a, b>), dim3(nb(nvox, 16), b / 16), dim3(256), 0, s, X, U, W, bias, Y,     \
+                           (float *)nullptr, R, nvox, 1);                                                            \
+        return;                                                                                                      \
+    }
+    CASE(64, 32) CASE(32, 16) CASE(16, 16)
 #undef CASE
     throw flip_arg_error{FLIP_E_ARG, "unsupported conv channel pair"};
 }
@@ -508,11 +634,15 @@ void unet_head_back(const uint16_t *X, int C, const float *w, const float *mask,
     hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, s, partial, blocks, (long)(C + 1), dwb);
 }
 
-void unet_adam(long n, float *w, const float *g, float *m, float *v, float lr, int t, uint16_t *wbf, hipStream_t s)
+void unet_adam(long n, float *w, const float *g, float *m, float *v, float lr, int t, float clip, double *partial,
+               float *gscale, uint16_t *wbf, hipStream_t s)
 {
     const float b1 = 0.9f, b2 = 0.999f;
     float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
-    hipLaunchKernelGGL(k_adam, dim3(nb(n)), dim3(256), 0, s, n, w, g, m, v, lr, b1, b2, 1e-8f, c1, c2, wbf);
+    const int blocks = 256;
+    hipLaunchKernelGGL(k_sumsq, dim3(blocks), dim3(256), 0, s, n, g, partial);
+    hipLaunchKernelGGL(k_clip_scale, dim3(1), dim3(64), 0, s, partial, blocks, clip, gscale);
+    hipLaunchKernelGGL(k_adam, dim3(nb(n)), dim3(256), 0, s, n, w, g, m, v, lr, b1, b2, 1e-8f, c1, c2, gscale, wbf);
 }
 
 void unet_to_bf16(long n, const float *a, uint16_t *b, hipStream_t s)

@@ -1,0 +1,719 @@
+// k_learned.hip - FLIP_PRECOND_LEARNED: flexible CG whose preconditioner is a
+// multigrid V-cycle with the U-Net as its coarsest-level solver.  No
+// counterpart in the reference, which runs PCG only (src/solver.cpp:183-218);
+// the reference PCG checks (and if needed finishes) every learned solve.
+//
+// Hierarchy.  Level 0 is the pressure system itself (the FLUID list of the
+// PCG).  Level l >= 1 is a dense R_l^3 grid, R_l = n >> l, rediscretised from
+// level l-1: a cell is FLUID if any of its 8 children is, WALL if all are,
+// AIR otherwise; its level set is the mean over its FLUID children (FLUID
+// cells) or its non-WALL children (AIR cells).  Every level uses the solver's
+// ghost-fluid 7-point operator (solver.cpp:19-41) at its own spacing
+// 1/R_l.  The coarsest level K has side 32 for n = 64, 128, 256 (16 for
+// n = 32); the U-Net solves it.
+//
+// V-cycle z = M(r) (damped Jacobi, weight MG_W, two sweeps down and two up):
+//   level 0:  z = S(r); b_1 = R(r - A z)
+//   level l:  x_l = S(b_l); b_{l+1} = R(b_l - A_l x_l)        l = 1..K-1
+//   level K:  x_K = UNet(A_K, L_K, b_K)
+//   up:       x_l += P x_{l+1} / MG_S; x_l = S(x_l; b_l)       l = K-1..1
+//             z += P x_1 / MG_S; z = S(z; r)
+// R sums the residual over each coarse cell's FLUID children and P injects
+// the coarse value into the children.  With these transfers the Galerkin
+// coarse operator R A P is 16 times the rediscretised one; MG_S = 10
+// over-corrects that by 1.6, as piecewise-constant transfers want.
+//
+// Flexible CG (Polak-Ribiere: beta_k = -alpha_{k-1} (z_k.q_{k-1}) /
+// (z_{k-1}.r_{k-1})), because the network makes M nonlinear.  One iteration
+// (two CG kernels, the stop check, the V-cycle) is captured once as a HIP
+// graph: the iteration index, buffer parity, FLUID count and the stop flag
+// live in device memory, kernels after convergence return at once (the
+// network's own kernels still run), and the host launches iteration k+1
+// before it reads iteration k's stop flag.
+#include "pcg_device.hpp"
+#include "unet_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+using namespace pcgdev;
+
+namespace {
+
+constexpr float MG_W = 0.8f;    // damped Jacobi weight
+constexpr float MG_S = 10.0f;   // coarse-correction scale (see the file comment)
+constexpr int FCG_MAX = 60;     // flexible iterations before the plain PCG takes over
+constexpr int MG_COARSE = 32;   // target side of the coarsest (network) level
+constexpr int TBL = 256;        // block of the level kernels
+
+// fs (device doubles): z.r of the last two iterations, alpha of the last two, eps, r.r
+enum { F_ZR0 = 0, F_ZR1, F_AL0, F_AL1, F_EPS, F_RR };
+// fi (device ints): iteration index, stop flag, converged flag
+enum { FI_K = 0, FI_DONE, FI_CONV };
+
+inline unsigned nb(long n, int t = TBL) { return (unsigned)((n + t - 1) / t); }
+
+struct IJK {
+    int i, j, k;
+};
+
+__device__ __forceinline__ IJK ijk_of(long c, int n)
+{
+    const long nn = (long)n * n;
+    const int i = (int)(c / nn);
+    const long rem = c - (long)i * nn;
+    const int j = (int)(rem / n);
+    return IJK{i, j, (int)(rem - (long)j * n)};
+}
+
+// parent (side n/2) of the cell at p plus the offset of neighbour u (u = 6: the cell itself)
+__device__ __forceinline__ long parent_nb(const IJK &p, int u, int n)
+{
+    const int i = p.i + (u == 0) - (u == 1), j = p.j + (u == 2) - (u == 3), k = p.k + (u == 4) - (u == 5);
+    const int h = n >> 1;
+    return ((long)(i >> 1) * h + (j >> 1)) * h + (k >> 1);
+}
+
+__device__ __forceinline__ float nbr_or_zero(uint32_t m, int u, const float *__restrict__ v, long c, int n)
+{
+    return ((m >> (2 * u)) & 3) == NB_FLUID ? v[c + nb_offset(u, n)] : 0.0f;
+}
+
+// ------------------------------------------------------------ hierarchy
+// coarse flags and level set from the 8 children
+__global__ void k_ml_coarsen(int Rc, int Rf, const int8_t *__restrict__ Af, const float *__restrict__ Lf, int8_t *Ac,
+                             float *Lc)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long Cc = (long)Rc * Rc * Rc;
+    if (g >= Cc) return;
+    int K = (int)(g % Rc), J = (int)((g / Rc) % Rc), I = (int)(g / ((long)Rc * Rc));
+    int fluid = 0, wall = 0, nw = 0;
+    float lf = 0.0f, ls = 0.0f;
+    for (int d = 0; d < 8; ++d) {
+        long f = ((long)(2 * I + (d >> 2)) * Rf + 2 * J + ((d >> 1) & 1)) * Rf + 2 * K + (d & 1);
+        int a = Af[f];
+        fluid += a == FLUID;
+        wall += a == WALL;
+        if (a == FLUID) lf += Lf[f];
+        if (a != WALL) {
+            ls += Lf[f];
+            nw++;
+        }
+    }
+    Ac[g] = fluid ? FLUID : (wall == 8 ? WALL : AIR);
+    Lc[g] = fluid ? lf / (float)fluid : (nw ? ls / (float)nw : 1.0f);
+}
+
+// neighbour metadata and Jacobi weights of a dense level (FLUID cells)
+__global__ void k_ml_meta(int R, const int8_t *__restrict__ A, const float *__restrict__ L, int subcell, double h2,
+                          uint32_t *meta, float *dinv)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long C = (long)R * R * R;
+    if (c >= C) return;
+    if (A[c] != FLUID) {
+        meta[c] = 0;
+        dinv[c] = 0.0f;
+        return;
+    }
+    int k = (int)(c % R), j = (int)((c / R) % R), i = (int)(c / ((long)R * R));
+    const bool inr[6] = {i + 1 <= R - 1, i - 1 >= 0, j + 1 <= R - 1, j - 1 >= 0, k + 1 <= R - 1, k - 1 >= 0};
+    uint32_t m = 0;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        int kd = NB_OUT;
+        if (inr[u]) {
+            int8_t a = A[c + nb_offset(u, R)];
+            kd = a == FLUID ? NB_FLUID : (a == WALL ? NB_WALL : NB_AIR);
+        }
+        m |= (uint32_t)kd << (2 * u);
+        if (kd == NB_AIR) m |= META_HAS_AIR;
+    }
+    meta[c] = m;
+    float diag = a_diag(R, c, A, L, subcell);
+    dinv[c] = diag > 0.0f ? (float)(h2 / (double)diag) : 0.0f;
+}
+
+// fine level: Jacobi weights h2 P^2 from the PCG diagonal on FLUID cells
+__global__ void k_ml_setup0(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fl,
+                            const double *__restrict__ P, double h2, float *dinv)
+{
+    const int nfl = iflags[I_NFL];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) {
+        long c = fl[t];
+        dinv[c] = (float)(h2 * P[c] * P[c]);
+    }
+}
+
+__global__ void k_ml_zero(long n, float *a)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0f;
+}
+
+// ------------------------------------------------------------ V-cycle
+// level 0 pre-smoothing: z1 = w D^-1 r, z2 = z1 + w D^-1 (r - A z1)
+__global__ __launch_bounds__(TBL) void k_ml_pre0(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+                                                 const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                                 int n, const float *__restrict__ L, int subcell, double h2,
+                                                 const float *__restrict__ r, const float *__restrict__ dinv,
+                                                 float *z2)
+{
+    if (fi[FI_DONE]) return;
+    const int nfl = iflags[I_NFL];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) {
+        long c = fl[t];
+        uint32_t m = meta[t];
+        float zv[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int kd = (m >> (2 * u)) & 3;
+            const long q = c + (kd == NB_FLUID ? nb_offset(u, n) : 0);
+            zv[u] = kd == NB_FLUID ? MG_W * dinv[q] * r[q] : 0.0f;
+        }
+        float rc = r[c], dc = dinv[c];
+        float z1 = MG_W * dc * rc;
+        float az = ax_row(m, zv, z1, c, n, L, subcell, h2);
+        z2[c] = z1 + MG_W * dc * (rc - az);
+    }
+}
+
+// dense level pre-smoothing from zero: t = w D^-1 b, x = t + w D^-1 (b - A t)
+__global__ __launch_bounds__(TBL) void k_ml_pre(const int32_t *__restrict__ fi, int R, const int8_t *__restrict__ A,
+                                                const uint32_t *__restrict__ meta, const float *__restrict__ L,
+                                                int subcell, double h2, const float *__restrict__ b,
+                                                const float *__restrict__ dinv, float *x)
+{
+    if (fi[FI_DONE]) return;
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long C = (long)R * R * R;
+    if (c >= C) return;
+    if (A[c] != FLUID) {
+        x[c] = 0.0f;
+        return;
+    }
+    uint32_t m = meta[c];
+    float tv[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (m >> (2 * u)) & 3;
+        const long q = c + (kd == NB_FLUID ? nb_offset(u, R) : 0);
+        tv[u] = kd == NB_FLUID ? MG_W * dinv[q] * b[q] : 0.0f;
+    }
+    float bc = b[c], dc = dinv[c];
+    float t = MG_W * dc * bc;
+    float at = ax_row(m, tv, t, c, R, L, subcell, h2);
+    x[c] = t + MG_W * dc * (bc - at);
+}
+
+// level 0 residual: res = r - A z2 on FLUID cells (res is zero elsewhere)
+__global__ __launch_bounds__(TBL) void k_ml_resid0(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+                                                   const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                                   int n, const float *__restrict__ L, int subcell, double h2,
+                                                   const float *__restrict__ r, const float *__restrict__ z2,
+                                                   float *res)
+{
+    if (fi[FI_DONE]) return;
+    const int nfl = iflags[I_NFL];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) {
+        const long c = fl[t];
+        const uint32_t m = meta[t];
+        float zv[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int kd = (m >> (2 * u)) & 3;
+            const float w = z2[c + (kd == NB_FLUID ? nb_offset(u, n) : 0)];
+            zv[u] = kd == NB_FLUID ? w : 0.0f;
+        }
+        res[c] = r[c] - ax_row(m, zv, z2[c], c, n, L, subcell, h2);
+    }
+}
+
+// dense level residual: res = b - A x on FLUID cells, 0 elsewhere
+__global__ __launch_bounds__(TBL) void k_ml_resid(const int32_t *__restrict__ fi, int R, const int8_t *__restrict__ A,
+                                                  const uint32_t *__restrict__ meta, const float *__restrict__ L,
+                                                  int subcell, double h2, const float *__restrict__ b,
+                                                  const float *__restrict__ x, float *res)
+{
+    if (fi[FI_DONE]) return;
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long C = (long)R * R * R;
+    if (c >= C) return;
+    if (A[c] != FLUID) {
+        res[c] = 0.0f;
+        return;
+    }
+    const uint32_t m = meta[c];
+    float xv[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (m >> (2 * u)) & 3;
+        const float w = x[c + (kd == NB_FLUID ? nb_offset(u, R) : 0)];
+        xv[u] = kd == NB_FLUID ? w : 0.0f;
+    }
+    res[c] = b[c] - ax_row(m, xv, x[c], c, R, L, subcell, h2);
+}
+
+// restriction: bc[I] = sum of the 8 children's residuals (zero off FLUID)
+__global__ __launch_bounds__(TBL) void k_ml_sum8(const int32_t *__restrict__ fi, int Rc, int Rf,
+                                                 const float *__restrict__ res, float *bc)
+{
+    if (fi[FI_DONE]) return;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long Cc = (long)Rc * Rc * Rc;
+    if (g >= Cc) return;
+    const IJK p = ijk_of(g, Rc);
+    const long f0 = ((long)(2 * p.i) * Rf + 2 * p.j) * Rf + 2 * p.k;
+    const long di = (long)Rf * Rf, dj = Rf;
+    const float2 a = *reinterpret_cast<const float2 *>(res + f0);
+    const float2 b = *reinterpret_cast<const float2 *>(res + f0 + dj);
+    const float2 c = *reinterpret_cast<const float2 *>(res + f0 + di);
+    const float2 d = *reinterpret_cast<const float2 *>(res + f0 + di + dj);
+    float sum = 0.0f;
+    sum += a.x;
+    sum += a.y;
+    sum += b.x;
+    sum += b.y;
+    sum += c.x;
+    sum += c.y;
+    sum += d.x;
+    sum += d.y;
+    bc[g] = sum;
+}
+
+// dense level post-smoothing, first sweep: x3 = x + xc(parent) / MG_S,
+// t = x3 + w D^-1 (b - A x3)
+__global__ __launch_bounds__(TBL) void k_ml_post_a(const int32_t *__restrict__ fi, int R,
+                                                   const int8_t *__restrict__ A, const uint32_t *__restrict__ meta,
+                                                   const float *__restrict__ L, int subcell, double h2,
+                                                   const float *__restrict__ b, const float *__restrict__ dinv,
+                                                   const float *__restrict__ x, const float *__restrict__ xc,
+                                                   float *t)
+{
+    if (fi[FI_DONE]) return;
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long C = (long)R * R * R;
+    if (c >= C) return;
+    if (A[c] != FLUID) {
+        t[c] = 0.0f;
+        return;
+    }
+    const float ws = 1.0f / MG_S;
+    uint32_t m = meta[c];
+    const IJK p = ijk_of(c, R);
+    float xv[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (m >> (2 * u)) & 3;
+        const bool f = kd == NB_FLUID;
+        const float xn = x[c + (f ? nb_offset(u, R) : 0)];
+        const float xp = xc[f ? parent_nb(p, u, R) : parent_nb(p, 6, R)];
+        xv[u] = f ? xn + ws * xp : 0.0f;
+    }
+    float x3 = x[c] + ws * xc[parent_nb(p, 6, R)];
+    float ax = ax_row(m, xv, x3, c, R, L, subcell, h2);
+    t[c] = x3 + MG_W * dinv[c] * (b[c] - ax);
+}
+
+// dense level post-smoothing, second sweep: x = t + w D^-1 (b - A t)
+__global__ __launch_bounds__(TBL) void k_ml_post_b(const int32_t *__restrict__ fi, int R,
+                                                   const int8_t *__restrict__ A, const uint32_t *__restrict__ meta,
+                                                   const float *__restrict__ L, int subcell, double h2,
+                                                   const float *__restrict__ b, const float *__restrict__ dinv,
+                                                   const float *__restrict__ t, float *x)
+{
+    if (fi[FI_DONE]) return;
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long C = (long)R * R * R;
+    if (c >= C) return;
+    if (A[c] != FLUID) {
+        x[c] = 0.0f;
+        return;
+    }
+    uint32_t m = meta[c];
+    float tv[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) tv[u] = nbr_or_zero(m, u, t, c, R);
+    float ax = ax_row(m, tv, t[c], c, R, L, subcell, h2);
+    x[c] = t[c] + MG_W * dinv[c] * (b[c] - ax);
+}
+
+// level 0 post-smoothing, first sweep: z3 = z2 + x1(parent) / MG_S,
+// z4 = z3 + w D^-1 (r - A z3)
+__global__ __launch_bounds__(TBL) void k_ml_post0_a(const int32_t *__restrict__ iflags,
+                                                    const int32_t *__restrict__ fi, const int32_t *__restrict__ fl,
+                                                    const uint32_t *__restrict__ meta, int n,
+                                                    const float *__restrict__ L, int subcell, double h2,
+                                                    const float *__restrict__ r, const float *__restrict__ dinv,
+                                                    const float *__restrict__ z2, const float *__restrict__ x1,
+                                                    float *z4)
+{
+    if (fi[FI_DONE]) return;
+    const int nfl = iflags[I_NFL];
+    const float ws = 1.0f / MG_S;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) {
+        long c = fl[t];
+        uint32_t m = meta[t];
+        const IJK p = ijk_of(c, n);
+        float zv[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int kd = (m >> (2 * u)) & 3;
+            const bool f = kd == NB_FLUID;
+            const float zn = z2[c + (f ? nb_offset(u, n) : 0)];
+            const float xp = x1[f ? parent_nb(p, u, n) : parent_nb(p, 6, n)];
+            zv[u] = f ? zn + ws * xp : 0.0f;
+        }
+        float z3 = z2[c] + ws * x1[parent_nb(p, 6, n)];
+        float az = ax_row(m, zv, z3, c, n, L, subcell, h2);
+        z4[c] = z3 + MG_W * dinv[c] * (r[c] - az);
+    }
+}
+
+// level 0 post-smoothing, second sweep: z = z4 + w D^-1 (r - A z4) into the
+// .x slots of the (z, s) buffer the next flexible-CG iteration reads (parity
+// (k + 1) & 1 with k = fi[FI_K]); partials z.r and, when q is given, z.q
+__global__ __launch_bounds__(TB) void k_ml_post0_b(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+                                                   const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                                   int n, const float *__restrict__ L, int subcell, double h2,
+                                                   const float *__restrict__ r, const float *__restrict__ dinv,
+                                                   const float *__restrict__ z4, const float *__restrict__ q,
+                                                   float2 *B0, float2 *B1, double *part)
+{
+    if (fi[FI_DONE]) return;
+    const int nfl = iflags[I_NFL];
+    float2 *B = ((fi[FI_K] + 1) & 1) ? B1 : B0;
+    double zr = 0.0, zq = 0.0;
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
+        long c = fl[t];
+        uint32_t m = meta[t];
+        float zv[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) zv[u] = nbr_or_zero(m, u, z4, c, n);
+        float rc = r[c];
+        float az = ax_row(m, zv, z4[c], c, n, L, subcell, h2);
+        float z = z4[c] + MG_W * dinv[c] * (rc - az);
+        B[c].x = z;
+        zr += (double)(z * rc);
+        if (q) zq += (double)(z * q[c]);
+    }
+    store_partials(part, 2, zr, zq, 0.0, 2);
+}
+
+// ------------------------------------------------------------ flexible CG
+// part a(k): z.r_k from the preconditioner partials, beta, s_k = z + beta
+// s_{k-1} (neighbours recomputed), q = A s_k, partials of q.s
+__global__ __launch_bounds__(TB) void k_fcg_a(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+                                              const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                              int n, const float *__restrict__ L, int subcell, double h2, float2 *B0,
+                                              float2 *B1, float *q, const double *__restrict__ pz, double *fs,
+                                              double *pa, int G)
+{
+    if (fi[FI_DONE]) return;
+    const int k = fi[FI_K];
+    const int nfl = iflags[I_NFL];
+    const float2 *__restrict__ rd = (k & 1) ? B0 : B1;
+    float2 *wr = (k & 1) ? B1 : B0;
+    double zr, zq;
+    sum_partials2(pz, G, zr, zq);
+    float beta = 0.0f;
+    if (k > 0) beta = (float)(-fs[F_AL0 + ((k - 1) & 1)] * zq / fs[F_ZR0 + ((k - 1) & 1)]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) fs[F_ZR0 + (k & 1)] = zr;
+    double qs = 0.0;
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
+        long c = fl[t];
+        uint32_t m = meta[t];
+        float2 v = rd[c];
+        float sv[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int kd = (m >> (2 * u)) & 3;
+            const float2 w = rd[c + (kd == NB_OUT ? 0 : nb_offset(u, n))];
+            sv[u] = kd == NB_FLUID ? (k == 0 ? w.x : w.x + beta * w.y) : 0.0f;
+        }
+        float ss = k == 0 ? v.x : v.x + beta * v.y;
+        float qc = ax_row(m, sv, ss, c, n, L, subcell, h2);
+        wr[c] = make_float2(v.x, ss);
+        q[c] = qc;
+        qs += (double)(qc * ss);
+    }
+    store_partials(pa, 1, qs, 0.0, 0.0, 1);
+}
+
+// part b(k): alpha = zr_k / (q.s); x += alpha s; r -= alpha q; partials r.r
+__global__ __launch_bounds__(TB) void k_fcg_b(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+                                              const int32_t *__restrict__ fl, float *x, float *r,
+                                              const float2 *__restrict__ B0, const float2 *__restrict__ B1,
+                                              const float *__restrict__ q, const double *__restrict__ pa, double *fs,
+                                              double *pb, int G)
+{
+    if (fi[FI_DONE]) return;
+    const int k = fi[FI_K];
+    const int nfl = iflags[I_NFL];
+    const float2 *__restrict__ bk = (k & 1) ? B1 : B0;
+    double qs = sum_partials(pa, G, 1, 0);
+    double alpha = fs[F_ZR0 + (k & 1)] / qs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) fs[F_AL0 + (k & 1)] = alpha;
+    const float a = (float)alpha, na = (float)-alpha;
+    double rr = 0.0;
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
+        long c = fl[t];
+        float xc = x[c], sc = bk[c].y, rc = r[c], qc = q[c];
+        x[c] = xc + a * sc;
+        rc = rc + na * qc;
+        r[c] = rc;
+        rr += (double)(rc * rc);
+    }
+    store_partials(pb, 1, rr, 0.0, 0.0, 1);
+}
+
+// single block: stop test after part b; advances the iteration index
+__global__ __launch_bounds__(TB) void k_fcg_check(const double *__restrict__ pb, double *fs, int32_t *fi, int G, int kmax)
+{
+    if (fi[FI_DONE]) return;
+    double rr = sum_partials(pb, G, 1, 0);
+    if (threadIdx.x == 0) {
+        const int k = fi[FI_K] + 1;
+        fs[F_RR] = rr;
+        fi[FI_K] = k;
+        const bool conv = rr <= fs[F_EPS];
+        if (conv || !(rr == rr) || k >= kmax) {
+            fi[FI_DONE] = 1;
+            fi[FI_CONV] = conv ? 1 : 0;
+        }
+    }
+}
+
+struct Level {
+    int R = 0;
+    long C = 0;
+    double h2 = 0.0;
+    int8_t *A = nullptr;
+    float *L = nullptr, *dinv = nullptr, *b = nullptr, *x = nullptr, *t = nullptr;
+    uint32_t *meta = nullptr;
+};
+
+template <class T>
+void dmalloc(T **p, long n)
+{
+    FLIP_CHECK(hipMalloc((void **)p, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+}
+
+}  // namespace
+
+// ------------------------------------------------- learned solver state
+struct LearnedSolver {
+    int n = 0, K = 0;
+    std::vector<Level> lv;  // lv[1..K]
+    UNet *net = nullptr;
+    float *dinv0 = nullptr, *z2 = nullptr, *z4 = nullptr, *res0 = nullptr;
+    float *sigma = nullptr;
+    double *part = nullptr;  // [4 * GMAX]: pa, pb, pz (stride 2)
+    double *fs = nullptr;
+    int32_t *fi = nullptr;
+    double *upart = nullptr;
+    int32_t *h_flag = nullptr;  // pinned: stop flags of the last two launches
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraphExec_t graph = nullptr;
+    int graph_subcell = -1;
+};
+
+void learned_free(LearnedSolver *ls)
+{
+    if (!ls) return;
+    if (ls->graph) (void)hipGraphExecDestroy(ls->graph);
+    if (ls->net) unet_free(ls->net);
+    for (Level &l : ls->lv) {
+        void *bufs[] = {l.A, l.L, l.dinv, l.b, l.x, l.t, l.meta};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+    }
+    void *bufs[] = {ls->dinv0, ls->z2, ls->z4, ls->res0, ls->sigma, ls->part, ls->fs, ls->fi, ls->upart};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (ls->h_flag) (void)hipHostFree(ls->h_flag);
+    for (hipEvent_t e : ls->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete ls;
+}
+
+// number of coarse levels for grid side n: halve while the next level is at
+// least MG_COARSE cells wide and a multiple of 8 (at least one level)
+int learned_levels(int n)
+{
+    int K = 1;
+    while ((n % (1 << (K + 1))) == 0 && (n >> (K + 1)) >= MG_COARSE && ((n >> (K + 1)) % 8) == 0) ++K;
+    return K;
+}
+
+static LearnedSolver *learned_get(FlipCtx *c)
+{
+    if (c->learned) return c->learned;
+    const int n = c->n;
+    if (n % 16 != 0) throw flip_arg_error{FLIP_E_ARG, "the learned preconditioner needs n to be a multiple of 16"};
+    if (c->learned_weights.empty())
+        throw flip_arg_error{FLIP_E_STATE, "FLIP_PRECOND_LEARNED needs weights (flip_gpu_set_learned_weights)"};
+    LearnedSolver *ls = new LearnedSolver();
+    try {
+        ls->n = n;
+        ls->K = learned_levels(n);
+        ls->lv.resize(ls->K + 1);
+        for (int l = 1; l <= ls->K; ++l) {
+            Level &L = ls->lv[l];
+            L.R = n >> l;
+            L.C = (long)L.R * L.R * L.R;
+            L.h2 = (double)(float)(1.0 / ((double)L.R * L.R));
+            dmalloc(&L.A, L.C);
+            dmalloc(&L.L, L.C);
+            dmalloc(&L.dinv, L.C);
+            dmalloc(&L.b, L.C);
+            dmalloc(&L.x, L.C);
+            dmalloc(&L.t, L.C);
+            dmalloc(&L.meta, L.C);
+        }
+        ls->net = unet_new(ls->lv[ls->K].R, 1, c->stream);
+        std::vector<float> w;
+        std::string err;
+        if (!unet_read_weights(c->learned_weights.c_str(), ls->net->nparam, w, err))
+            throw flip_arg_error{FLIP_E_ARG, err};
+        FLIP_CHECK(hipMemcpy(ls->net->pw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+        unet_refresh_bf16(ls->net);
+        const long C = c->C;
+        dmalloc(&ls->dinv0, C);
+        dmalloc(&ls->z2, C);
+        dmalloc(&ls->z4, C);
+        dmalloc(&ls->res0, C);
+        dmalloc(&ls->sigma, 4);
+        dmalloc(&ls->part, 4 * GMAX);
+        dmalloc(&ls->fs, 8);
+        dmalloc(&ls->fi, 4);
+        dmalloc(&ls->upart, 256);
+        FLIP_CHECK(hipHostMalloc((void **)&ls->h_flag, sizeof(int32_t) * 2));
+        FLIP_CHECK(hipEventCreateWithFlags(&ls->ev[0], hipEventDisableTiming));
+        FLIP_CHECK(hipEventCreateWithFlags(&ls->ev[1], hipEventDisableTiming));
+    } catch (...) {
+        learned_free(ls);
+        throw;
+    }
+    c->learned = ls;
+    return ls;
+}
+
+// z = M(r) into the (z, s) buffer of parity (fi[FI_K] + 1) & 1, partials (z.r,
+// z.q) in pz (q may be null)
+static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell, const float *q, double *pz)
+{
+    const int n = c->n, K = ls->K;
+    const int32_t *fi = ls->fi;
+    const unsigned G0 = (unsigned)std::min<long>(nb(c->C), 4096);  // level-0 smoothing grid (grid-stride)
+    hipLaunchKernelGGL(k_ml_pre0, G0, TBL, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
+                       ls->dinv0, ls->z2);
+    // down: restrict level l-1's residual into b_l, smooth level l
+    for (int l = 1; l <= K; ++l) {
+        Level &L = ls->lv[l];
+        if (l == 1) {
+            hipLaunchKernelGGL(k_ml_resid0, G0, TBL, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
+                               c->r, ls->z2, ls->res0);
+            hipLaunchKernelGGL(k_ml_sum8, nb(L.C), TBL, 0, c->stream, fi, L.R, n, ls->res0, L.b);
+        } else {
+            Level &F = ls->lv[l - 1];
+            hipLaunchKernelGGL(k_ml_resid, nb(F.C), TBL, 0, c->stream, fi, F.R, F.A, F.meta, F.L, subcell, F.h2, F.b,
+                               F.x, F.t);
+            hipLaunchKernelGGL(k_ml_sum8, nb(L.C), TBL, 0, c->stream, fi, L.R, F.R, F.t, L.b);
+        }
+        if (l < K)
+            hipLaunchKernelGGL(k_ml_pre, nb(L.C), TBL, 0, c->stream, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
+                               L.dinv, L.x);
+    }
+    Level &KL = ls->lv[K];
+    unet_solve(ls->net, KL.A, KL.L, KL.b, KL.x, ls->sigma, ls->upart);
+    // up: correct level l with level l+1, smooth
+    for (int l = K - 1; l >= 1; --l) {
+        Level &L = ls->lv[l];
+        hipLaunchKernelGGL(k_ml_post_a, nb(L.C), TBL, 0, c->stream, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
+                           L.dinv, L.x, ls->lv[l + 1].x, L.t);
+        hipLaunchKernelGGL(k_ml_post_b, nb(L.C), TBL, 0, c->stream, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
+                           L.dinv, L.t, L.x);
+    }
+    hipLaunchKernelGGL(k_ml_post0_a, G0, TBL, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
+                       c->r, ls->dinv0, ls->z2, ls->lv[1].x, ls->z4);
+    hipLaunchKernelGGL(k_ml_post0_b, G, TB, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
+                       ls->dinv0, ls->z4, q, c->zs[0], c->zs[1], pz);
+}
+
+// Flexible CG with the multilevel preconditioner from the current x (press)
+// and r = b - A x (c->r, from the PCG init); the FLUID list, P and iflags
+// (I_NFL) are current.  Returns the iterations run; *converged when the
+// recursive residual met eps.
+int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *converged)
+{
+    LearnedSolver *ls = learned_get(c);
+    const int n = c->n, K = ls->K;
+    double *pa = ls->part, *pb = ls->part + GMAX, *pz = ls->part + 2 * GMAX;
+    t_begin(c, "learned_setup");
+    // hierarchy of this system
+    hipLaunchKernelGGL(k_ml_zero, nb(c->C), TBL, 0, c->stream, c->C, ls->res0);
+    hipLaunchKernelGGL(k_ml_setup0, G, TBL, 0, c->stream, c->iflags, c->fl, c->P, h2, ls->dinv0);
+    for (int l = 1; l <= K; ++l) {
+        Level &L = ls->lv[l];
+        const int8_t *Af = l == 1 ? c->A : ls->lv[l - 1].A;
+        const float *Lf = l == 1 ? c->L : ls->lv[l - 1].L;
+        const int Rf = l == 1 ? n : ls->lv[l - 1].R;
+        hipLaunchKernelGGL(k_ml_coarsen, nb(L.C), TBL, 0, c->stream, L.R, Rf, Af, Lf, L.A, L.L);
+        hipLaunchKernelGGL(k_ml_meta, nb(L.C), TBL, 0, c->stream, L.R, L.A, L.L, subcell, L.h2, L.meta, L.dinv);
+    }
+    double hs[8] = {0, 0, 0, 0, eps, 0, 0, 0};
+    int32_t hf[4] = {0, 0, 0, 0};
+    FLIP_CHECK(hipMemcpyAsync(ls->fs, hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
+    FLIP_CHECK(hipMemcpyAsync(ls->fi, hf, sizeof(hf), hipMemcpyHostToDevice, c->stream));
+    vcycle(c, ls, G, h2, subcell, nullptr, pz);  // z_0 into B_1.x
+    t_end(c);
+
+    if (!ls->graph || ls->graph_subcell != subcell || c->profiling) {
+        if (ls->graph) {
+            (void)hipGraphExecDestroy(ls->graph);
+            ls->graph = nullptr;
+        }
+    }
+    auto iteration = [&]() {
+        launch_timed(c, "fcg_a", k_fcg_a, G, TB, c->iflags, ls->fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->zs[0],
+                     c->zs[1], c->q, pz, ls->fs, pa, G);
+        launch_timed(c, "fcg_b", k_fcg_b, G, TB, c->iflags, ls->fi, c->fl, c->press, c->r, c->zs[0], c->zs[1], c->q,
+                     pa, ls->fs, pb, G);
+        hipLaunchKernelGGL(k_fcg_check, 1, TB, 0, c->stream, pb, ls->fs, ls->fi, G, FCG_MAX);
+        vcycle(c, ls, G, h2, subcell, c->q, pz);
+    };
+    if (!c->profiling && !ls->graph) {
+        hipGraph_t g;
+        FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        iteration();
+        FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
+        FLIP_CHECK(hipGraphInstantiate(&ls->graph, g, nullptr, nullptr, 0));
+        FLIP_CHECK(hipGraphDestroy(g));
+        ls->graph_subcell = subcell;
+    }
+    // launch iteration k, then wait for iteration k-1's stop flag
+    for (int k = 0; k < FCG_MAX; ++k) {
+        if (c->profiling) {
+            t_begin(c, "learned_iteration");
+            iteration();
+            t_end(c);
+        } else {
+            FLIP_CHECK(hipGraphLaunch(ls->graph, c->stream));
+        }
+        FLIP_CHECK(hipMemcpyAsync(ls->h_flag + (k & 1), ls->fi + FI_DONE, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        FLIP_CHECK(hipEventRecord(ls->ev[k & 1], c->stream));
+        if (k > 0) {
+            FLIP_CHECK(hipEventSynchronize(ls->ev[(k - 1) & 1]));
+            if (ls->h_flag[(k - 1) & 1]) break;
+        }
+    }
+    int32_t hfi[4];
+    FLIP_CHECK(hipMemcpyAsync(hfi, ls->fi, sizeof(hfi), hipMemcpyDeviceToHost, c->stream));
+    FLIP_CHECK(hipStreamSynchronize(c->stream));
+    *converged = hfi[FI_CONV] != 0;
+    return hfi[FI_K];
+}

@@ -35,156 +35,18 @@
 // the triangular solves run as i+j+k hyperplane wavefronts; each cell's
 // formula and operand order are the reference's, so any topological order
 // gives identical results.
-#include "flip_device.hpp"
-#include "flip_internal.hpp"
-#include "unet_internal.hpp"
+#include "pcg_device.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
-using namespace flipdev;
+using namespace pcgdev;
 
 namespace {
 
-constexpr int TB = 1024;   // one 16-wave block per CU for the iteration kernels
-constexpr int GMAX = 256;  // grid of the iteration kernels (partials per reduction)
-constexpr int BATCH = 8;    // iterations per captured graph
-inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+constexpr int BATCH = 8;  // iterations per captured graph
 
-// device scalars: a_k lives in S_A0 + (k & 1)
-enum { S_A0 = 0, S_A1, S_ALPHA, S_BETA, S_ERR2, S_ERR2_0, S_BNORM2, S_EPS };
-// device int flags; I_KA is the next pcg_a iteration, I_KB the next pcg_b one
-enum { I_DONE = 0, I_ITERS, I_UNIT, I_CAPPED, I_TOLPOL, I_CAP, I_NFL, I_KA, I_KB };
-
-// neighbour kinds in the metadata word, 2 bits each in x_ref order
-// (+i, -i, +j, -j, +k, -k), solver.cpp:19-26 / 161-163
-enum { NB_FLUID = 0, NB_WALL = 1, NB_AIR = 2, NB_OUT = 3 };
-constexpr uint32_t META_HAS_AIR = 1u << 12;
-
-// Block-wide sum of one double per thread, fixed order, result in every thread.
-__device__ __forceinline__ double block_sum_all(double v)
-{
-    __shared__ double lds[TB / 64];
-    v = wave_sum(v);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-#pragma unroll
-    for (int i = 0; i < TB / 64; ++i) r += lds[i];
-    return r;
-}
-
-// Sum of cnt partials p[i*stride + off], identical in every block.
-__device__ __forceinline__ double sum_partials(const double *__restrict__ p, int cnt, int stride, int off)
-{
-    double a = 0.0;
-    for (int i = threadIdx.x; i < cnt; i += TB) a += p[i * stride + off];
-    return block_sum_all(a);
-}
-
-__device__ __forceinline__ void store_partials(double *p, int stride, double v0, double v1, double v2, int nv)
-{
-    double a = block_sum_all(v0);
-    double b = nv > 1 ? block_sum_all(v1) : 0.0;
-    double c = nv > 2 ? block_sum_all(v2) : 0.0;
-    if (threadIdx.x == 0) {
-        p[blockIdx.x * stride + 0] = a;
-        if (nv > 1) p[blockIdx.x * stride + 1] = b;
-        if (nv > 2) p[blockIdx.x * stride + 2] = c;
-    }
-}
-
-// solver.cpp:331-337 stop test after `it` completed iterations
-__device__ __forceinline__ bool stop_test(double rr, int it, const int32_t *iflags, const double *scal, bool &capped)
-{
-    bool stop = iflags[I_TOLPOL] == 0 ? (double)(float)rr <= scal[S_EPS] : rr <= scal[S_EPS];
-    if (!(rr == rr)) stop = true;  // NaN: report instead of running to the cap
-    capped = !stop && it >= iflags[I_CAP];
-    return stop || capped;
-}
-
-__device__ __forceinline__ long nb_offset(int u, int n)
-{
-    long nn = (long)n * n;
-    switch (u) {
-    case 0: return nn;
-    case 1: return -nn;
-    case 2: return n;
-    case 3: return -n;
-    case 4: return 1;
-    default: return -1;
-    }
-}
-
-// solver.cpp:19-26 x_ref of neighbour u (kind kd) for FLUID cell c, value xs
-__device__ __forceinline__ float x_ref(int kd, float xn, float xs, long c, long off, float Lc,
-                                       const float *__restrict__ L, int subcell)
-{
-    if (kd == NB_FLUID) return xn;
-    if (kd != NB_AIR) return xs;  // WALL, or clamped to the cell itself
-    return subcell ? (float)((double)L[c + off] / fmin(1.0e-6, (double)Lc) * (double)xs) : 0.0f;
-}
-
-// solver.cpp:29-41 one row of compute_Ax; xv[u] holds FLUID neighbours' values
-__device__ __forceinline__ float ax_row(uint32_t meta, const float xv[6], float xs, long c, int n,
-                                        const float *__restrict__ L, int subcell, double h2)
-{
-    double acc = 6.0 * (double)xs;
-    float Lc = (meta & META_HAS_AIR) ? L[c] : 0.0f;
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-        int kd = (meta >> (2 * u)) & 3;
-        acc -= (double)x_ref(kd, xv[u], xs, c, nb_offset(u, n), Lc, L, subcell);
-    }
-    return (float)(acc / h2);
-}
-
-// solver.cpp:103-117 A_diag (FLOAT accumulation, neighbour order -i,+i,-j,+j,-k,+k)
-__device__ __forceinline__ float a_diag(int n, long c, const int8_t *__restrict__ A, const float *__restrict__ L,
-                                        int subcell)
-{
-    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
-    long nn = (long)n * n;
-    float diag = 6.0f;
-    float Lc = L[c];
-    const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        int qi = i + di[m], qj = j + dj[m], qk = k + dk[m];
-        if (qi < 0 || qi > n - 1 || qj < 0 || qj > n - 1 || qk < 0 || qk > n - 1) {
-            diag = (float)((double)diag - 1.0);
-            continue;
-        }
-        long q = c + di[m] * nn + dj[m] * n + dk[m];
-        int8_t aq = A[q];
-        if (aq == WALL) diag = (float)((double)diag - 1.0);
-        else if (aq == AIR && subcell) diag = (float)((double)diag - (double)L[q] / fmin(1.0e-6, (double)Lc));
-    }
-    return diag;
-}
-
-// Per-cell preconditioner when no P == 1.0 exists: each coupling term
-// A_ref*P_ref(P)*P_ref(.) is -0 for an in-range FLUID neighbour and +0
-// otherwise (solver.cpp:147-176); the subtractions keep the signed zeros.
-__device__ __forceinline__ double coupling_zero(uint32_t meta, int u)
-{
-    return ((meta >> (2 * u)) & 3) == NB_FLUID ? -0.0 : 0.0;
-}
-
-__device__ __forceinline__ float precond_cell(uint32_t meta, float r, double P)
-{
-    double t = (double)r;
-    t = t - coupling_zero(meta, 1);  // -i
-    t = t - coupling_zero(meta, 3);  // -j
-    t = t - coupling_zero(meta, 5);  // -k
-    double q = t * P;
-    double u = q;
-    u = u - coupling_zero(meta, 0);  // +i
-    u = u - coupling_zero(meta, 2);  // +j
-    u = u - coupling_zero(meta, 4);  // +k
-    return (float)(u * P);
-}
 
 // ---------------------------------------------------------------- setup
 __global__ void k_negate(long C, float *b)
@@ -288,9 +150,68 @@ __global__ __launch_bounds__(TB) void k_pcg_init_fin(int G, const double *__rest
     }
 }
 
+// One FLUID cell's operands for the stencil, all loads issued together: the
+// cell's own (z, s) pair, the six neighbours' pairs (a non-FLUID neighbour's
+// slot is in range and ignored; outside the grid the cell's own slot is read)
+// and, for a cell next to AIR, the level set of the cell and of its
+// neighbours.
+struct StencilItem {
+    long c;
+    uint32_t m;
+    float2 v;
+    float2 w[6];
+    float Lc;
+    float Ln[6];
+};
+
+__device__ __forceinline__ void stencil_load(StencilItem &it, int t, const int32_t *__restrict__ fl,
+                                             const uint32_t *__restrict__ meta, const float2 *__restrict__ rd,
+                                             const float *__restrict__ L, int n)
+{
+    it.c = fl[t];
+    it.m = meta[t];
+    it.v = rd[it.c];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (it.m >> (2 * u)) & 3;
+        it.w[u] = rd[it.c + (kd == NB_OUT ? 0 : nb_offset(u, n))];
+    }
+    it.Lc = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) it.Ln[u] = 0.0f;
+    if (it.m & META_HAS_AIR) {
+        it.Lc = L[it.c];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int kd = (it.m >> (2 * u)) & 3;
+            it.Ln[u] = L[it.c + (kd == NB_AIR ? nb_offset(u, n) : 0)];
+        }
+    }
+}
+
+// solver.cpp:19-41 compute_Ax row of s = z + beta*s_old (k > 0) or s = z (k == 0)
+__device__ __forceinline__ float stencil_row(const StencilItem &it, int k, float beta, int subcell, double h2,
+                                             float &ss)
+{
+    ss = k == 0 ? it.v.x : it.v.x + beta * it.v.y;
+    double acc = 6.0 * (double)ss;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (it.m >> (2 * u)) & 3;
+        float xr;
+        if (kd == NB_FLUID) xr = k == 0 ? it.w[u].x : it.w[u].x + beta * it.w[u].y;
+        else if (kd != NB_AIR) xr = ss;  // WALL, or clamped to the cell itself
+        else xr = subcell ? (float)((double)it.Ln[u] / fmin(1.0e-6, (double)it.Lc) * (double)ss) : 0.0f;
+        acc -= (double)xr;
+    }
+    return (float)(acc / h2);
+}
+
 // iteration part a (see the file comment).  Buffer B_p holds (z, s) pairs:
 // pcg_a(k) reads (z, s_{k-1}) from B_{(k+1)&1} and writes s_k into B_{k&1}.y;
 // pcg_b(k) reads s_k from B_{k&1}.y and writes the new z into B_{k&1}.x.
+// The pair is written whole (its .x, about to be replaced by pcg_b, gets the
+// current z) so every written line is complete.
 __global__ __launch_bounds__(TB) void k_pcg_a(const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
                                               int n, const float *__restrict__ L, int subcell, double h2,
                                               float2 *B0, float2 *B1, float *q, const double *__restrict__ pb,
@@ -303,18 +224,12 @@ __global__ __launch_bounds__(TB) void k_pcg_a(const int32_t *__restrict__ fl, co
     float2 *wr = (k & 1) ? B1 : B0;
     // the first item's loads go out before the partial-sum prologue
     const int t0 = blockIdx.x * TB + threadIdx.x;
-    long c0 = 0;
-    uint32_t m0 = 0;
-    float2 v0 = make_float2(0.0f, 0.0f);
-    if (t0 < nfl) {
-        c0 = fl[t0];
-        m0 = meta[t0];
-        v0 = rd[c0];
-    }
+    StencilItem it;
+    if (t0 < nfl) stencil_load(it, t0, fl, meta, rd, L, n);
     float beta = 0.0f;
     if (k > 0) {
-        double rr = sum_partials(pb, G, 2, 0);
-        double zr = sum_partials(pb, G, 2, 1);
+        double rr, zr;
+        sum_partials2(pb, G, rr, zr);
         bool capped;
         bool stop = stop_test(rr, k, iflags, scal, capped);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -337,31 +252,11 @@ __global__ __launch_bounds__(TB) void k_pcg_a(const int32_t *__restrict__ fl, co
     }
     double qs = 0.0;
     for (int t = t0; t < nfl; t += gridDim.x * TB) {
-        long c;
-        uint32_t m;
-        float2 v;
-        if (t == t0) {
-            c = c0;
-            m = m0;
-            v = v0;
-        } else {
-            c = fl[t];
-            m = meta[t];
-            v = rd[c];
-        }
-        float sv[6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            sv[u] = 0.0f;
-            if (((m >> (2 * u)) & 3) == NB_FLUID) {
-                float2 w = rd[c + nb_offset(u, n)];
-                sv[u] = k == 0 ? w.x : w.x + beta * w.y;
-            }
-        }
-        float ss = k == 0 ? v.x : v.x + beta * v.y;
-        float qc = ax_row(m, sv, ss, c, n, L, subcell, h2);
-        wr[c].y = ss;
-        q[c] = qc;
+        if (t != t0) stencil_load(it, t, fl, meta, rd, L, n);
+        float ss;
+        const float qc = stencil_row(it, k, beta, subcell, h2, ss);
+        wr[it.c] = make_float2(it.v.x, ss);
+        q[it.c] = qc;
         qs += (double)(qc * ss);
     }
     store_partials(pa, 1, qs, 0.0, 0.0, 1);
@@ -381,13 +276,20 @@ __global__ __launch_bounds__(TB) void k_pcg_b(const int32_t *__restrict__ fl, co
     float2 *bk = (k & 1) ? B1 : B0;
     const int t0 = blockIdx.x * TB + threadIdx.x;
     long c0 = 0;
-    float x0 = 0.0f, sc0 = 0.0f, r0 = 0.0f, q0 = 0.0f;
+    uint32_t m0 = 0;
+    float x0 = 0.0f, r0 = 0.0f, q0 = 0.0f;
+    float2 b0 = make_float2(0.0f, 0.0f);
+    double P0 = 0.0;
     if (t0 < nfl) {
         c0 = fl[t0];
         x0 = x[c0];
-        sc0 = bk[c0].y;
+        b0 = bk[c0];
         r0 = r[c0];
         q0 = q[c0];
+        if (JACOBI) {
+            m0 = meta[t0];
+            P0 = P[c0];
+        }
     }
     double qs = sum_partials(pa, G, 1, 0);
     const double alpha = scal[S_A0 + (k & 1)] / qs;
@@ -399,27 +301,34 @@ __global__ __launch_bounds__(TB) void k_pcg_b(const int32_t *__restrict__ fl, co
     double rr = 0.0, zr = 0.0;
     for (int t = t0; t < nfl; t += gridDim.x * TB) {
         long c;
-        float xc, sc, rc, qc;
+        uint32_t m;
+        float xc, rc, qc;
+        float2 bc;
+        double Pc;
         if (t == t0) {
             c = c0;
             xc = x0;
-            sc = sc0;
+            bc = b0;
             rc = r0;
             qc = q0;
+            m = m0;
+            Pc = P0;
         } else {
             c = fl[t];
             xc = x[c];
-            sc = bk[c].y;
+            bc = bk[c];
             rc = r[c];
             qc = q[c];
+            m = JACOBI ? meta[t] : 0u;
+            Pc = JACOBI ? P[c] : 0.0;
         }
-        x[c] = xc + a * sc;
+        x[c] = xc + a * bc.y;
         rc = rc + na * qc;
         r[c] = rc;
         rr += (double)(rc * rc);
         if (JACOBI) {
-            float zc = precond_cell(meta[t], rc, P[c]);
-            bk[c].x = zc;
+            float zc = precond_cell(m, rc, Pc);
+            bk[c] = make_float2(zc, bc.y);
             zr += (double)(zc * rc);
         }
     }
@@ -446,7 +355,8 @@ __global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb,
 {
     if (iflags[I_DONE]) return;
     const int k = iflags[I_KA];  // completed iterations
-    double rr = sum_partials(pb, G, 2, 0);
+    double rr, zr_unused;
+    sum_partials2(pb, G, rr, zr_unused);
     bool capped;
     bool stop = stop_test(rr, k, iflags, scal, capped);
     if (threadIdx.x == 0) {
@@ -565,225 +475,6 @@ __global__ void k_fill_f64(long n, double *a, double v)
 }
 
 
-// ------------------------------------------- learned two-grid flexible CG
-// Preconditioner z = M(r), applied once per iteration of a flexible CG
-// (Polak-Ribiere beta = -alpha_{k-1} (z_k.q_{k-1}) / (z_{k-1}.r_{k-1})):
-//   z1 = w D^-1 r;  z2 = z1 + w D^-1 (r - A z1)                 pre-smoothing
-//   rc = sum over each coarse cell's FLUID children of (r - A z2)  restriction
-//   ec ~ Ac^-1 rc by the U-Net on the (n/2)^3 rediscretised system
-//   z3 = z2 + ec(parent)/TG_S;  z4 = z3 + w D^-1 (r - A z3);
-//   z  = z4 + w D^-1 (r - A z4)                                 post-smoothing
-// Coarse cells are FLUID if any child is, WALL if all are, AIR otherwise;
-// the coarse level set is the mean over the FLUID children of a FLUID cell
-// (negative, so the ghost-fluid diagonal stays positive) and over the
-// non-WALL children of an AIR cell.
-constexpr float TG_W = 0.8f;    // damped Jacobi weight
-constexpr float TG_S = 10.0f;   // aggregation / rediscretisation scale of the coarse correction
-constexpr int FCG_MAX = 80;     // flexible iterations before falling back to the plain PCG
-enum { F_ZR0 = 0, F_ZR1, F_AL0, F_AL1 };
-
-__global__ void k_tg_setup(int nfl, const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
-                           const double *__restrict__ P, double h2, uint32_t *cmeta, float *dinv)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nfl) return;
-    long c = fl[t];
-    cmeta[c] = meta[t];
-    dinv[c] = (float)(h2 * P[c] * P[c]);
-}
-
-__global__ void k_tg_coarse(int Rc, int n, const int8_t *__restrict__ A, const float *__restrict__ L, int8_t *Ac,
-                            float *Lc)
-{
-    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long Cc = (long)Rc * Rc * Rc;
-    if (g >= Cc) return;
-    int K = (int)(g % Rc), J = (int)((g / Rc) % Rc), I = (int)(g / ((long)Rc * Rc));
-    int fluid = 0, wall = 0, nw = 0;
-    float lf = 0.0f, ls = 0.0f;
-    for (int d = 0; d < 8; ++d) {
-        long f = ((long)(2 * I + (d >> 2)) * n + 2 * J + ((d >> 1) & 1)) * n + 2 * K + (d & 1);
-        int a = A[f];
-        fluid += a == FLUID;
-        wall += a == WALL;
-        if (a == FLUID) lf += L[f];
-        if (a != WALL) {
-            ls += L[f];
-            nw++;
-        }
-    }
-    Ac[g] = fluid ? FLUID : (wall == 8 ? WALL : AIR);
-    Lc[g] = fluid ? lf / (float)fluid : (nw ? ls / (float)nw : 1.0f);
-}
-
-// z2 = z1 + w D^-1 (r - A z1), z1 = w D^-1 r formed on the fly
-__global__ void k_tg_smooth1(int nfl, const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta, int n,
-                             const float *__restrict__ L, int subcell, double h2, const float *__restrict__ r,
-                             const float *__restrict__ dinv, float *z2)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nfl) return;
-    long c = fl[t];
-    uint32_t m = meta[t];
-    float zv[6];
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-        zv[u] = 0.0f;
-        if (((m >> (2 * u)) & 3) == NB_FLUID) {
-            long q = c + nb_offset(u, n);
-            zv[u] = TG_W * dinv[q] * r[q];
-        }
-    }
-    float rc = r[c], dc = dinv[c];
-    float z1 = TG_W * dc * rc;
-    float az = ax_row(m, zv, z1, c, n, L, subcell, h2);
-    z2[c] = z1 + TG_W * dc * (rc - az);
-}
-
-// rc = sum over FLUID children of (r - A z2)
-__global__ void k_tg_restrict(int Rc, int n, const int8_t *__restrict__ A, const uint32_t *__restrict__ cmeta,
-                              const float *__restrict__ L, int subcell, double h2, const float *__restrict__ r,
-                              const float *__restrict__ z2, float *rc)
-{
-    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long Cc = (long)Rc * Rc * Rc;
-    if (g >= Cc) return;
-    int K = (int)(g % Rc), J = (int)((g / Rc) % Rc), I = (int)(g / ((long)Rc * Rc));
-    float sum = 0.0f;
-    for (int d = 0; d < 8; ++d) {
-        long f = ((long)(2 * I + (d >> 2)) * n + 2 * J + ((d >> 1) & 1)) * n + 2 * K + (d & 1);
-        if (A[f] != FLUID) continue;
-        uint32_t m = cmeta[f];
-        float zv[6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) zv[u] = ((m >> (2 * u)) & 3) == NB_FLUID ? z2[f + nb_offset(u, n)] : 0.0f;
-        sum += r[f] - ax_row(m, zv, z2[f], f, n, L, subcell, h2);
-    }
-    rc[g] = sum;
-}
-
-__device__ __forceinline__ long tg_parent(long c, int n)
-{
-    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
-    int Rc = n / 2;
-    return ((long)(i >> 1) * Rc + (j >> 1)) * Rc + (k >> 1);
-}
-
-// z4 = z3 + w D^-1 (r - A z3), z3 = z2 + ec(parent) / TG_S on the fly
-__global__ void k_tg_smooth2(int nfl, const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta, int n,
-                             const float *__restrict__ L, int subcell, double h2, const float *__restrict__ r,
-                             const float *__restrict__ dinv, const float *__restrict__ z2,
-                             const float *__restrict__ ec, float *z4)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nfl) return;
-    long c = fl[t];
-    uint32_t m = meta[t];
-    const float ws = 1.0f / TG_S;
-    float zv[6];
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-        zv[u] = 0.0f;
-        if (((m >> (2 * u)) & 3) == NB_FLUID) {
-            long q = c + nb_offset(u, n);
-            zv[u] = z2[q] + ws * ec[tg_parent(q, n)];
-        }
-    }
-    float z3 = z2[c] + ws * ec[tg_parent(c, n)];
-    float az = ax_row(m, zv, z3, c, n, L, subcell, h2);
-    z4[c] = z3 + TG_W * dinv[c] * (r[c] - az);
-}
-
-// z = z4 + w D^-1 (r - A z4) into B.x; partials z.r and z.q (q may be null)
-__global__ __launch_bounds__(TB) void k_tg_smooth3(int nfl, const int32_t *__restrict__ fl,
-                                                   const uint32_t *__restrict__ meta, int n,
-                                                   const float *__restrict__ L, int subcell, double h2,
-                                                   const float *__restrict__ r, const float *__restrict__ dinv,
-                                                   const float *__restrict__ z4, const float *__restrict__ q,
-                                                   float2 *B, double *part)
-{
-    double zr = 0.0, zq = 0.0;
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        uint32_t m = meta[t];
-        float zv[6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) zv[u] = ((m >> (2 * u)) & 3) == NB_FLUID ? z4[c + nb_offset(u, n)] : 0.0f;
-        float rc = r[c];
-        float az = ax_row(m, zv, z4[c], c, n, L, subcell, h2);
-        float z = z4[c] + TG_W * dinv[c] * (rc - az);
-        B[c].x = z;
-        zr += (double)(z * rc);
-        if (q) zq += (double)(z * q[c]);
-    }
-    store_partials(part, 2, zr, zq, 0.0, 2);
-}
-
-// flexible CG part a(k): zr_k from the preconditioner partials, beta, s_k =
-// z + beta s_{k-1} (neighbours recomputed), q = A s_k, partials of q.s
-__global__ __launch_bounds__(TB) void k_fcg_a(int k, int nfl, const int32_t *__restrict__ fl,
-                                              const uint32_t *__restrict__ meta, int n, const float *__restrict__ L,
-                                              int subcell, double h2, float2 *B0, float2 *B1, float *q,
-                                              const double *__restrict__ pz, double *fs, double *pa, int G)
-{
-    const float2 *__restrict__ rd = (k & 1) ? B0 : B1;
-    float2 *wr = (k & 1) ? B1 : B0;
-    double zr = sum_partials(pz, G, 2, 0);
-    float beta = 0.0f;
-    if (k > 0) {
-        double zq = sum_partials(pz, G, 2, 1);
-        double b = -fs[F_AL0 + ((k - 1) & 1)] * zq / fs[F_ZR0 + ((k - 1) & 1)];
-        beta = (float)b;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) fs[F_ZR0 + (k & 1)] = zr;
-    double qs = 0.0;
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        uint32_t m = meta[t];
-        float2 v = rd[c];
-        float sv[6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            sv[u] = 0.0f;
-            if (((m >> (2 * u)) & 3) == NB_FLUID) {
-                float2 w = rd[c + nb_offset(u, n)];
-                sv[u] = k == 0 ? w.x : w.x + beta * w.y;
-            }
-        }
-        float ss = k == 0 ? v.x : v.x + beta * v.y;
-        float qc = ax_row(m, sv, ss, c, n, L, subcell, h2);
-        wr[c].y = ss;
-        q[c] = qc;
-        qs += (double)(qc * ss);
-    }
-    store_partials(pa, 1, qs, 0.0, 0.0, 1);
-}
-
-// flexible CG part b(k): alpha = zr_k / (q.s); x += alpha s; r -= alpha q; partials r.r
-__global__ __launch_bounds__(TB) void k_fcg_b(int k, int nfl, const int32_t *__restrict__ fl, float *x, float *r,
-                                              const float2 *__restrict__ B0, const float2 *__restrict__ B1,
-                                              const float *__restrict__ q, const double *__restrict__ pa, double *fs,
-                                              double *pb, int G)
-{
-    const float2 *__restrict__ bk = (k & 1) ? B1 : B0;
-    double qs = sum_partials(pa, G, 1, 0);
-    double alpha = fs[F_ZR0 + (k & 1)] / qs;
-    if (blockIdx.x == 0 && threadIdx.x == 0) fs[F_AL0 + (k & 1)] = alpha;
-    const float a = (float)alpha, na = (float)-alpha;
-    double rr = 0.0;
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        x[c] = x[c] + a * bk[c].y;
-        float rc = r[c] + na * q[c];
-        r[c] = rc;
-        rr += (double)(rc * rc);
-    }
-    store_partials(pb, 1, rr, 0.0, 0.0, 1);
-}
-
-// grid of the iteration kernels for a context (fixed so graphs stay valid)
-int pcg_grid(const FlipCtx *c) { return (int)std::min<unsigned>(nblk(c->C, TB), GMAX); }
-
 void launch_pair(FlipCtx *c, bool jacobi, int G, double h2, int subcell, double *pa, double *pb)
 {
     int n = c->n;
@@ -823,136 +514,9 @@ static void precond_wavefront(FlipCtx *c, int mode, const float *r, float *z)
         hipLaunchKernelGGL(k_ic_bwd_plane, nb, TB, 0, c->stream, n, d, mode, c->A, c->P, c->qv, z);
 }
 
-
-// ------------------------------------------------- learned solver state
-struct LearnedSolver {
-    int n = 0, Rc = 0;
-    UNet *net = nullptr;
-    uint32_t *cmeta = nullptr;
-    float *dinv = nullptr, *z2 = nullptr, *z4 = nullptr;
-    int8_t *Ac = nullptr;
-    float *Lc = nullptr, *rc = nullptr, *ec = nullptr, *sigma = nullptr;
-    double *part = nullptr;  // [4 * GMAX]: pa, pb, pz (stride 2)
-    double *fs = nullptr;
-    double *upart = nullptr;
-    double *h_rr = nullptr;  // pinned copy of the r.r partials
-};
-
-void learned_free(LearnedSolver *ls)
-{
-    if (!ls) return;
-    if (ls->net) unet_free(ls->net);
-    void *bufs[] = {ls->cmeta, ls->dinv, ls->z2, ls->z4, ls->Ac, ls->Lc, ls->rc, ls->ec, ls->sigma,
-                    ls->part, ls->fs, ls->upart};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
-    if (ls->h_rr) (void)hipHostFree(ls->h_rr);
-    delete ls;
-}
-
-static LearnedSolver *learned_get(FlipCtx *c)
-{
-    if (c->learned) return c->learned;
-    const int n = c->n;
-    if (n % 16 != 0) throw flip_arg_error{FLIP_E_ARG, "the learned preconditioner needs n to be a multiple of 16"};
-    if (c->learned_weights.empty())
-        throw flip_arg_error{FLIP_E_STATE, "FLIP_PRECOND_LEARNED needs weights (flip_gpu_set_learned_weights)"};
-    LearnedSolver *ls = new LearnedSolver();
-    try {
-        ls->n = n;
-        ls->Rc = n / 2;
-        ls->net = unet_new(ls->Rc, 1, c->stream);
-        std::vector<float> w;
-        std::string err;
-        if (!unet_read_weights(c->learned_weights.c_str(), ls->net->nparam, w, err))
-            throw flip_arg_error{FLIP_E_ARG, err};
-        FLIP_CHECK(hipMemcpy(ls->net->pw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
-        unet_refresh_bf16(ls->net);
-        const long C = c->C, Cc = (long)ls->Rc * ls->Rc * ls->Rc;
-        FLIP_CHECK(hipMalloc((void **)&ls->cmeta, sizeof(uint32_t) * C));
-        FLIP_CHECK(hipMalloc((void **)&ls->dinv, sizeof(float) * C));
-        FLIP_CHECK(hipMalloc((void **)&ls->z2, sizeof(float) * C));
-        FLIP_CHECK(hipMalloc((void **)&ls->z4, sizeof(float) * C));
-        FLIP_CHECK(hipMalloc((void **)&ls->Ac, Cc));
-        FLIP_CHECK(hipMalloc((void **)&ls->Lc, sizeof(float) * Cc));
-        FLIP_CHECK(hipMalloc((void **)&ls->rc, sizeof(float) * Cc));
-        FLIP_CHECK(hipMalloc((void **)&ls->ec, sizeof(float) * Cc));
-        FLIP_CHECK(hipMalloc((void **)&ls->sigma, sizeof(float) * 4));
-        FLIP_CHECK(hipMalloc((void **)&ls->part, sizeof(double) * 4 * GMAX));
-        FLIP_CHECK(hipMalloc((void **)&ls->fs, sizeof(double) * 8));
-        FLIP_CHECK(hipMalloc((void **)&ls->upart, sizeof(double) * 256));
-        FLIP_CHECK(hipHostMalloc((void **)&ls->h_rr, sizeof(double) * GMAX));
-    } catch (...) {
-        learned_free(ls);
-        throw;
-    }
-    c->learned = ls;
-    return ls;
-}
-
-// z = M(r) into B.x with partials (z.r, z.q) in pz
-static void tg_apply(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell, const float *q, float2 *B,
-                     double *pz)
-{
-    const int n = c->n, nfl = c->nfl;
-    const long Cc = (long)ls->Rc * ls->Rc * ls->Rc;
-    hipLaunchKernelGGL(k_tg_smooth1, nblk(nfl, 256), 256, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
-                       c->r, ls->dinv, ls->z2);
-    hipLaunchKernelGGL(k_tg_restrict, nblk(Cc, 256), 256, 0, c->stream, ls->Rc, n, c->A, ls->cmeta, c->L, subcell,
-                       h2, c->r, ls->z2, ls->rc);
-    unet_solve(ls->net, ls->Ac, ls->Lc, ls->rc, ls->ec, ls->sigma, ls->upart);
-    hipLaunchKernelGGL(k_tg_smooth2, nblk(nfl, 256), 256, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
-                       c->r, ls->dinv, ls->z2, ls->ec, ls->z4);
-    hipLaunchKernelGGL(k_tg_smooth3, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
-                       ls->dinv, ls->z4, q, B, pz);
-}
-
-// Flexible CG with the two-grid preconditioner from the current x and r
-// (r = b - A x); returns the iterations run, *converged when the recursive
-// residual met eps.
-static int fcg_learned(FlipCtx *c, int G, double h2, int subcell, double eps, bool *converged)
-{
-    LearnedSolver *ls = learned_get(c);
-    const int n = c->n, nfl = c->nfl;
-    const long Cc = (long)ls->Rc * ls->Rc * ls->Rc;
-    double *pa = ls->part, *pb = ls->part + GMAX, *pz = ls->part + 2 * GMAX;
-    t_begin(c, "learned_setup");
-    hipLaunchKernelGGL(k_tg_setup, nblk(nfl, 256), 256, 0, c->stream, nfl, c->fl, c->flmeta, c->P, h2, ls->cmeta,
-                       ls->dinv);
-    hipLaunchKernelGGL(k_tg_coarse, nblk(Cc, 256), 256, 0, c->stream, ls->Rc, n, c->A, c->L, ls->Ac, ls->Lc);
-    tg_apply(c, ls, G, h2, subcell, nullptr, c->zs[1], pz);
-    t_end(c);
-    *converged = false;
-    int k = 0;
-    for (; k < FCG_MAX; ++k) {
-        t_begin(c, "learned_iteration");
-        hipLaunchKernelGGL(k_fcg_a, G, TB, 0, c->stream, k, nfl, c->fl, c->flmeta, n, c->L, subcell, h2, c->zs[0],
-                           c->zs[1], c->q, pz, ls->fs, pa, G);
-        hipLaunchKernelGGL(k_fcg_b, G, TB, 0, c->stream, k, nfl, c->fl, c->press, c->r, c->zs[0], c->zs[1], c->q,
-                           pa, ls->fs, pb, G);
-        FLIP_CHECK(hipMemcpyAsync(ls->h_rr, pb, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        double rr = 0.0;
-        for (int i = 0; i < G; ++i) rr += ls->h_rr[i];
-        if (!(rr == rr)) {
-            t_end(c);
-            ++k;
-            break;
-        }
-        if (rr <= eps) {
-            t_end(c);
-            *converged = true;
-            ++k;
-            break;
-        }
-        tg_apply(c, ls, G, h2, subcell, c->q, (k & 1) ? c->zs[1] : c->zs[0], pz);
-        t_end(c);
-    }
-    return k;
-}
-
-// PCG on the context's A, L, press (x) and div (b, negated in place).
-int k_solve(FlipCtx *c, flip_step_stats *st)
+// PCG on the context's A, L, press (x) and div (b, negated in place).  With
+// setup == false the caller has negated b and built the FLUID list.
+static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
 {
     int n = c->n;
     long C = c->C;
@@ -961,8 +525,11 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
     int mode = c->prm.precond == FLIP_PRECOND_IC0 ? 1 : 0;
 
     t_begin(c, "pcg_setup");
-    hipLaunchKernelGGL(k_negate, nblk(C, TB), TB, 0, c->stream, C, c->div);  // solver.cpp:231 flip(b)
-    int nfl = k_build_fluid_list(c);
+    if (setup) {
+        hipLaunchKernelGGL(k_negate, nblk(C, TB), TB, 0, c->stream, C, c->div);  // solver.cpp:231 flip(b)
+        k_build_fluid_list(c);
+    }
+    const int nfl = c->nfl;
     long cap = c->prm.max_iters > 0 ? c->prm.max_iters : (long)n * n * n;
     int32_t hf[16] = {0};
     hf[I_TOLPOL] = c->prm.tol_policy;
@@ -1026,7 +593,7 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
         // reference PCG from its x: a fresh true residual, which ends the
         // solve at once when it meets the tolerance (else PCG finishes)
         bool conv = false;
-        learned_iters = fcg_learned(c, G, h2, subcell, eps, &conv);
+        learned_iters = learned_fcg(c, G, h2, subcell, eps, &conv);
         hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
                            c->press, c->div, c->P, c->r, c->zs[1], pi);
         hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal);
@@ -1100,4 +667,354 @@ int k_solve(FlipCtx *c, flip_step_stats *st)
     }
     if (!(e2 == e2)) return FLIP_E_NUMERIC;
     return capped ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ FP64
+// The solver's operator in FP64: the same ghost-fluid 7-point rows as
+// compute_Ax (solver.cpp:19-41) with x, the AIR coefficient and the sum in
+// double (no rounding to float anywhere).
+namespace {
+
+constexpr int TB64 = 256;
+
+__device__ __forceinline__ double ax_row64(uint32_t m, const double *__restrict__ x, long c, int n,
+                                           const float *__restrict__ L, int subcell, double h2)
+{
+    const double xs = x[c];
+    double xn[6];
+    double Ln[6];
+    const double Lc = (m & META_HAS_AIR) ? (double)L[c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (m >> (2 * u)) & 3;
+        const long o = kd == NB_OUT ? 0 : nb_offset(u, n);
+        xn[u] = x[c + o];
+        Ln[u] = (kd == NB_AIR) ? (double)L[c + o] : 0.0;
+    }
+    double acc = 6.0 * xs;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (m >> (2 * u)) & 3;
+        if (kd == NB_FLUID) acc -= xn[u];
+        else if (kd != NB_AIR) acc -= xs;
+        else if (subcell) acc -= Ln[u] / fmin(1.0e-6, Lc) * xs;
+    }
+    return acc / h2;
+}
+
+// per-block sums of up to three doubles into part[blockIdx * 3 + i]
+__device__ __forceinline__ void partials3(double *part, double a, double b, double c)
+{
+    __shared__ double lds[3][TB64 / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = a;
+        lds[1][threadIdx.x >> 6] = b;
+        lds[2][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double r = 0.0;
+        for (int i = 0; i < TB64 / 64; ++i) r += lds[threadIdx.x][i];
+        part[blockIdx.x * 3 + threadIdx.x] = r;
+    }
+}
+
+// x64 = press and b64 = b on FLUID cells (b already negated in place)
+__global__ __launch_bounds__(TB64) void k64_load(int nfl, const int32_t *__restrict__ fl, const float *__restrict__ x,
+                                                 const float *__restrict__ b, double *x64, double *b64)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    if (t >= nfl) return;
+    long c = fl[t];
+    x64[c] = (double)x[c];
+    b64[c] = (double)b[c];
+}
+
+// r64 = b64 - A x64; partials (r.r, b.b, 0).  For refinement also the FP32
+// right-hand side of the next correction solve: div = -(float)r (the inner
+// solve negates it back) and a zero warm start.
+__global__ __launch_bounds__(TB64) void k64_residual(int nfl, const int32_t *__restrict__ fl,
+                                                     const uint32_t *__restrict__ meta, int n,
+                                                     const float *__restrict__ L, int subcell, double h2,
+                                                     const double *__restrict__ x64, const double *__restrict__ b64,
+                                                     double *r64, float *div, float *press, double *part)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    double rr = 0.0, bb = 0.0;
+    if (t < nfl) {
+        long c = fl[t];
+        double b = b64[c];
+        double r = b - ax_row64(meta[t], x64, c, n, L, subcell, h2);
+        if (r64) r64[c] = r;
+        if (div) {
+            div[c] = -(float)r;
+            press[c] = 0.0f;
+        }
+        rr = r * r;
+        bb = b * b;
+    }
+    partials3(part, rr, bb, 0.0);
+}
+
+// x64 += press (the correction); press = (float)x64 when last
+__global__ __launch_bounds__(TB64) void k64_correct(int nfl, const int32_t *__restrict__ fl, float *press,
+                                                    double *x64, int last)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    if (t >= nfl) return;
+    long c = fl[t];
+    double x = x64[c] + (double)press[c];
+    x64[c] = x;
+    if (last) press[c] = (float)x;
+}
+
+// final state: press = (float)x64, div = (float)b64 (b negated in place, as
+// solve() leaves it)
+__global__ __launch_bounds__(TB64) void k64_store(int nfl, const int32_t *__restrict__ fl,
+                                                  const double *__restrict__ x64, const double *__restrict__ b64,
+                                                  float *press, float *div)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    if (t >= nfl) return;
+    long c = fl[t];
+    press[c] = (float)x64[c];
+    div[c] = (float)b64[c];
+}
+
+// FP64 Jacobi-PCG (M^-1 = P^2, P the reference diagonal factor)
+// init: r = b - A x, s = M r; partials (r.r, r.z, b.b)
+__global__ __launch_bounds__(TB64) void k64_cg_init(int nfl, const int32_t *__restrict__ fl,
+                                                    const uint32_t *__restrict__ meta, int n,
+                                                    const float *__restrict__ L, int subcell, double h2,
+                                                    const double *__restrict__ x64, const double *__restrict__ b64,
+                                                    const double *__restrict__ P, double *r64, double *s64,
+                                                    double *part)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    double rr = 0.0, rz = 0.0, bb = 0.0;
+    if (t < nfl) {
+        long c = fl[t];
+        double b = b64[c];
+        double r = b - ax_row64(meta[t], x64, c, n, L, subcell, h2);
+        double z = r * P[c] * P[c];
+        r64[c] = r;
+        s64[c] = z;
+        rr = r * r;
+        rz = r * z;
+        bb = b * b;
+    }
+    partials3(part, rr, rz, bb);
+}
+
+// q = A s; partials (s.q)
+__global__ __launch_bounds__(TB64) void k64_cg_q(int nfl, const int32_t *__restrict__ fl,
+                                                 const uint32_t *__restrict__ meta, int n,
+                                                 const float *__restrict__ L, int subcell, double h2,
+                                                 const double *__restrict__ s64, double *q64, double *part)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    double sq = 0.0;
+    if (t < nfl) {
+        long c = fl[t];
+        double q = ax_row64(meta[t], s64, c, n, L, subcell, h2);
+        q64[c] = q;
+        sq = s64[c] * q;
+    }
+    partials3(part, sq, 0.0, 0.0);
+}
+
+// x += alpha s; r -= alpha q; partials (r.r, r.z) with z = M r
+__global__ __launch_bounds__(TB64) void k64_cg_update(int nfl, const int32_t *__restrict__ fl, double alpha,
+                                                      double *x64, double *r64, const double *__restrict__ s64,
+                                                      const double *__restrict__ q64, const double *__restrict__ P,
+                                                      double *part)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    double rr = 0.0, rz = 0.0;
+    if (t < nfl) {
+        long c = fl[t];
+        x64[c] += alpha * s64[c];
+        double r = r64[c] - alpha * q64[c];
+        r64[c] = r;
+        rr = r * r;
+        rz = r * (r * P[c] * P[c]);
+    }
+    partials3(part, rr, rz, 0.0);
+}
+
+// s = M r + beta s
+__global__ __launch_bounds__(TB64) void k64_cg_dir(int nfl, const int32_t *__restrict__ fl, double beta,
+                                                   const double *__restrict__ r64, const double *__restrict__ P,
+                                                   double *s64)
+{
+    int t = blockIdx.x * TB64 + threadIdx.x;
+    if (t >= nfl) return;
+    long c = fl[t];
+    s64[c] = r64[c] * P[c] * P[c] + beta * s64[c];
+}
+
+void alloc64(FlipCtx *c)
+{
+    if (c->x64) return;
+    const long C = c->C;
+    const long G = nblk(C, TB64);
+    FLIP_CHECK(hipMalloc((void **)&c->x64, sizeof(double) * C));
+    FLIP_CHECK(hipMalloc((void **)&c->b64, sizeof(double) * C));
+    FLIP_CHECK(hipMalloc((void **)&c->r64, sizeof(double) * C));
+    FLIP_CHECK(hipMalloc((void **)&c->s64, sizeof(double) * C));
+    FLIP_CHECK(hipMalloc((void **)&c->q64, sizeof(double) * C));
+    FLIP_CHECK(hipMalloc((void **)&c->part64, sizeof(double) * 3 * G));
+    FLIP_CHECK(hipHostMalloc((void **)&c->h_part64, sizeof(double) * 3 * G));
+}
+
+// host sums of the first `nv` columns of the per-block partials (ascending
+// block order)
+void sum64(FlipCtx *c, int blocks, int nv, double *out)
+{
+    FLIP_CHECK(hipMemcpyAsync(c->h_part64, c->part64, sizeof(double) * 3 * blocks, hipMemcpyDeviceToHost,
+                              c->stream));
+    FLIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int v = 0; v < nv; ++v) {
+        double a = 0.0;
+        for (int b = 0; b < blocks; ++b) a += c->h_part64[3 * b + v];
+        out[v] = a;
+    }
+}
+
+}  // namespace
+
+// Negates b in place (solver.cpp:231), builds the FLUID list and the
+// reference diagonal factor P (solver.cpp:125-139, the per-cell form) and
+// loads the FP64 copies of x and b.
+static int setup64(FlipCtx *c)
+{
+    hipLaunchKernelGGL(k_negate, nblk(c->C, TB), TB, 0, c->stream, c->C, c->div);
+    int nfl = k_build_fluid_list(c);
+    alloc64(c);
+    if (nfl > 0) {
+        hipLaunchKernelGGL(k_build_diag, nblk(nfl, TB), TB, 0, c->stream, nfl, c->fl, c->n, c->A, c->L,
+                           c->prm.subcell, c->P, c->iflags);
+        hipLaunchKernelGGL(k64_load, nblk(nfl, TB64), TB64, 0, c->stream, nfl, c->fl, c->press, c->div, c->x64,
+                           c->b64);
+    }
+    return nfl;
+}
+
+// Whole solve in FP64: Jacobi-preconditioned CG to ||r|| <= tol ||b||.
+static int solve_fp64(FlipCtx *c, flip_step_stats *st)
+{
+    const int n = c->n, subcell = c->prm.subcell;
+    const double h2 = (double)(float)(1.0 / ((double)n * n));
+    t_begin(c, "pcg64");
+    const int nfl = setup64(c);
+    if (st) st->n_fluid_cells = nfl;
+    if (nfl == 0) {
+        t_end(c);
+        return 0;
+    }
+    const unsigned G = nblk(nfl, TB64);
+    const long cap = c->prm.max_iters > 0 ? c->prm.max_iters : (long)n * n * n;
+    double v[3];
+    hipLaunchKernelGGL(k64_cg_init, G, TB64, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2, c->x64,
+                       c->b64, c->P, c->r64, c->s64, c->part64);
+    sum64(c, G, 3, v);
+    double rr = v[0], rz = v[1];
+    const double bb = v[2], rr0 = rr;
+    const double eps = c->prm.tol * c->prm.tol * bb;
+    int it = 0;
+    bool capped = false;
+    while (!(rr <= eps) && rr == rr) {
+        if (it >= cap) {
+            capped = true;
+            break;
+        }
+        hipLaunchKernelGGL(k64_cg_q, G, TB64, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2, c->s64,
+                           c->q64, c->part64);
+        sum64(c, G, 1, v);
+        const double alpha = rz / v[0];
+        hipLaunchKernelGGL(k64_cg_update, G, TB64, 0, c->stream, nfl, c->fl, alpha, c->x64, c->r64, c->s64, c->q64,
+                           c->P, c->part64);
+        sum64(c, G, 2, v);
+        rr = v[0];
+        const double beta = v[1] / rz;
+        rz = v[1];
+        ++it;
+        if (rr <= eps || !(rr == rr)) break;
+        hipLaunchKernelGGL(k64_cg_dir, G, TB64, 0, c->stream, nfl, c->fl, beta, c->r64, c->P, c->s64);
+    }
+    hipLaunchKernelGGL(k64_store, G, TB64, 0, c->stream, nfl, c->fl, c->x64, c->b64, c->press, c->div);
+    t_end(c);
+    if (st) {
+        st->iterations = it;
+        st->error2 = rr;
+        st->error2_0 = rr0;
+        st->bnorm2 = bb;
+    }
+    if (!(rr == rr)) return FLIP_E_NUMERIC;
+    return capped ? 1 : 0;
+}
+
+// FP64-residual iterative refinement around the FP32 PCG: r = b - A x in
+// FP64 on the FP64 x; each sweep solves A d = (float)r to relative tol with
+// the FP32 solver and adds d; stops at ||r|| <= refine_tol ||b||.
+static int solve_refine(FlipCtx *c, flip_step_stats *st)
+{
+    const int n = c->n, subcell = c->prm.subcell;
+    const double h2 = (double)(float)(1.0 / ((double)n * n));
+    const int nfl = setup64(c);
+    if (st) st->n_fluid_cells = nfl;
+    if (nfl == 0) return 0;
+    const unsigned G = nblk(nfl, TB64);
+    double v[3];
+    double rr0 = -1.0, rr = 0.0, bb = 0.0;
+    int total = 0, sweeps = 0, learned = 0, rc = 0;
+    for (;;) {
+        t_begin(c, "refine_residual");
+        const bool more = sweeps < c->prm.refine;
+        hipLaunchKernelGGL(k64_residual, G, TB64, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
+                           c->x64, c->b64, nullptr, c->div, c->press, c->part64);
+        sum64(c, G, 2, v);
+        t_end(c);
+        rr = v[0];
+        bb = v[1];
+        if (rr0 < 0.0) rr0 = rr;
+        const double eps = c->prm.refine_tol * c->prm.refine_tol * bb;
+        if (rr <= eps || !(rr == rr) || !more) break;
+        flip_step_stats inner;
+        memset(&inner, 0, sizeof(inner));
+        // the residual kernel left div = -(float)r and press = 0: the inner
+        // solve negates div itself; the FLUID list is current
+        hipLaunchKernelGGL(k_negate, nblk(c->C, TB), TB, 0, c->stream, c->C, c->div);
+        rc = pcg32(c, &inner, false);
+        total += inner.iterations;
+        learned += inner.learned_iterations;
+        ++sweeps;
+        if (rc == FLIP_E_NUMERIC) break;
+        t_begin(c, "refine_update");
+        hipLaunchKernelGGL(k64_correct, G, TB64, 0, c->stream, nfl, c->fl, c->press, c->x64, 0);
+        t_end(c);
+    }
+    hipLaunchKernelGGL(k64_store, G, TB64, 0, c->stream, nfl, c->fl, c->x64, c->b64, c->press, c->div);
+    if (st) {
+        st->iterations = total;
+        st->learned_iterations = learned;
+        st->refine_sweeps = sweeps;
+        st->error2 = rr;
+        st->error2_0 = rr0;
+        st->bnorm2 = bb;
+    }
+    if (!(rr == rr) || rc == FLIP_E_NUMERIC) return FLIP_E_NUMERIC;
+    return rr <= c->prm.refine_tol * c->prm.refine_tol * bb ? 0 : 1;
+}
+
+// solver::solve (solver.cpp:224-241) on the context's grids: the FP32 PCG,
+// or its FP64-residual refinement, or the FP64 reference solve.
+int k_solve(FlipCtx *c, flip_step_stats *st)
+{
+    if (c->prm.solve_fp64) return solve_fp64(c, st);
+    if (c->prm.refine > 0) return solve_refine(c, st);
+    return pcg32(c, st, true);
 }

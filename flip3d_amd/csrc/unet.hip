@@ -46,6 +46,8 @@ void dfree(T *&p)
     p = nullptr;
 }
 
+__device__ __forceinline__ float bf2f_d(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 __device__ __forceinline__ uint16_t f2bf_d(float f)
 {
     uint32_t u = __float_as_uint(f);
@@ -227,6 +229,8 @@ struct Trainer {
     float *L = nullptr, *b = nullptr, *x = nullptr, *t = nullptr, *err = nullptr, *Aerr = nullptr, *At = nullptr;
     float *sigma = nullptr, *coef = nullptr;
     double *part = nullptr, *loss = nullptr;
+    double *gpart = nullptr;  // gradient-norm partials (256)
+    float *gscale = nullptr;  // clip factor of the last step
     int blocks = 64;
 };
 
@@ -454,17 +458,83 @@ void unet_backward(UNet *u, int S)
 
 // ----------------------------------------------------- solver-side inference
 // features from (A, L, r) of one R^3 grid, forward, solution in x
-void unet_solve(UNet *u, const int8_t *A, const float *L, const float *r, float *x, float *sigma, double *part)
+// single block: sigma = sqrt(sum b^2 / count) over FLUID cells (1 when there is none)
+__global__ __launch_bounds__(1024) void k_rhs_sigma1(const int8_t *__restrict__ A, const float *__restrict__ b, long n,
+                                                     float *sigma)
+{
+    __shared__ double s0[16], s1[16];
+    double a = 0.0, c = 0.0;
+    for (long i = threadIdx.x; i < n; i += 1024)
+        if (A[i] == FLIP_FLUID) {
+            a += (double)b[i] * b[i];
+            c += 1.0;
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s0[threadIdx.x >> 6] = a;
+        s1[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tc = 0.0;
+        for (int w = 0; w < 16; ++w) {
+            ta += s0[w];
+            tc += s1[w];
+        }
+        double v = tc > 0.0 ? sqrt(ta / tc) : 0.0;
+        sigma[0] = v > 0.0 ? (float)v : 1.0f;
+    }
+}
+
+// output layer and unit conversion in one pass: x = mask (w . d0 + b) sigma scale
+__global__ void k_head_solve(const uint16_t *__restrict__ X, const float *__restrict__ w, const float *__restrict__ b,
+                             const float *__restrict__ mask, const float *__restrict__ sigma, float scale, long nvox,
+                             float *x)
+{
+    long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    float s = b[0];
+    for (int c = 0; c < 16; ++c) s += w[c] * bf2f_d(X[v * 16 + c]);
+    float out = mask[v] * s;
+    x[v] = out * sigma[0] * scale;
+}
+
+// Inference forward of one sample: the decoder's upsample-adds are formed
+// inside the convolutions and the head writes the solution directly.
+static void unet_infer(UNet *u, const float *sigma, float *x)
 {
     hipStream_t s = u->stream;
-    const long n = u->vox(0, 1);
-    const int blocks = 64;
-    hipLaunchKernelGGL(k_rhs_stats, dim3(blocks, 1), dim3(256), 0, s, A, r, n, part);
-    hipLaunchKernelGGL(k_rhs_sigma, dim3(1), dim3(64), 0, s, part, blocks, 1, sigma);
-    hipLaunchKernelGGL(k_features, dim3(nb(n)), dim3(256), 0, s, A, L, r, sigma, n, n, u->F0, u->mask);
-    unet_forward(u, 1);
+    const int R0 = u->R, R1 = R0 / 2, R2 = R0 / 4, R3 = R0 / 8;
+    const long v0 = u->vox(0, 1), v1 = u->vox(1, 1), v2 = u->vox(2, 1), v3 = u->vox(3, 1);
+    auto W = [&](int i) { return u->pwb + u->conv[i].w_off; };
+    auto B = [&](int i) { return u->pw + u->conv[i].b_off; };
+    unet_conv3_fwd(UNET_FEATURES, 16, u->F0, W(0), B(0), u->e0, nullptr, R0, v0, true, s);
+    unet_pool2(u->e0, R0, 16, v1, u->p0, s);
+    unet_conv3_fwd(16, 32, u->p0, W(1), B(1), u->e1, nullptr, R1, v1, true, s);
+    unet_pool2(u->e1, R1, 32, v2, u->p1, s);
+    unet_conv3_fwd(32, 64, u->p1, W(2), B(2), u->e2, nullptr, R2, v2, true, s);
+    unet_pool2(u->e2, R2, 64, v3, u->p2, s);
+    unet_conv3_fwd(64, 64, u->p2, W(3), B(3), u->e3, nullptr, R3, v3, true, s);
+    unet_conv3_fwd_up(64, 32, u->e3, u->e2, W(4), B(4), u->d2, R2, v2, s);
+    unet_conv3_fwd_up(32, 16, u->d2, u->e1, W(5), B(5), u->d1, R1, v1, s);
+    unet_conv3_fwd_up(16, 16, u->d1, u->e0, W(6), B(6), u->d0, R0, v0, s);
     const float scale = OUT_UNIT / ((float)u->R * (float)u->R);
-    hipLaunchKernelGGL(k_unscale, dim3(nb(n)), dim3(256), 0, s, u->out, sigma, n, n, scale, x);
+    hipLaunchKernelGGL(k_head_solve, dim3(nb(v0)), dim3(256), 0, s, u->d0, u->pw + u->head_w, u->pw + u->head_b, u->mask,
+                       sigma, scale, v0, x);
+}
+
+void unet_solve(UNet *u, const int8_t *A, const float *L, const float *r, float *x, float *sigma, double *part)
+{
+    (void)part;
+    hipStream_t s = u->stream;
+    const long n = u->vox(0, 1);
+    hipLaunchKernelGGL(k_rhs_sigma1, dim3(1), dim3(1024), 0, s, A, r, n, sigma);
+    hipLaunchKernelGGL(k_features, dim3(nb(n)), dim3(256), 0, s, A, L, r, sigma, n, n, u->F0, u->mask);
+    unet_infer(u, sigma, x);
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -489,6 +559,8 @@ void trainer_alloc(Trainer &t, UNet *u)
     dmalloc(&t.coef, u->cap);
     dmalloc(&t.part, 2L * t.blocks * u->cap);
     dmalloc(&t.loss, u->cap);
+    dmalloc(&t.gpart, 256);
+    dmalloc(&t.gscale, 1);
 }
 
 void trainer_free(Trainer &t)
@@ -505,6 +577,8 @@ void trainer_free(Trainer &t)
     dfree(t.coef);
     dfree(t.part);
     dfree(t.loss);
+    dfree(t.gpart);
+    dfree(t.gscale);
     if (t.net) unet_free(t.net);
     t.net = nullptr;
 }
@@ -647,7 +721,7 @@ int flip_unet_train_step(flip_unet *h, int S, const int8_t *A, const float *L, c
         hipLaunchKernelGGL(k_scale_per_sample, dim3(nb(n)), dim3(256), 0, s, n, per, t.coef, t.Aerr, u->gout);
         unet_backward(u, S);
         u->step++;
-        unet_adam(u->nparam, u->pw, u->pg, u->pm, u->pv, lr, u->step, u->pwb, s);
+        unet_adam(u->nparam, u->pw, u->pg, u->pm, u->pv, lr, u->step, UNET_CLIP, t.gpart, t.gscale, u->pwb, s);
         std::vector<double> ls(S);
         FLIP_CHECK(hipMemcpyAsync(ls.data(), t.loss, sizeof(double) * S, hipMemcpyDeviceToHost, s));
         FLIP_CHECK(hipStreamSynchronize(s));

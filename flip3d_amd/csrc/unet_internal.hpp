@@ -11,6 +11,10 @@
 // ---- k_conv.hip launchers ----
 void unet_conv3_fwd(int cin, int cout, const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, float *Yf,
                     int R, long nvox, bool relu, hipStream_t s);
+// conv(up(U) + X) with leaky ReLU, the upsampled sum formed on the fly (the
+// bf16 result equals conv of k_up_add's output)
+void unet_conv3_fwd_up(int cin, int cout, const uint16_t *U, const uint16_t *X, const uint16_t *W, const float *bias,
+                       uint16_t *Y, int R, long nvox, hipStream_t s);
 void unet_conv3_wgrad(int cin, int cout, const uint16_t *X, const uint16_t *dY, float *partial, float *dW, int R,
                       long nvox, hipStream_t s);
 int unet_wgrad_chunks(long nvox);
@@ -27,7 +31,10 @@ void unet_head_fwd(const uint16_t *X, int C, const float *w, const float *b, con
 int unet_head_blocks(long nvox);
 void unet_head_back(const uint16_t *X, int C, const float *w, const float *mask, const float *g, long nvox, uint16_t *dX,
                     float *partial, float *dwb, hipStream_t s);
-void unet_adam(long n, float *w, const float *g, float *m, float *v, float lr, int t, uint16_t *wbf, hipStream_t s);
+// Adam step with the gradient clipped to global norm `clip` (partial: 256
+// doubles of scratch, gscale: 1 float of scratch)
+void unet_adam(long n, float *w, const float *g, float *m, float *v, float lr, int t, float clip, double *partial,
+               float *gscale, uint16_t *wbf, hipStream_t s);
 void unet_to_bf16(long n, const float *a, uint16_t *b, hipStream_t s);
 void unet_to_f32(long n, const uint16_t *a, float *b, hipStream_t s);
 
@@ -35,6 +42,7 @@ void unet_to_f32(long n, const uint16_t *a, float *b, hipStream_t s);
 constexpr int UNET_FEATURES = 8;  // input channels (5 used, padded to 8)
 constexpr float UNET_LEAK = 0.1f;  // negative slope of the leaky ReLU
 constexpr int UNET_NCONV = 7;
+constexpr float UNET_CLIP = 1.0f;  // global gradient-norm clip of the training step
 
 struct UNetConv {
     int cin, cout, level;  // level: spatial side R >> level

@@ -75,8 +75,15 @@ typedef struct flip_params {
     int correction;     /* !DISABLE_CORRECTION */
     int reposition;     /* stuck-particle repositioning, flip3D.cpp:932-968 */
     unsigned rand_seed; /* glibc-compatible rand() stream seed; the reference's is 1 */
-    int refine;         /* > 0: FP64-residual iterative refinement sweeps (config N=256) */
-    double refine_tol;  /* outer relative tolerance for refine > 0 */
+    int refine;         /* > 0: at most this many FP64-residual iterative-refinement sweeps: the
+                           residual b - A x is formed in FP64 on an FP64 copy of x, each sweep
+                           solves A d = r with the FP32 PCG to relative tol (tol_policy 1
+                           required) and adds d to the FP64 x */
+    double refine_tol;  /* outer relative tolerance ||b - A x|| <= refine_tol ||b|| for refine > 0 */
+    int solve_fp64;     /* 1: the pressure solve runs entirely in FP64 (Jacobi-preconditioned CG on
+                           the same operator, stop at relative tol; tol_policy 1 required); the
+                           error reference for the FP32 and refined solves.  No counterpart in
+                           the reference, whose PCG is FP32 with FP64 reductions. */
 } flip_params;
 
 typedef struct flip_step_stats {
