@@ -9,9 +9,13 @@ PCG reductions.  A step is one full simulateStep (sort, density, P2G, flags,
 projection, extrapolation, FLIP G2P, advection with wall push-out and
 repositioning, corrector), with all state resident in HBM.
 
-Prints one JSON line with the throughput, the roofline of the dominant kernel
-(live HIP-event durations of its launches) and the CPU oracle timed on this
-host on a bounded sample of the same workload.
+The reported value is the learned solver's (FLIP_PRECOND_LEARNED, flexible CG
+with the multigrid/U-Net preconditioner; --precond selects another), with
+the plain reference-preconditioned PCG timed beside it from the same initial
+state.  Prints one JSON line with both throughputs, the roofline of the
+dominant kernel of the reported run (live HIP-event durations of its
+launches) and the CPU oracle timed on this host on a bounded sample of the
+same workload.
 """
 import argparse
 import json
@@ -69,6 +73,9 @@ def kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid):
         # PCG iteration kernels, per FLUID cell
         "pcg_a": 25 * n_fluid,    # fl 4, A 1, L 4, z 4, s_old 4 -> s_new 4, q 4
         "pcg_b": 41 * n_fluid,    # fl 4, A 1, x 4, r 4, s 4, q 4, P 8 -> x 4, r 4, z 4
+        # flexible-CG kernels of the learned solver, per FLUID cell
+        "fcg_a": 28 * n_fluid,    # fl 4, meta 4, L 4 (AIR-adjacent), (z, s) 8 -> (z, s) 8, q 4
+        "fcg_b": 28 * n_fluid,    # fl 4, x 4, s 4, r 4, q 4 -> x 4, r 4
         # particle gathers, per particle (float4 pos/vel = 16 B)
         "density": 16 * n_particles + 4 * n_particles + 4 * n_cells,
         "p2g": 32 * n_particles + 4 * n_cells + 2 * 4 * n_faces + n_cells,
@@ -99,13 +106,73 @@ def cpu_baseline(n, seconds_budget):
     }
 
 
+def run_mode(fa, n, precond, weights, steps, warmup):
+    """Timed steps of one solver mode from the initial scene, then one
+    untimed step with every kernel launch timed."""
+    code = {"reference": fa.PRECOND_REFERENCE, "ic0": fa.PRECOND_IC0, "learned": fa.PRECOND_LEARNED}[precond]
+    ctx = fa.FlipContext(n, tol_policy=1, tol=TOL, dt_policy=1, precond=code)
+    if precond == "learned":
+        ctx.set_learned_weights(weights)
+    n_particles = ctx.build_scene(fa.SCENE_BENCHMARK)
+    ctx.srand(1)
+    for _ in range(warmup):
+        ctx.step()
+    ctx.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(ctx.step())
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(True)
+    ctx.step()
+    ktimes = ctx.kernel_times()
+    ctx.set_profiling(False)
+    ctx.close()
+    agg = {}
+    for name, ms in ktimes:
+        tot, cnt = agg.get(name, (0.0, 0))
+        agg[name] = (tot + ms, cnt + 1)
+    mean = lambda k: sum(st[k] for st in stats) / len(stats)  # noqa: E731
+    return {
+        "precond": precond, "steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+        "particles": n_particles, "n_fluid": stats[-1]["n_fluid_cells"],
+        "iterations_mean": mean("iterations"), "learned_iterations_mean": mean("learned_iterations"),
+        "solve_ms_mean": mean("ms_solve"),
+        "rel_residual_max": max((st["error2"] / max(st["bnorm2"], 1e-300)) ** 0.5 for st in stats),
+        "agg": agg,
+    }
+
+
+def roofline_of(agg, n, n_particles, n_fluid):
+    """HBM roofline of the kernel with the largest share of the profiled
+    step among those with an algorithmic-bytes model."""
+    n_cells, n_faces = n ** 3, (n + 1) * n * n
+    ranked = sorted(((tot, name, cnt) for name, (tot, cnt) in agg.items()), reverse=True)
+    dom = next((r for r in ranked if kernel_bytes(r[1], n_particles, n_cells, n_faces, n_fluid)), None)
+    if not dom:
+        return None
+    tot, name, cnt = dom
+    avg_ms = tot / cnt
+    b = kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid)
+    achieved = b / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
+            "bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt}
+
+
+def summary(r):
+    return {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in r.items() if k != "agg"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--precond", default="reference", choices=["reference", "ic0", "learned"])
+    ap.add_argument("--precond", default="learned", choices=["reference", "ic0", "learned"],
+                    help="solver of the reported value; the plain (reference) PCG is always timed beside it")
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS,
                     help="coarse-level network weights for --precond learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -119,48 +186,12 @@ def main():
 
     import flip3d_amd as fa
 
-    precond = {"reference": fa.PRECOND_REFERENCE, "ic0": fa.PRECOND_IC0, "learned": fa.PRECOND_LEARNED}[args.precond]
-    ctx = fa.FlipContext(args.n, tol_policy=1, tol=TOL, dt_policy=1, precond=precond)
-    if args.precond == "learned":
-        ctx.set_learned_weights(args.weights)
-    n_particles = ctx.build_scene(fa.SCENE_BENCHMARK)
-    ctx.srand(1)
-    for _ in range(args.warmup):
-        ctx.step()
-    ctx.synchronize()
-    iters, solve_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = ctx.step()
-        iters.append(st["iterations"])
-        solve_ms.append(st["ms_solve"])
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    n_fluid = st["n_fluid_cells"]
-
-    # one more (untimed) step with every launch bracketed by HIP events
-    ctx.set_profiling(True)
-    pst = ctx.step()
-    ktimes = ctx.kernel_times()
-    ctx.set_profiling(False)
-    ctx.close()
-    agg = {}
-    for name, ms in ktimes:
-        tot, cnt = agg.get(name, (0.0, 0))
-        agg[name] = (tot + ms, cnt + 1)
     n = args.n
-    n_cells, n_faces = n ** 3, (n + 1) * n * n
-    ranked = sorted(((tot, name, cnt) for name, (tot, cnt) in agg.items()), reverse=True)
-    dom = next((r for r in ranked if kernel_bytes(r[1], n_particles, n_cells, n_faces, n_fluid)), None)
-    roofline = None
-    if dom:
-        tot, name, cnt = dom
-        avg_ms = tot / cnt
-        b = kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid)
-        achieved = b / (avg_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
-                    "bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt}
+    plain = run_mode(fa, n, "reference", None, args.steps, args.warmup)
+    main_run = plain if args.precond == "reference" else run_mode(fa, n, args.precond, args.weights, args.steps,
+                                                                 args.warmup)
+    n_particles = main_run["particles"]
+    roofline = roofline_of(main_run["agg"], n, n_particles, main_run["n_fluid"])
 
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
@@ -169,21 +200,24 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n, 30)
 
-    value = args.steps / elapsed
+    value = main_run["steps_per_s"]
     out = {
         "metric": METRIC,
         "value": round(value, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": round(main_run["ms_per_step"], 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"benchmark dam-break scene N={n}, {n_particles} particles "
-                               f"({n_fluid} fluid cells), CFL dt, PCG to rel 1e-5, {args.precond} preconditioner",
+                               f"({main_run['n_fluid']} fluid cells), CFL dt, pressure solve to rel 1e-5, "
+                               f"{args.precond} preconditioner; plain reference-preconditioned PCG timed beside it",
                    "n": n, "particles": n_particles, "precond": args.precond, "tol": TOL},
         "particles_per_s": round(value * n_particles, 1),
-        "pcg_iterations_mean": round(sum(iters) / len(iters), 1),
-        "solve_ms_mean": round(sum(solve_ms) / len(solve_ms), 3),
+        "speedup_vs_plain_step": round(value / plain["steps_per_s"], 3),
+        "speedup_vs_plain_solve": round(plain["solve_ms_mean"] / main_run["solve_ms_mean"], 3),
+        "solver": summary(main_run),
+        "plain": summary(plain),
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "profile_step_ms": {k: round(v[0], 4) for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])},
+        "profile_step_ms": {k: round(v[0], 4) for k, v in sorted(main_run["agg"].items(), key=lambda kv: -kv[1][0])},
     }
     print(json.dumps(out))
 

@@ -146,19 +146,41 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t *__restrict__ 
 // are loaded while the current MFMA runs) and are summed through LDS.  This
 // gives layers of a few hundred voxels (the bottom of the U-Net) 4 * NT
 // times the waves of k_conv3_fwd.
-// When U is given, the input is up(U) + X (nearest upsampling of the
-// (R/2)^3 tensor U, rounded to bf16 exactly as k_up_add rounds it), so the
-// sum is never stored.
-template <int CIN, int COUT>
+// Input modes (the fused forms round to bf16 exactly as the stored
+// intermediate would be rounded, so results equal the unfused network):
+//   IN_PLAIN  X
+//   IN_UP     up(U) + X: nearest upsampling of the (R/2)^3 tensor U (k_up_add)
+//   IN_POOL   pool(X): 2x2x2 average of the (2R)^3 tensor X (k_pool2)
+// HEAD: the layer is the last one (16 output channels, one tile); instead of
+// storing it, the block applies the 1x1x1 head and writes the solution
+// x = mask (w . y + b) sigma scale (k_head_fwd followed by the unit change).
+enum { IN_PLAIN = 0, IN_UP = 1, IN_POOL = 2 };
+
+struct HeadArgs {
+    const float *w, *b, *mask, *sigma;
+    float scale;
+    float *x;
+};
+
+__device__ __forceinline__ uint32_t add_bf16x2(uint32_t u, uint32_t x)
+{
+    const uint16_t lo = f2bf(bf2f((uint16_t)(u & 0xffff)) + bf2f((uint16_t)(x & 0xffff)));
+    const uint16_t hi = f2bf(bf2f((uint16_t)(u >> 16)) + bf2f((uint16_t)(x >> 16)));
+    return (uint32_t)lo | ((uint32_t)hi << 16);
+}
+
+template <int CIN, int COUT, int MODE, bool HEAD>
 __global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict__ X, const uint16_t *__restrict__ U,
                                                       const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
-                                                      uint16_t *Y, float *Yf, int R, long nvox, int relu)
+                                                      uint16_t *Y, float *Yf, int R, long nvox, int relu, HeadArgs hd)
 {
     static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+    static_assert(!HEAD || COUT == 16, "the head follows a 16-channel layer");
     constexpr int CPT = CIN / 8;
     constexpr int NCH = 27 * CPT;
     constexpr int STEPS = (NCH + 3) / 4;
     __shared__ f32x4 red[3][64];
+    __shared__ float hy[16][17];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, grp = lane >> 4;
     const long base = (long)blockIdx.x * 16;
@@ -172,19 +194,34 @@ __global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict
         const int tap = q / CPT, c0 = (q % CPT) * 8;
         const long sv = vox_shift(vo, tap, R);
         if (sv < 0) return zero8();
-        if (!U) return load8(X + sv * CIN + c0);
-        const int z = vo.z + tap / 9 - 1, y = vo.y + (tap / 3) % 3 - 1, x = vo.x + tap % 3 - 1, Rc = R >> 1;
-        const long p = (((long)vo.s * Rc + (z >> 1)) * Rc + (y >> 1)) * Rc + (x >> 1);
-        const uint4 xs = *reinterpret_cast<const uint4 *>(X + sv * CIN + c0);
-        const uint4 us = *reinterpret_cast<const uint4 *>(U + p * CIN + c0);
-        const uint32_t xw[4] = {xs.x, xs.y, xs.z, xs.w}, uw[4] = {us.x, us.y, us.z, us.w};
+        if (MODE == IN_PLAIN) return load8(X + sv * CIN + c0);
+        const int z = vo.z + tap / 9 - 1, y = vo.y + (tap / 3) % 3 - 1, x = vo.x + tap % 3 - 1;
+        if (MODE == IN_UP) {
+            const int Rc = R >> 1;
+            const long p = (((long)vo.s * Rc + (z >> 1)) * Rc + (y >> 1)) * Rc + (x >> 1);
+            const uint4 xs = *reinterpret_cast<const uint4 *>(X + sv * CIN + c0);
+            const uint4 us = *reinterpret_cast<const uint4 *>(U + p * CIN + c0);
+            return __builtin_bit_cast(bf16x8, make_uint4(add_bf16x2(us.x, xs.x), add_bf16x2(us.y, xs.y),
+                                                         add_bf16x2(us.z, xs.z), add_bf16x2(us.w, xs.w)));
+        }
+        // IN_POOL: X has side 2R; the eight children in k_pool2's order
+        const int Rf = R << 1;
+        float acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            const long f = (((long)vo.s * Rf + 2 * z + (d >> 2)) * Rf + 2 * y + ((d >> 1) & 1)) * Rf + 2 * x + (d & 1);
+            const uint4 u = *reinterpret_cast<const uint4 *>(X + f * CIN + c0);
+            const uint32_t w4[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                acc8[2 * h] += bf2f((uint16_t)(w4[h] & 0xffff));
+                acc8[2 * h + 1] += bf2f((uint16_t)(w4[h] >> 16));
+            }
+        }
         uint32_t o[4];
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const uint16_t lo = f2bf(bf2f((uint16_t)(uw[h] & 0xffff)) + bf2f((uint16_t)(xw[h] & 0xffff)));
-            const uint16_t hi = f2bf(bf2f((uint16_t)(uw[h] >> 16)) + bf2f((uint16_t)(xw[h] >> 16)));
-            o[h] = (uint32_t)lo | ((uint32_t)hi << 16);
-        }
+        for (int h = 0; h < 4; ++h)
+            o[h] = (uint32_t)f2bf(acc8[2 * h] * 0.125f) | ((uint32_t)f2bf(acc8[2 * h + 1] * 0.125f) << 16);
         return __builtin_bit_cast(bf16x8, make_uint4(o[0], o[1], o[2], o[3]));
     };
     auto load_b = [&](int st) -> bf16x8 {
@@ -202,23 +239,39 @@ __global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict
     }
     if (wave > 0) red[wave - 1][lane] = acc;
     __syncthreads();
-    if (wave != 0) return;
+    if (wave == 0) {
 #pragma unroll
-    for (int w = 0; w < 3; ++w) {
-        const f32x4 o = red[w][lane];
+        for (int w = 0; w < 3; ++w) {
+            const f32x4 o = red[w][lane];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] += o[i];
+            for (int i = 0; i < 4; ++i) acc[i] += o[i];
+        }
+        const int co = n0 + row;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long vv = base + grp * 4 + i;
+            float val = acc[i] + bv;
+            if (relu && val < 0.0f) val *= UNET_LEAK;
+            if (HEAD) {
+                hy[grp * 4 + i][co] = bf2f(f2bf(val));
+                continue;
+            }
+            if (vv >= nvox) continue;
+            if (Y) Y[vv * COUT + co] = f2bf(val);
+            if (Yf) Yf[vv * COUT + co] = val;
+        }
     }
-    const int co = n0 + row;
-    const float bv = bias ? bias[co] : 0.0f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        long vv = base + grp * 4 + i;
-        if (vv >= nvox) continue;
-        float val = acc[i] + bv;
-        if (relu && val < 0.0f) val *= UNET_LEAK;
-        if (Y) Y[vv * COUT + co] = f2bf(val);
-        if (Yf) Yf[vv * COUT + co] = val;
+    if (!HEAD) return;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        const long vv = base + threadIdx.x;
+        if (vv < nvox) {
+            float sh = hd.b[0];
+            for (int c = 0; c < 16; ++c) sh += hd.w[c] * hy[threadIdx.x][c];
+            const float out = hd.mask[vv] * sh;
+            hd.x[vv] = out * hd.sigma[0] * hd.scale;
+        }
     }
 }
 
@@ -508,8 +561,8 @@ void fwd_t(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y,
     const long waves = (nvox + 16 * MT - 1) / (16 * MT);
     if (waves < 2048) {
         // too few voxel tiles to fill the chip: split K and the output channels
-        hipLaunchKernelGGL((k_conv3_fwd_sk<CIN, COUT>), dim3(nb(nvox, 16), COUT / 16), dim3(256), 0, s, X,
-                           (const uint16_t *)nullptr, W, bias, Y, Yf, R, nvox, relu);
+        hipLaunchKernelGGL((k_conv3_fwd_sk<CIN, COUT, IN_PLAIN, false>), dim3(nb(nvox, 16), COUT / 16), dim3(256), 0,
+                           s, X, (const uint16_t *)nullptr, W, bias, Y, Yf, R, nvox, relu, HeadArgs{});
         return;
     }
     hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, MT>), dim3(nb(nvox, (int)per_block)), dim3(256), 0, s, X, W, bias, Y,
@@ -544,13 +597,36 @@ void unet_conv3_fwd_up(int cin, int cout, const uint16_t *U, const uint16_t *X, 
 {
 #define CASE(a, b)                                                                                                   \
     if (cin == a && cout == b) {                                                                                     \
-        hipLaunchKernelGGL((k_conv3_fwd_sk<a, b>), dim3(nb(nvox, 16), b / 16), dim3(256), 0, s, X, U, W, bias, Y,     \
-                           (float *)nullptr, R, nvox, 1);                                                            \
+        hipLaunchKernelGGL((k_conv3_fwd_sk<a, b, IN_UP, false>), dim3(nb(nvox, 16), b / 16), dim3(256), 0, s, X, U, W,  \
+                           bias, Y, (float *)nullptr, R, nvox, 1, HeadArgs{});                                       \
         return;                                                                                                      \
     }
     CASE(64, 32) CASE(32, 16) CASE(16, 16)
 #undef CASE
     throw flip_arg_error{FLIP_E_ARG, "unsupported conv channel pair"};
+}
+
+void unet_conv3_fwd_pool(int cin, int cout, const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int R,
+                         long nvox, hipStream_t s)
+{
+#define CASE(a, b)                                                                                                   \
+    if (cin == a && cout == b) {                                                                                     \
+        hipLaunchKernelGGL((k_conv3_fwd_sk<a, b, IN_POOL, false>), dim3(nb(nvox, 16), b / 16), dim3(256), 0, s, X,     \
+                           (const uint16_t *)nullptr, W, bias, Y, (float *)nullptr, R, nvox, 1, HeadArgs{});         \
+        return;                                                                                                      \
+    }
+    CASE(16, 32) CASE(32, 64) CASE(64, 64)
+#undef CASE
+    throw flip_arg_error{FLIP_E_ARG, "unsupported conv channel pair"};
+}
+
+void unet_conv3_fwd_up_head(const uint16_t *U, const uint16_t *X, const uint16_t *W, const float *bias,
+                            const float *hw, const float *hb, const float *mask, const float *sigma, float scale,
+                            float *x, int R, long nvox, hipStream_t s)
+{
+    HeadArgs hd{hw, hb, mask, sigma, scale, x};
+    hipLaunchKernelGGL((k_conv3_fwd_sk<16, 16, IN_UP, true>), dim3(nb(nvox, 16), 1), dim3(256), 0, s, X, U, W, bias,
+                       (uint16_t *)nullptr, (float *)nullptr, R, nvox, 1, hd);
 }
 
 void unet_conv3_wgrad(int cin, int cout, const uint16_t *X, const uint16_t *dY, float *partial, float *dW, int R,

@@ -154,16 +154,35 @@ __global__ void k_ml_zero(long n, float *a)
 }
 
 // ------------------------------------------------------------ V-cycle
-// level 0 pre-smoothing: z1 = w D^-1 r, z2 = z1 + w D^-1 (r - A z1)
-__global__ __launch_bounds__(TBL) void k_ml_pre0(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
-                                                 const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
-                                                 int n, const float *__restrict__ L, int subcell, double h2,
-                                                 const float *__restrict__ r, const float *__restrict__ dinv,
-                                                 float *z2)
+// level 0 pre-smoothing: z1 = w D^-1 r, z2 = z1 + w D^-1 (r - A z1).  With
+// check set it first applies the flexible-CG stop test of the iteration
+// that has just updated r: every block sums the r.r partials in the same
+// order and stops on convergence or NaN; block 0 records the result,
+// advances the iteration index and also stops at kmax.
+__global__ __launch_bounds__(TB) void k_ml_pre0(const int32_t *__restrict__ iflags, int32_t *fi,
+                                                const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                                int n, const float *__restrict__ L, int subcell, double h2,
+                                                const float *__restrict__ r, const float *__restrict__ dinv, float *z2,
+                                                const double *__restrict__ pb, double *fs, int G, int kmax, int check)
 {
     if (fi[FI_DONE]) return;
+    if (check) {
+        const double rr = sum_partials(pb, G, 1, 0);
+        const bool conv = rr <= fs[F_EPS];
+        const bool stop = conv || !(rr == rr);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const int k = fi[FI_K] + 1;
+            fs[F_RR] = rr;
+            fi[FI_K] = k;
+            if (stop || k >= kmax) {
+                fi[FI_DONE] = 1;
+                fi[FI_CONV] = conv ? 1 : 0;
+            }
+        }
+        if (stop) return;
+    }
     const int nfl = iflags[I_NFL];
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) {
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
         uint32_t m = meta[t];
         float zv[6];
@@ -256,31 +275,55 @@ __global__ __launch_bounds__(TBL) void k_ml_resid(const int32_t *__restrict__ fi
     res[c] = b[c] - ax_row(m, xv, x[c], c, R, L, subcell, h2);
 }
 
-// restriction: bc[I] = sum of the 8 children's residuals (zero off FLUID)
+// restriction: bc[I] = sum of the 8 children's residuals (zero off FLUID).
+// With Ac given (the network's level) it also writes per-block partials
+// (sum bc^2 over FLUID coarse cells, FLUID count) for the network's scaling.
 __global__ __launch_bounds__(TBL) void k_ml_sum8(const int32_t *__restrict__ fi, int Rc, int Rf,
-                                                 const float *__restrict__ res, float *bc)
+                                                 const float *__restrict__ res, float *bc,
+                                                 const int8_t *__restrict__ Ac, double *part)
 {
     if (fi[FI_DONE]) return;
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long Cc = (long)Rc * Rc * Rc;
-    if (g >= Cc) return;
-    const IJK p = ijk_of(g, Rc);
-    const long f0 = ((long)(2 * p.i) * Rf + 2 * p.j) * Rf + 2 * p.k;
-    const long di = (long)Rf * Rf, dj = Rf;
-    const float2 a = *reinterpret_cast<const float2 *>(res + f0);
-    const float2 b = *reinterpret_cast<const float2 *>(res + f0 + dj);
-    const float2 c = *reinterpret_cast<const float2 *>(res + f0 + di);
-    const float2 d = *reinterpret_cast<const float2 *>(res + f0 + di + dj);
     float sum = 0.0f;
-    sum += a.x;
-    sum += a.y;
-    sum += b.x;
-    sum += b.y;
-    sum += c.x;
-    sum += c.y;
-    sum += d.x;
-    sum += d.y;
-    bc[g] = sum;
+    if (g < Cc) {
+        const IJK p = ijk_of(g, Rc);
+        const long f0 = ((long)(2 * p.i) * Rf + 2 * p.j) * Rf + 2 * p.k;
+        const long di = (long)Rf * Rf, dj = Rf;
+        const float2 a = *reinterpret_cast<const float2 *>(res + f0);
+        const float2 b = *reinterpret_cast<const float2 *>(res + f0 + dj);
+        const float2 c = *reinterpret_cast<const float2 *>(res + f0 + di);
+        const float2 d = *reinterpret_cast<const float2 *>(res + f0 + di + dj);
+        sum += a.x;
+        sum += a.y;
+        sum += b.x;
+        sum += b.y;
+        sum += c.x;
+        sum += c.y;
+        sum += d.x;
+        sum += d.y;
+        bc[g] = sum;
+    }
+    if (!Ac) return;
+    __shared__ double s0[TBL / 64], s1[TBL / 64];
+    const bool fl = g < Cc && Ac[g] == FLUID;
+    double a2 = fl ? (double)sum * (double)sum : 0.0, cnt = fl ? 1.0 : 0.0;
+    a2 = wave_sum(a2);
+    cnt = wave_sum(cnt);
+    if ((threadIdx.x & 63) == 0) {
+        s0[threadIdx.x >> 6] = a2;
+        s1[threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tc = 0.0;
+        for (int w = 0; w < TBL / 64; ++w) {
+            ta += s0[w];
+            tc += s1[w];
+        }
+        part[2 * blockIdx.x] = ta;
+        part[2 * blockIdx.x + 1] = tc;
+    }
 }
 
 // dense level post-smoothing, first sweep: x3 = x + xc(parent) / MG_S,
@@ -469,23 +512,6 @@ __global__ __launch_bounds__(TB) void k_fcg_b(const int32_t *__restrict__ iflags
     store_partials(pb, 1, rr, 0.0, 0.0, 1);
 }
 
-// single block: stop test after part b; advances the iteration index
-__global__ __launch_bounds__(TB) void k_fcg_check(const double *__restrict__ pb, double *fs, int32_t *fi, int G, int kmax)
-{
-    if (fi[FI_DONE]) return;
-    double rr = sum_partials(pb, G, 1, 0);
-    if (threadIdx.x == 0) {
-        const int k = fi[FI_K] + 1;
-        fs[F_RR] = rr;
-        fi[FI_K] = k;
-        const bool conv = rr <= fs[F_EPS];
-        if (conv || !(rr == rr) || k >= kmax) {
-            fi[FI_DONE] = 1;
-            fi[FI_CONV] = conv ? 1 : 0;
-        }
-    }
-}
-
 struct Level {
     int R = 0;
     long C = 0;
@@ -589,7 +615,7 @@ static LearnedSolver *learned_get(FlipCtx *c)
         dmalloc(&ls->part, 4 * GMAX);
         dmalloc(&ls->fs, 8);
         dmalloc(&ls->fi, 4);
-        dmalloc(&ls->upart, 256);
+        dmalloc(&ls->upart, 2 * (long)nb(ls->lv[ls->K].C));
         FLIP_CHECK(hipHostMalloc((void **)&ls->h_flag, sizeof(int32_t) * 2));
         FLIP_CHECK(hipEventCreateWithFlags(&ls->ev[0], hipEventDisableTiming));
         FLIP_CHECK(hipEventCreateWithFlags(&ls->ev[1], hipEventDisableTiming));
@@ -603,43 +629,48 @@ static LearnedSolver *learned_get(FlipCtx *c)
 
 // z = M(r) into the (z, s) buffer of parity (fi[FI_K] + 1) & 1, partials (z.r,
 // z.q) in pz (q may be null)
-static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell, const float *q, double *pz)
+static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell, const float *q, double *pz,
+                   const double *pb, bool check)
 {
     const int n = c->n, K = ls->K;
-    const int32_t *fi = ls->fi;
+    int32_t *fi = ls->fi;
     const unsigned G0 = (unsigned)std::min<long>(nb(c->C), 4096);  // level-0 smoothing grid (grid-stride)
-    hipLaunchKernelGGL(k_ml_pre0, G0, TBL, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
-                       ls->dinv0, ls->z2);
+    launch_timed(c, "ml_pre0", k_ml_pre0, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
+                       ls->dinv0, ls->z2, pb, ls->fs, G, FCG_MAX, check ? 1 : 0);
     // down: restrict level l-1's residual into b_l, smooth level l
     for (int l = 1; l <= K; ++l) {
         Level &L = ls->lv[l];
         if (l == 1) {
-            hipLaunchKernelGGL(k_ml_resid0, G0, TBL, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
+            launch_timed(c, "ml_resid0", k_ml_resid0, G0, TBL, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
                                c->r, ls->z2, ls->res0);
-            hipLaunchKernelGGL(k_ml_sum8, nb(L.C), TBL, 0, c->stream, fi, L.R, n, ls->res0, L.b);
+            launch_timed(c, "ml_sum8", k_ml_sum8, nb(L.C), TBL, fi, L.R, n, ls->res0, L.b,
+                               l == K ? L.A : nullptr, ls->upart);
         } else {
             Level &F = ls->lv[l - 1];
-            hipLaunchKernelGGL(k_ml_resid, nb(F.C), TBL, 0, c->stream, fi, F.R, F.A, F.meta, F.L, subcell, F.h2, F.b,
+            launch_timed(c, "ml_resid", k_ml_resid, nb(F.C), TBL, fi, F.R, F.A, F.meta, F.L, subcell, F.h2, F.b,
                                F.x, F.t);
-            hipLaunchKernelGGL(k_ml_sum8, nb(L.C), TBL, 0, c->stream, fi, L.R, F.R, F.t, L.b);
+            launch_timed(c, "ml_sum8", k_ml_sum8, nb(L.C), TBL, fi, L.R, F.R, F.t, L.b, l == K ? L.A : nullptr,
+                               ls->upart);
         }
         if (l < K)
-            hipLaunchKernelGGL(k_ml_pre, nb(L.C), TBL, 0, c->stream, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
+            launch_timed(c, "ml_pre", k_ml_pre, nb(L.C), TBL, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
                                L.dinv, L.x);
     }
     Level &KL = ls->lv[K];
-    unet_solve(ls->net, KL.A, KL.L, KL.b, KL.x, ls->sigma, ls->upart);
+    t_begin(c, "unet");
+    unet_solve_partials(ls->net, KL.A, KL.L, KL.b, ls->upart, KL.x, ls->sigma);
+    t_end(c);
     // up: correct level l with level l+1, smooth
     for (int l = K - 1; l >= 1; --l) {
         Level &L = ls->lv[l];
-        hipLaunchKernelGGL(k_ml_post_a, nb(L.C), TBL, 0, c->stream, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
+        launch_timed(c, "ml_post_a", k_ml_post_a, nb(L.C), TBL, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
                            L.dinv, L.x, ls->lv[l + 1].x, L.t);
-        hipLaunchKernelGGL(k_ml_post_b, nb(L.C), TBL, 0, c->stream, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
+        launch_timed(c, "ml_post_b", k_ml_post_b, nb(L.C), TBL, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
                            L.dinv, L.t, L.x);
     }
-    hipLaunchKernelGGL(k_ml_post0_a, G0, TBL, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
+    launch_timed(c, "ml_post0_a", k_ml_post0_a, G0, TBL, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
                        c->r, ls->dinv0, ls->z2, ls->lv[1].x, ls->z4);
-    hipLaunchKernelGGL(k_ml_post0_b, G, TB, 0, c->stream, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
+    launch_timed(c, "ml_post0_b", k_ml_post0_b, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
                        ls->dinv0, ls->z4, q, c->zs[0], c->zs[1], pz);
 }
 
@@ -668,8 +699,8 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
     int32_t hf[4] = {0, 0, 0, 0};
     FLIP_CHECK(hipMemcpyAsync(ls->fs, hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
     FLIP_CHECK(hipMemcpyAsync(ls->fi, hf, sizeof(hf), hipMemcpyHostToDevice, c->stream));
-    vcycle(c, ls, G, h2, subcell, nullptr, pz);  // z_0 into B_1.x
     t_end(c);
+    vcycle(c, ls, G, h2, subcell, nullptr, pz, pb, false);  // z_0 into B_1.x
 
     if (!ls->graph || ls->graph_subcell != subcell || c->profiling) {
         if (ls->graph) {
@@ -682,8 +713,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
                      c->zs[1], c->q, pz, ls->fs, pa, G);
         launch_timed(c, "fcg_b", k_fcg_b, G, TB, c->iflags, ls->fi, c->fl, c->press, c->r, c->zs[0], c->zs[1], c->q,
                      pa, ls->fs, pb, G);
-        hipLaunchKernelGGL(k_fcg_check, 1, TB, 0, c->stream, pb, ls->fs, ls->fi, G, FCG_MAX);
-        vcycle(c, ls, G, h2, subcell, c->q, pz);
+        vcycle(c, ls, G, h2, subcell, c->q, pz, pb, true);  // its first kernel applies the stop test
     };
     if (!c->profiling && !ls->graph) {
         hipGraph_t g;
@@ -697,9 +727,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
     // launch iteration k, then wait for iteration k-1's stop flag
     for (int k = 0; k < FCG_MAX; ++k) {
         if (c->profiling) {
-            t_begin(c, "learned_iteration");
             iteration();
-            t_end(c);
         } else {
             FLIP_CHECK(hipGraphLaunch(ls->graph, c->stream));
         }

@@ -458,17 +458,48 @@ void unet_backward(UNet *u, int S)
 
 // ----------------------------------------------------- solver-side inference
 // features from (A, L, r) of one R^3 grid, forward, solution in x
-// single block: sigma = sqrt(sum b^2 / count) over FLUID cells (1 when there is none)
-__global__ __launch_bounds__(1024) void k_rhs_sigma1(const int8_t *__restrict__ A, const float *__restrict__ b, long n,
-                                                     float *sigma)
+// Features of one sample with sigma from per-block partials (sum b^2, FLUID
+// count) that the producer of b wrote; every block sums them in the same
+// order, block 0 stores sigma for the head.
+__global__ void k_features_p(const int8_t *__restrict__ A, const float *__restrict__ L, const float *__restrict__ b,
+                             const double *__restrict__ part, int nparts, long n, uint16_t *F, float *mask,
+                             float *sigma_out)
 {
-    __shared__ double s0[16], s1[16];
-    double a = 0.0, c = 0.0;
-    for (long i = threadIdx.x; i < n; i += 1024)
-        if (A[i] == FLIP_FLUID) {
-            a += (double)b[i] * b[i];
-            c += 1.0;
+    __shared__ float sig;
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int k = 0; k < nparts; ++k) {
+            a += part[2 * k];
+            c += part[2 * k + 1];
         }
+        double v = c > 0.0 ? sqrt(a / c) : 0.0;
+        sig = v > 0.0 ? (float)v : 1.0f;
+        if (blockIdx.x == 0) sigma_out[0] = sig;
+    }
+    __syncthreads();
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int a = A[g];
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (a == FLIP_FLUID) f[0] = b[g] / sig;
+    f[1] = a == FLIP_FLUID;
+    f[2] = a == FLIP_AIR;
+    f[3] = a == FLIP_WALL;
+    if (a != FLIP_WALL) f[4] = fminf(fmaxf(L[g], -2.0f), 2.0f);
+    for (int c = 0; c < 8; ++c) F[g * 8 + c] = f2bf_d(f[c]);
+    mask[g] = a == FLIP_FLUID ? 1.0f : 0.0f;
+}
+
+// per-block partials (sum b^2 over FLUID cells, FLUID count), blocks of 256
+__global__ void k_rhs_partials(const int8_t *__restrict__ A, const float *__restrict__ b, long n, double *part)
+{
+    __shared__ double s0[4], s1[4];
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, c = 0.0;
+    if (g < n && A[g] == FLIP_FLUID) {
+        a = (double)b[g] * b[g];
+        c = 1.0;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o, 64);
@@ -480,31 +511,14 @@ __global__ __launch_bounds__(1024) void k_rhs_sigma1(const int8_t *__restrict__ 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double ta = 0.0, tc = 0.0;
-        for (int w = 0; w < 16; ++w) {
-            ta += s0[w];
-            tc += s1[w];
-        }
-        double v = tc > 0.0 ? sqrt(ta / tc) : 0.0;
-        sigma[0] = v > 0.0 ? (float)v : 1.0f;
+        part[2 * blockIdx.x] = s0[0] + s0[1] + s0[2] + s0[3];
+        part[2 * blockIdx.x + 1] = s1[0] + s1[1] + s1[2] + s1[3];
     }
 }
 
-// output layer and unit conversion in one pass: x = mask (w . d0 + b) sigma scale
-__global__ void k_head_solve(const uint16_t *__restrict__ X, const float *__restrict__ w, const float *__restrict__ b,
-                             const float *__restrict__ mask, const float *__restrict__ sigma, float scale, long nvox,
-                             float *x)
-{
-    long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox) return;
-    float s = b[0];
-    for (int c = 0; c < 16; ++c) s += w[c] * bf2f_d(X[v * 16 + c]);
-    float out = mask[v] * s;
-    x[v] = out * sigma[0] * scale;
-}
-
-// Inference forward of one sample: the decoder's upsample-adds are formed
-// inside the convolutions and the head writes the solution directly.
+// Inference forward of one sample: pooling and the decoder's upsample-adds
+// are formed inside the convolutions, and the last convolution applies the
+// head and writes the solution directly.
 static void unet_infer(UNet *u, const float *sigma, float *x)
 {
     hipStream_t s = u->stream;
@@ -513,28 +527,32 @@ static void unet_infer(UNet *u, const float *sigma, float *x)
     auto W = [&](int i) { return u->pwb + u->conv[i].w_off; };
     auto B = [&](int i) { return u->pw + u->conv[i].b_off; };
     unet_conv3_fwd(UNET_FEATURES, 16, u->F0, W(0), B(0), u->e0, nullptr, R0, v0, true, s);
-    unet_pool2(u->e0, R0, 16, v1, u->p0, s);
-    unet_conv3_fwd(16, 32, u->p0, W(1), B(1), u->e1, nullptr, R1, v1, true, s);
-    unet_pool2(u->e1, R1, 32, v2, u->p1, s);
-    unet_conv3_fwd(32, 64, u->p1, W(2), B(2), u->e2, nullptr, R2, v2, true, s);
-    unet_pool2(u->e2, R2, 64, v3, u->p2, s);
-    unet_conv3_fwd(64, 64, u->p2, W(3), B(3), u->e3, nullptr, R3, v3, true, s);
+    unet_conv3_fwd_pool(16, 32, u->e0, W(1), B(1), u->e1, R1, v1, s);
+    unet_conv3_fwd_pool(32, 64, u->e1, W(2), B(2), u->e2, R2, v2, s);
+    unet_conv3_fwd_pool(64, 64, u->e2, W(3), B(3), u->e3, R3, v3, s);
     unet_conv3_fwd_up(64, 32, u->e3, u->e2, W(4), B(4), u->d2, R2, v2, s);
     unet_conv3_fwd_up(32, 16, u->d2, u->e1, W(5), B(5), u->d1, R1, v1, s);
-    unet_conv3_fwd_up(16, 16, u->d1, u->e0, W(6), B(6), u->d0, R0, v0, s);
     const float scale = OUT_UNIT / ((float)u->R * (float)u->R);
-    hipLaunchKernelGGL(k_head_solve, dim3(nb(v0)), dim3(256), 0, s, u->d0, u->pw + u->head_w, u->pw + u->head_b, u->mask,
-                       sigma, scale, v0, x);
+    unet_conv3_fwd_up_head(u->d1, u->e0, W(6), B(6), u->pw + u->head_w, u->pw + u->head_b, u->mask, sigma, scale, x,
+                           R0, v0, s);
+}
+
+int unet_partials_count(const UNet *u) { return (int)nb(u->vox(0, 1)); }
+
+void unet_solve_partials(UNet *u, const int8_t *A, const float *L, const float *r, const double *part, float *x,
+                         float *sigma)
+{
+    const long n = u->vox(0, 1);
+    hipLaunchKernelGGL(k_features_p, dim3(nb(n)), dim3(256), 0, u->stream, A, L, r, part, unet_partials_count(u), n,
+                       u->F0, u->mask, sigma);
+    unet_infer(u, sigma, x);
 }
 
 void unet_solve(UNet *u, const int8_t *A, const float *L, const float *r, float *x, float *sigma, double *part)
 {
-    (void)part;
-    hipStream_t s = u->stream;
     const long n = u->vox(0, 1);
-    hipLaunchKernelGGL(k_rhs_sigma1, dim3(1), dim3(1024), 0, s, A, r, n, sigma);
-    hipLaunchKernelGGL(k_features, dim3(nb(n)), dim3(256), 0, s, A, L, r, sigma, n, n, u->F0, u->mask);
-    unet_infer(u, sigma, x);
+    hipLaunchKernelGGL(k_rhs_partials, dim3(nb(n)), dim3(256), 0, u->stream, A, r, n, part);
+    unet_solve_partials(u, A, L, r, part, x, sigma);
 }
 
 // ------------------------------------------------------------------ C ABI

@@ -15,6 +15,15 @@ void unet_conv3_fwd(int cin, int cout, const uint16_t *X, const uint16_t *W, con
 // bf16 result equals conv of k_up_add's output)
 void unet_conv3_fwd_up(int cin, int cout, const uint16_t *U, const uint16_t *X, const uint16_t *W, const float *bias,
                        uint16_t *Y, int R, long nvox, hipStream_t s);
+// conv(pool(X)) with leaky ReLU, X of side 2R pooled on the fly (equals conv
+// of k_pool2's output)
+void unet_conv3_fwd_pool(int cin, int cout, const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int R,
+                         long nvox, hipStream_t s);
+// last layer conv(up(U) + X) (16 -> 16) with the head and the unit change
+// fused: x = mask (hw . y + hb) sigma scale
+void unet_conv3_fwd_up_head(const uint16_t *U, const uint16_t *X, const uint16_t *W, const float *bias,
+                            const float *hw, const float *hb, const float *mask, const float *sigma, float scale,
+                            float *x, int R, long nvox, hipStream_t s);
 void unet_conv3_wgrad(int cin, int cout, const uint16_t *X, const uint16_t *dY, float *partial, float *dW, int R,
                       long nvox, hipStream_t s);
 int unet_wgrad_chunks(long nvox);
@@ -85,6 +94,11 @@ void unet_forward(UNet *u, int samples);
 void unet_backward(UNet *u, int samples);
 void unet_refresh_bf16(UNet *u);
 // one R^3 system on the device: x ~ A^-1 r by the network (sigma: 1 float,
-// part: 128 doubles of scratch)
+// part: 2 * unet_partials_count doubles of scratch)
 void unet_solve(UNet *u, const int8_t *A, const float *L, const float *r, float *x, float *sigma, double *part);
+// the same with the right-hand side's per-256-cell partials (sum r^2 over
+// FLUID cells, FLUID count) already in part, as the producer of r wrote them
+void unet_solve_partials(UNet *u, const int8_t *A, const float *L, const float *r, const double *part, float *x,
+                         float *sigma);
+int unet_partials_count(const UNet *u);
 bool unet_read_weights(const char *path, long nparam, std::vector<float> &w, std::string &err);

@@ -1,7 +1,7 @@
 """Train the coarsest-level network of FLIP_PRECOND_LEARNED on pressure
 systems that the plain GPU solver produces.
 
-    python -m flip3d_amd.train_unet --out weights.bin [--scenes 2000] [--steps 6000]
+    python -m flip3d_amd.train_unet --out weights.bin [--scenes 2000] [--steps 20000]
 
 The network solves the coarsest level of the learned preconditioner's
 multigrid hierarchy: a 32^3 system (16^3 for n = 32) rediscretised from the
@@ -18,8 +18,14 @@ relative residual 1e-6 at the coarse size):
     (A, L, b = -div) is coarsened once to 32^3;
   * `--coarse-scenes` N=128 scenes, coarsened twice to 32^3;
   * for every coarse system also a second right-hand side: random noise on
-    the FLUID cells smoothed by masked neighbour averaging, which stands for
-    the smooth residuals the V-cycle restricts.
+    the FLUID cells smoothed by masked neighbour averaging;
+  * the right-hand sides the network actually receives: for every
+    `--vcycle-every`-th N=64 scene and `--vcycle-coarse-every`-th N=128
+    scene system, the first
+    `--vcycle-iters` flexible-CG iterations of the preconditioner are
+    replayed in numpy (the same hierarchy, damped-Jacobi smoothing and
+    transfers as the GPU V-cycle) with the GPU solver as the coarsest
+    solve, and each coarsest-level right-hand side becomes a system.
 Training: Adam with the gradient clipped to norm 1, cosine learning rate,
 random axis and sign flips, loss = relative A-norm error
 (flip_unet_train_step).
@@ -85,6 +91,92 @@ def smooth_noise(A, rs, passes=4):
     return v
 
 
+# ---------------------------------------------------------------- V-cycle replay
+MG_W = 0.8   # damped Jacobi weight of the GPU V-cycle
+MG_S = 10.0  # its coarse-correction scale
+
+
+def level_operator(A, L):
+    """Sparse ghost-fluid operator of a level (spacing 1/n) on its FLUID
+    cells, with the FLUID index map."""
+    import scipy.sparse as sp
+
+    n = A.shape[0]
+    M = unit_operator(A, L, sp) * float(n * n)
+    idx = -np.ones(A.shape, np.int64)
+    idx[A == fa.FLUID] = np.arange(int((A == fa.FLUID).sum()))
+    return M.tocsr(), idx
+
+
+class Hierarchy:
+    """The GPU V-cycle's levels for one system: fine level, then halvings
+    down to side R (the network's level)."""
+
+    def __init__(self, A, L):
+        import scipy.sparse as sp
+
+        self.levels = []
+        while True:
+            M, idx = level_operator(A, L)
+            self.levels.append({"A": A, "L": L, "M": M, "D": M.diagonal(), "idx": idx})
+            if A.shape[0] <= R:
+                break
+            A, L, _ = coarsen(A, L, np.zeros(A.shape, np.float32))
+        for f, c in zip(self.levels[:-1], self.levels[1:]):
+            fl = np.argwhere(f["A"] == fa.FLUID)
+            par = fl // 2
+            ci = c["idx"][par[:, 0], par[:, 1], par[:, 2]]
+            f["R"] = sp.csr_matrix((np.ones(len(fl)), (ci, np.arange(len(fl)))), shape=(c["M"].shape[0], len(fl)))
+
+    def vcycle(self, r, coarse):
+        def rec(l, b):
+            lv = self.levels[l]
+            if l == len(self.levels) - 1:
+                return coarse(lv, b)
+            x = np.zeros_like(b)
+            for _ in range(2):
+                x = x + MG_W * (b - lv["M"] @ x) / lv["D"]
+            xc = rec(l + 1, lv["R"] @ (b - lv["M"] @ x))
+            x = x + (lv["R"].T @ xc) / MG_S
+            for _ in range(2):
+                x = x + MG_W * (b - lv["M"] @ x) / lv["D"]
+            return x
+
+        return rec(0, r)
+
+
+def vcycle_systems(A, L, b, solver, iters):
+    """Coarsest-level systems met by the first `iters` flexible-CG
+    iterations on (A, L, b), the coarsest solves done by the GPU solver."""
+    h = Hierarchy(A, L)
+    out = []
+    K = h.levels[-1]
+
+    def coarse(lv, bk):
+        g = np.zeros(lv["A"].shape, np.float32)
+        g[lv["A"] == fa.FLUID] = bk
+        x, _, _ = solver.solve(lv["A"], lv["L"], np.zeros_like(g), -g)
+        out.append((lv["A"], lv["L"], g, x.astype(np.float32)))
+        return x[lv["A"] == fa.FLUID].astype(np.float64)
+
+    M = h.levels[0]["M"]
+    r = b[A == fa.FLUID].astype(np.float64)
+    x = np.zeros_like(r)
+    z = h.vcycle(r, coarse)
+    s, zr = z.copy(), z @ r
+    for _ in range(iters - 1):
+        q = M @ s
+        a = zr / (s @ q)
+        x += a * s
+        r -= a * q
+        zn = h.vcycle(r, coarse)
+        beta = -a * (zn @ q) / zr
+        zr = zn @ r
+        s = zn + beta * s
+    assert K["A"].shape[0] == R
+    return out
+
+
 def scene_systems(n, scene, seed, steps_at):
     out = []
     with fa.FlipContext(n, tol_policy=1, tol=TOL, dt_policy=1) as ctx:
@@ -97,8 +189,9 @@ def scene_systems(n, scene, seed, steps_at):
     return out
 
 
-def generate(n_scenes, n_coarse_scenes, log):
-    """(A, L, b, x) systems at side R: b as restricted, x = A^-1 b."""
+def generate(n_scenes, n_coarse_scenes, vc_every, vc_coarse_every, vc_iters, log):
+    """(A, L, b, x) systems at side R: b as restricted (or as the V-cycle
+    restricts it), x = A^-1 b."""
     rs = np.random.RandomState(12345)
     data = []
     t0 = time.time()
@@ -114,22 +207,31 @@ def generate(n_scenes, n_coarse_scenes, log):
                 x, _, _ = solver.solve(Ac, Lc, np.zeros_like(rhs), -rhs)
                 data.append((Ac, Lc, rhs.astype(np.float32), x.astype(np.float32)))
 
+        nvc = 0
         for sc in range(n_scenes):
             steps = sorted(set(rs.randint(2, 25, size=2).tolist()))
             scene = fa.SCENE_BENCHMARK if sc % 50 == 0 else fa.SCENE_RANDOM
             for A, L, div in scene_systems(2 * R, scene, sc, steps):
                 add(A, L, div)
+                if vc_every > 0 and sc % vc_every == 0:
+                    got = vcycle_systems(A, L, div, solver, vc_iters)
+                    data += got
+                    nvc += len(got)
             if sc % 100 == 0:
-                log(f"N=64 scenes {sc}/{n_scenes}: {len(data)} systems, {time.time() - t0:.0f} s")
+                log(f"N=64 scenes {sc}/{n_scenes}: {len(data)} systems ({nvc} V-cycle), {time.time() - t0:.0f} s")
         n64 = len(data)
         for sc in range(n_coarse_scenes):
             scene = fa.SCENE_BENCHMARK if sc % 4 == 0 else fa.SCENE_RANDOM
             steps = sorted(set(rs.randint(2, 30, size=3).tolist()))
             for A, L, div in scene_systems(4 * R, scene, 100000 + sc, steps):
                 add(A, L, div)
+                if vc_coarse_every > 0 and sc % vc_coarse_every == 0:
+                    got = vcycle_systems(A, L, div, solver, vc_iters)
+                    data += got
+                    nvc += len(got)
             if sc % 20 == 0:
                 log(f"N=128 scenes {sc}/{n_coarse_scenes}: {len(data) - n64} systems, {time.time() - t0:.0f} s")
-    log(f"{n64} systems from N=64 scenes, {len(data) - n64} from N=128 scenes")
+    log(f"{n64} systems from N=64 scenes, {len(data) - n64} from N=128 scenes, {nvc} of them V-cycle right-hand sides")
     return data
 
 
@@ -214,7 +316,10 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--scenes", type=int, default=2000)
     ap.add_argument("--coarse-scenes", type=int, default=60)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--vcycle-every", type=int, default=8)
+    ap.add_argument("--vcycle-coarse-every", type=int, default=3)
+    ap.add_argument("--vcycle-iters", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=1)
@@ -223,7 +328,8 @@ def main():
     def log(msg):
         print(msg, flush=True)
 
-    data = generate(args.scenes, args.coarse_scenes, log)
+    data = generate(args.scenes, args.coarse_scenes, args.vcycle_every, args.vcycle_coarse_every, args.vcycle_iters,
+                    log)
     rs = np.random.RandomState(7)
     data = [data[i] for i in rs.permutation(len(data))]
     nv = max(8, len(data) // 50)
