@@ -142,19 +142,19 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t *__restrict__ 
 }
 
 // Split-K variant for small layers: one block per (16-voxel tile, 16-channel
-// output tile); its 4 waves take every 4th K step (the next step's operands
-// are loaded while the current MFMA runs) and are summed through LDS.  This
-// gives layers of a few hundred voxels (the bottom of the U-Net) 4 * NT
-// times the waves of k_conv3_fwd.
-// Input modes (the fused forms round to bf16 exactly as the stored
+// output tile); its KS waves take every KS-th K step (the next step's
+// operands are loaded while the current MFMA runs) and are summed through
+// LDS.  This gives layers of a few hundred voxels (the bottom of the U-Net)
+// KS * NT times the waves of k_conv3_fwd, and each wave a K loop of
+// STEPS / KS steps.
+// Input modes (the fused form rounds to bf16 exactly as the stored
 // intermediate would be rounded, so results equal the unfused network):
 //   IN_PLAIN  X
 //   IN_UP     up(U) + X: nearest upsampling of the (R/2)^3 tensor U (k_up_add)
-//   IN_POOL   pool(X): 2x2x2 average of the (2R)^3 tensor X (k_pool2)
 // HEAD: the layer is the last one (16 output channels, one tile); instead of
 // storing it, the block applies the 1x1x1 head and writes the solution
 // x = mask (w . y + b) sigma scale (k_head_fwd followed by the unit change).
-enum { IN_PLAIN = 0, IN_UP = 1, IN_POOL = 2 };
+enum { IN_PLAIN = 0, IN_UP = 1 };
 
 struct HeadArgs {
     const float *w, *b, *mask, *sigma;
@@ -169,8 +169,8 @@ __device__ __forceinline__ uint32_t add_bf16x2(uint32_t u, uint32_t x)
     return (uint32_t)lo | ((uint32_t)hi << 16);
 }
 
-template <int CIN, int COUT, int MODE, bool HEAD>
-__global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict__ X, const uint16_t *__restrict__ U,
+template <int CIN, int COUT, int MODE, bool HEAD, int KS>
+__global__ __launch_bounds__(64 * KS) void k_conv3_fwd_sk(const uint16_t *__restrict__ X, const uint16_t *__restrict__ U,
                                                       const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
                                                       uint16_t *Y, float *Yf, int R, long nvox, int relu, HeadArgs hd)
 {
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict
     constexpr int CPT = CIN / 8;
     constexpr int NCH = 27 * CPT;
     constexpr int STEPS = (NCH + 3) / 4;
-    __shared__ f32x4 red[3][64];
+    __shared__ f32x4 red[KS - 1][64];
     __shared__ float hy[16][17];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, grp = lane >> 4;
@@ -196,33 +196,12 @@ __global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict
         if (sv < 0) return zero8();
         if (MODE == IN_PLAIN) return load8(X + sv * CIN + c0);
         const int z = vo.z + tap / 9 - 1, y = vo.y + (tap / 3) % 3 - 1, x = vo.x + tap % 3 - 1;
-        if (MODE == IN_UP) {
-            const int Rc = R >> 1;
-            const long p = (((long)vo.s * Rc + (z >> 1)) * Rc + (y >> 1)) * Rc + (x >> 1);
-            const uint4 xs = *reinterpret_cast<const uint4 *>(X + sv * CIN + c0);
-            const uint4 us = *reinterpret_cast<const uint4 *>(U + p * CIN + c0);
-            return __builtin_bit_cast(bf16x8, make_uint4(add_bf16x2(us.x, xs.x), add_bf16x2(us.y, xs.y),
-                                                         add_bf16x2(us.z, xs.z), add_bf16x2(us.w, xs.w)));
-        }
-        // IN_POOL: X has side 2R; the eight children in k_pool2's order
-        const int Rf = R << 1;
-        float acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            const long f = (((long)vo.s * Rf + 2 * z + (d >> 2)) * Rf + 2 * y + ((d >> 1) & 1)) * Rf + 2 * x + (d & 1);
-            const uint4 u = *reinterpret_cast<const uint4 *>(X + f * CIN + c0);
-            const uint32_t w4[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                acc8[2 * h] += bf2f((uint16_t)(w4[h] & 0xffff));
-                acc8[2 * h + 1] += bf2f((uint16_t)(w4[h] >> 16));
-            }
-        }
-        uint32_t o[4];
-#pragma unroll
-        for (int h = 0; h < 4; ++h)
-            o[h] = (uint32_t)f2bf(acc8[2 * h] * 0.125f) | ((uint32_t)f2bf(acc8[2 * h + 1] * 0.125f) << 16);
-        return __builtin_bit_cast(bf16x8, make_uint4(o[0], o[1], o[2], o[3]));
+        const int Rc = R >> 1;
+        const long p = (((long)vo.s * Rc + (z >> 1)) * Rc + (y >> 1)) * Rc + (x >> 1);
+        const uint4 xs = *reinterpret_cast<const uint4 *>(X + sv * CIN + c0);
+        const uint4 us = *reinterpret_cast<const uint4 *>(U + p * CIN + c0);
+        return __builtin_bit_cast(bf16x8, make_uint4(add_bf16x2(us.x, xs.x), add_bf16x2(us.y, xs.y),
+                                                     add_bf16x2(us.z, xs.z), add_bf16x2(us.w, xs.w)));
     };
     auto load_b = [&](int st) -> bf16x8 {
         const int q = st * 4 + grp;
@@ -231,8 +210,8 @@ __global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict
     };
     f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     bf16x8 a = load_a(wave), b = load_b(wave);
-    for (int st = wave; st < STEPS; st += 4) {
-        const bf16x8 an = load_a(st + 4), bn = load_b(st + 4);
+    for (int st = wave; st < STEPS; st += KS) {
+        const bf16x8 an = load_a(st + KS), bn = load_b(st + KS);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         a = an;
         b = bn;
@@ -241,7 +220,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd_sk(const uint16_t *__restrict
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-        for (int w = 0; w < 3; ++w) {
+        for (int w = 0; w < KS - 1; ++w) {
             const f32x4 o = red[w][lane];
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i] += o[i];
@@ -552,6 +531,13 @@ __global__ void k_to_f32(long n, const uint16_t *__restrict__ a, float *b)
 
 inline unsigned nb(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// waves per block of the split-K kernel: about four K steps per wave,
+// between 4 and 16 waves
+constexpr int sk_waves(int cin)
+{
+    return ((27 * cin / 8 + 3) / 4) >= 48 ? 16 : (((27 * cin / 8 + 3) / 4) >= 24 ? 8 : 4);
+}
+
 template <int CIN, int COUT>
 void fwd_t(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, float *Yf, int R, long nvox, int relu,
            hipStream_t s)
@@ -561,8 +547,10 @@ void fwd_t(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y,
     const long waves = (nvox + 16 * MT - 1) / (16 * MT);
     if (waves < 2048) {
         // too few voxel tiles to fill the chip: split K and the output channels
-        hipLaunchKernelGGL((k_conv3_fwd_sk<CIN, COUT, IN_PLAIN, false>), dim3(nb(nvox, 16), COUT / 16), dim3(256), 0,
-                           s, X, (const uint16_t *)nullptr, W, bias, Y, Yf, R, nvox, relu, HeadArgs{});
+        constexpr int KS = sk_waves(CIN);
+        hipLaunchKernelGGL((k_conv3_fwd_sk<CIN, COUT, IN_PLAIN, false, KS>), dim3(nb(nvox, 16), COUT / 16),
+                           dim3(64 * KS), 0, s, X, (const uint16_t *)nullptr, W, bias, Y, Yf, R, nvox, relu,
+                           HeadArgs{});
         return;
     }
     hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, MT>), dim3(nb(nvox, (int)per_block)), dim3(256), 0, s, X, W, bias, Y,
@@ -597,25 +585,12 @@ void unet_conv3_fwd_up(int cin, int cout, const uint16_t *U, const uint16_t *X, 
 {
 #define CASE(a, b)                                                                                                   \
     if (cin == a && cout == b) {                                                                                     \
-        hipLaunchKernelGGL((k_conv3_fwd_sk<a, b, IN_UP, false>), dim3(nb(nvox, 16), b / 16), dim3(256), 0, s, X, U, W,  \
-                           bias, Y, (float *)nullptr, R, nvox, 1, HeadArgs{});                                       \
+        constexpr int KS = sk_waves(a);                                                                              \
+        hipLaunchKernelGGL((k_conv3_fwd_sk<a, b, IN_UP, false, KS>), dim3(nb(nvox, 16), b / 16), dim3(64 * KS), 0, s, \
+                           X, U, W, bias, Y, (float *)nullptr, R, nvox, 1, HeadArgs{});                              \
         return;                                                                                                      \
     }
     CASE(64, 32) CASE(32, 16) CASE(16, 16)
-#undef CASE
-    throw flip_arg_error{FLIP_E_ARG, "unsupported conv channel pair"};
-}
-
-void unet_conv3_fwd_pool(int cin, int cout, const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int R,
-                         long nvox, hipStream_t s)
-{
-#define CASE(a, b)                                                                                                   \
-    if (cin == a && cout == b) {                                                                                     \
-        hipLaunchKernelGGL((k_conv3_fwd_sk<a, b, IN_POOL, false>), dim3(nb(nvox, 16), b / 16), dim3(256), 0, s, X,     \
-                           (const uint16_t *)nullptr, W, bias, Y, (float *)nullptr, R, nvox, 1, HeadArgs{});         \
-        return;                                                                                                      \
-    }
-    CASE(16, 32) CASE(32, 64) CASE(64, 64)
 #undef CASE
     throw flip_arg_error{FLIP_E_ARG, "unsupported conv channel pair"};
 }
@@ -625,8 +600,9 @@ void unet_conv3_fwd_up_head(const uint16_t *U, const uint16_t *X, const uint16_t
                             float *x, int R, long nvox, hipStream_t s)
 {
     HeadArgs hd{hw, hb, mask, sigma, scale, x};
-    hipLaunchKernelGGL((k_conv3_fwd_sk<16, 16, IN_UP, true>), dim3(nb(nvox, 16), 1), dim3(256), 0, s, X, U, W, bias,
-                       (uint16_t *)nullptr, (float *)nullptr, R, nvox, 1, hd);
+    constexpr int KS = sk_waves(16);
+    hipLaunchKernelGGL((k_conv3_fwd_sk<16, 16, IN_UP, true, KS>), dim3(nb(nvox, 16), 1), dim3(64 * KS), 0, s, X, U, W,
+                       bias, (uint16_t *)nullptr, (float *)nullptr, R, nvox, 1, hd);
 }
 
 void unet_conv3_wgrad(int cin, int cout, const uint16_t *X, const uint16_t *dY, float *partial, float *dW, int R,

@@ -512,6 +512,12 @@ __global__ __launch_bounds__(TB) void k_fcg_b(const int32_t *__restrict__ iflags
     store_partials(pb, 1, rr, 0.0, 0.0, 1);
 }
 
+// the stop flag into mapped host memory, at the end of every iteration
+__global__ void k_fcg_flag(const int32_t *__restrict__ fi, int32_t *hflag)
+{
+    if (threadIdx.x == 0) hflag[0] = fi[FI_DONE];
+}
+
 struct Level {
     int R = 0;
     long C = 0;
@@ -540,7 +546,8 @@ struct LearnedSolver {
     double *fs = nullptr;
     int32_t *fi = nullptr;
     double *upart = nullptr;
-    int32_t *h_flag = nullptr;  // pinned: stop flags of the last two launches
+    int32_t *h_flag = nullptr;  // mapped pinned host word: the stop flag (monotonic within a solve)
+    int32_t *d_flag = nullptr;  // its device address
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipGraphExec_t graph = nullptr;
     int graph_subcell = -1;
@@ -616,7 +623,8 @@ static LearnedSolver *learned_get(FlipCtx *c)
         dmalloc(&ls->fs, 8);
         dmalloc(&ls->fi, 4);
         dmalloc(&ls->upart, 2 * (long)nb(ls->lv[ls->K].C));
-        FLIP_CHECK(hipHostMalloc((void **)&ls->h_flag, sizeof(int32_t) * 2));
+        FLIP_CHECK(hipHostMalloc((void **)&ls->h_flag, sizeof(int32_t) * 2, hipHostMallocMapped));
+        FLIP_CHECK(hipHostGetDevicePointer((void **)&ls->d_flag, ls->h_flag, 0));
         FLIP_CHECK(hipEventCreateWithFlags(&ls->ev[0], hipEventDisableTiming));
         FLIP_CHECK(hipEventCreateWithFlags(&ls->ev[1], hipEventDisableTiming));
     } catch (...) {
@@ -714,6 +722,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
         launch_timed(c, "fcg_b", k_fcg_b, G, TB, c->iflags, ls->fi, c->fl, c->press, c->r, c->zs[0], c->zs[1], c->q,
                      pa, ls->fs, pb, G);
         vcycle(c, ls, G, h2, subcell, c->q, pz, pb, true);  // its first kernel applies the stop test
+        hipLaunchKernelGGL(k_fcg_flag, 1, 64, 0, c->stream, ls->fi, ls->d_flag);
     };
     if (!c->profiling && !ls->graph) {
         hipGraph_t g;
@@ -724,19 +733,20 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
         FLIP_CHECK(hipGraphDestroy(g));
         ls->graph_subcell = subcell;
     }
-    // launch iteration k, then wait for iteration k-1's stop flag
+    // launch iteration k, then wait for iteration k-1 and read the stop flag
+    // (a set flag stays set, so reading a later iteration's write is fine)
+    volatile int32_t *flag = ls->h_flag;
+    flag[0] = 0;
     for (int k = 0; k < FCG_MAX; ++k) {
         if (c->profiling) {
             iteration();
         } else {
             FLIP_CHECK(hipGraphLaunch(ls->graph, c->stream));
         }
-        FLIP_CHECK(hipMemcpyAsync(ls->h_flag + (k & 1), ls->fi + FI_DONE, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                  c->stream));
         FLIP_CHECK(hipEventRecord(ls->ev[k & 1], c->stream));
         if (k > 0) {
             FLIP_CHECK(hipEventSynchronize(ls->ev[(k - 1) & 1]));
-            if (ls->h_flag[(k - 1) & 1]) break;
+            if (flag[0]) break;
         }
     }
     int32_t hfi[4];

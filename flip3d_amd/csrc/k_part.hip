@@ -693,15 +693,82 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
     return events;
 }
 
-__global__ __launch_bounds__(TB) void k_spring(int64_t np, int n, const float4 *__restrict__ pos,
-                                               const int32_t *__restrict__ meta, const float4 *__restrict__ nrm_id,
-                                               const int32_t *__restrict__ cs, float dt, float re, float4 *tmp0,
-                                               int32_t *list, int32_t *count)
+// The spring pass in two phases per particle, so that a wave's lanes run
+// the expensive near-neighbour arithmetic together: phase 1 scans the window
+// with the cheap squared-distance test and lists the survivors (window
+// order) in LDS; phase 2 applies spring_one's arithmetic to the list in that
+// order, so every sum is formed in the same order as in spring_one.  More
+// than SPRING_NEAR survivors (never seen in practice) falls back to
+// spring_one itself.
+constexpr int SPRING_TB = 256;
+constexpr int SPRING_NEAR = 32;
+
+__global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const float4 *__restrict__ pos,
+                                                      const int32_t *__restrict__ meta,
+                                                      const float4 *__restrict__ nrm_id,
+                                                      const int32_t *__restrict__ cs, float dt, float re, float4 *tmp0,
+                                                      int32_t *list, int32_t *count)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int32_t near[SPRING_NEAR][SPRING_TB];
+    const int tid = threadIdx.x;
+    int64_t i = (int64_t)blockIdx.x * SPRING_TB + tid;
     if (i >= np) return;
-    if (is_wall_w(pos[i].w)) return;
-    int r = spring_one<false>(i, n, pos, meta, nrm_id, cs, dt, re, nullptr, tmp0);
+    const float4 p = pos[i];
+    if (is_wall_w(p.w)) return;
+    float g = (float)n;
+    float cx = g * p.x, cy = g * p.y, cz = g * p.z;
+    float x = g > cx ? cx : g;
+    x = 0.0f > x ? 0.0f : x;
+    float y = g > cy ? cy : g;
+    y = 0.0f > y ? 0.0f : y;
+    float z = g > cz ? cz : g;
+    z = 0.0f > z ? 0.0f : z;
+    int ci = (int)x, cj = (int)y, ck = (int)z;
+    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    const float far2 = re * re * 1.0001f;  // as in spring_one
+    int cnt = 0;
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            if (klo > khi) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            for_range4(pos, b, e, [&](const float4 &q, int a) {
+                if (a == i) return;
+                if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
+                if (cnt < SPRING_NEAR) near[cnt][tid] = a;
+                ++cnt;
+            });
+        }
+    }
+    int r = 0;
+    if (cnt > SPRING_NEAR) {
+        r = spring_one<false>(i, n, pos, meta, nrm_id, cs, dt, re, nullptr, tmp0);
+    } else {
+        const double rre = 0.1 * (double)re;
+        const double cw = 0.05 * (double)re / (double)dt;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (int k = 0; k < cnt; ++k) {
+            const int a = near[k][tid];
+            const float4 q = pos[a];
+            float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+            float w = (float)(50.0 * (double)mass_of(q.w) * (double)smooth_kernel(dist * dist, re));
+            if ((double)dist > rre) {
+                sx += w * (p.x - q.x) / dist * re;
+                sy += w * (p.y - q.y) / dist * re;
+                sz += w * (p.z - q.z) / dist * re;
+            } else if (!is_wall_w(q.w)) {
+                r = -1;  // a random-jitter event: the host replays this particle
+                break;
+            } else {
+                float4 nr = nrm_id[meta_id(meta[a])];
+                sx = (float)((double)sx + cw * (double)nr.x);
+                sy = (float)((double)sy + cw * (double)nr.y);
+                sz = (float)((double)sz + cw * (double)nr.z);
+            }
+        }
+        if (r == 0) tmp0[i] = make_float4(p.x + dt * sx, p.y + dt * sy, p.z + dt * sz, 0.0f);
+    }
     if (r < 0) {
         int32_t at = atomicAdd(count, 1);
         list[2 * at] = meta_id(meta[i]);
@@ -1032,8 +1099,8 @@ int k_correct(FlipCtx *c)
     float dt = (float)c->dt;
     float re = (float)(c->prm.density / n);
     FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, sizeof(int32_t), c->stream));
-    launch_timed(c, "spring", k_spring, nblk(c->np, TB), TB, c->np, n, c->P4(), c->M(), c->nrm_id, c->cell_start, dt,
-                 re, c->tmp0, c->ev_list, c->ev_count);
+    launch_timed(c, "spring", k_spring, nblk(c->np, SPRING_TB), SPRING_TB, c->np, n, c->P4(), c->M(), c->nrm_id,
+                 c->cell_start, dt, re, c->tmp0, c->ev_list, c->ev_count);
     FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     FLIP_CHECK(hipStreamSynchronize(c->stream));
     int m = c->h_pinned[0];

@@ -461,33 +461,45 @@ void unet_backward(UNet *u, int S)
 // Features of one sample with sigma from per-block partials (sum b^2, FLUID
 // count) that the producer of b wrote; every block sums them in the same
 // order, block 0 stores sigma for the head.
-__global__ void k_features_p(const int8_t *__restrict__ A, const float *__restrict__ L, const float *__restrict__ b,
-                             const double *__restrict__ part, int nparts, long n, uint16_t *F, float *mask,
-                             float *sigma_out)
+__global__ __launch_bounds__(256) void k_features_p(const int8_t *__restrict__ A, const float *__restrict__ L,
+                                                    const float *__restrict__ b, const double *__restrict__ part,
+                                                    int nparts, long n, uint16_t *F, float *mask, float *sigma_out)
 {
+    __shared__ double s0[4], s1[4];
     __shared__ float sig;
+    double a = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += 256) {
+        a += part[2 * k];
+        c += part[2 * k + 1];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s0[threadIdx.x >> 6] = a;
+        s1[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        double a = 0.0, c = 0.0;
-        for (int k = 0; k < nparts; ++k) {
-            a += part[2 * k];
-            c += part[2 * k + 1];
-        }
-        double v = c > 0.0 ? sqrt(a / c) : 0.0;
+        const double ta = s0[0] + s0[1] + s0[2] + s0[3], tc = s1[0] + s1[1] + s1[2] + s1[3];
+        const double v = tc > 0.0 ? sqrt(ta / tc) : 0.0;
         sig = v > 0.0 ? (float)v : 1.0f;
         if (blockIdx.x == 0) sigma_out[0] = sig;
     }
     __syncthreads();
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    int a = A[g];
+    int av = A[g];
     float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (a == FLIP_FLUID) f[0] = b[g] / sig;
-    f[1] = a == FLIP_FLUID;
-    f[2] = a == FLIP_AIR;
-    f[3] = a == FLIP_WALL;
-    if (a != FLIP_WALL) f[4] = fminf(fmaxf(L[g], -2.0f), 2.0f);
-    for (int c = 0; c < 8; ++c) F[g * 8 + c] = f2bf_d(f[c]);
-    mask[g] = a == FLIP_FLUID ? 1.0f : 0.0f;
+    if (av == FLIP_FLUID) f[0] = b[g] / sig;
+    f[1] = av == FLIP_FLUID;
+    f[2] = av == FLIP_AIR;
+    f[3] = av == FLIP_WALL;
+    if (av != FLIP_WALL) f[4] = fminf(fmaxf(L[g], -2.0f), 2.0f);
+    for (int ch = 0; ch < 8; ++ch) F[g * 8 + ch] = f2bf_d(f[ch]);
+    mask[g] = av == FLIP_FLUID ? 1.0f : 0.0f;
 }
 
 // per-block partials (sum b^2 over FLUID cells, FLUID count), blocks of 256
@@ -516,9 +528,9 @@ __global__ void k_rhs_partials(const int8_t *__restrict__ A, const float *__rest
     }
 }
 
-// Inference forward of one sample: pooling and the decoder's upsample-adds
-// are formed inside the convolutions, and the last convolution applies the
-// head and writes the solution directly.
+// Inference forward of one sample: the decoder's upsample-adds are formed
+// inside the convolutions, and the last convolution applies the head and
+// writes the solution directly.
 static void unet_infer(UNet *u, const float *sigma, float *x)
 {
     hipStream_t s = u->stream;
@@ -527,9 +539,12 @@ static void unet_infer(UNet *u, const float *sigma, float *x)
     auto W = [&](int i) { return u->pwb + u->conv[i].w_off; };
     auto B = [&](int i) { return u->pw + u->conv[i].b_off; };
     unet_conv3_fwd(UNET_FEATURES, 16, u->F0, W(0), B(0), u->e0, nullptr, R0, v0, true, s);
-    unet_conv3_fwd_pool(16, 32, u->e0, W(1), B(1), u->e1, R1, v1, s);
-    unet_conv3_fwd_pool(32, 64, u->e1, W(2), B(2), u->e2, R2, v2, s);
-    unet_conv3_fwd_pool(64, 64, u->e2, W(3), B(3), u->e3, R3, v3, s);
+    unet_pool2(u->e0, R0, 16, v1, u->p0, s);
+    unet_conv3_fwd(16, 32, u->p0, W(1), B(1), u->e1, nullptr, R1, v1, true, s);
+    unet_pool2(u->e1, R1, 32, v2, u->p1, s);
+    unet_conv3_fwd(32, 64, u->p1, W(2), B(2), u->e2, nullptr, R2, v2, true, s);
+    unet_pool2(u->e2, R2, 64, v3, u->p2, s);
+    unet_conv3_fwd(64, 64, u->p2, W(3), B(3), u->e3, nullptr, R3, v3, true, s);
     unet_conv3_fwd_up(64, 32, u->e3, u->e2, W(4), B(4), u->d2, R2, v2, s);
     unet_conv3_fwd_up(32, 16, u->d2, u->e1, W(5), B(5), u->d1, R1, v1, s);
     const float scale = OUT_UNIT / ((float)u->R * (float)u->R);

@@ -15,10 +15,6 @@ void unet_conv3_fwd(int cin, int cout, const uint16_t *X, const uint16_t *W, con
 // bf16 result equals conv of k_up_add's output)
 void unet_conv3_fwd_up(int cin, int cout, const uint16_t *U, const uint16_t *X, const uint16_t *W, const float *bias,
                        uint16_t *Y, int R, long nvox, hipStream_t s);
-// conv(pool(X)) with leaky ReLU, X of side 2R pooled on the fly (equals conv
-// of k_pool2's output)
-void unet_conv3_fwd_pool(int cin, int cout, const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int R,
-                         long nvox, hipStream_t s);
 // last layer conv(up(U) + X) (16 -> 16) with the head and the unit change
 // fused: x = mask (hw . y + hb) sigma scale
 void unet_conv3_fwd_up_head(const uint16_t *U, const uint16_t *X, const uint16_t *W, const float *bias,
