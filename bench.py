@@ -106,9 +106,9 @@ def cpu_baseline(n, seconds_budget):
     }
 
 
-def run_mode(fa, n, precond, weights, steps, warmup):
-    """Timed steps of one solver mode from the initial scene, then one
-    untimed step with every kernel launch timed."""
+def run_mode(fa, n, precond, weights, steps, warmup, profile_step=True):
+    """Timed steps of one solver mode from the initial scene, then (unless
+    profile_step is off) one untimed step with every kernel launch timed."""
     code = {"reference": fa.PRECOND_REFERENCE, "ic0": fa.PRECOND_IC0, "learned": fa.PRECOND_LEARNED}[precond]
     ctx = fa.FlipContext(n, tol_policy=1, tol=TOL, dt_policy=1, precond=code)
     if precond == "learned":
@@ -124,10 +124,12 @@ def run_mode(fa, n, precond, weights, steps, warmup):
         stats.append(ctx.step())
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    ctx.set_profiling(True)
-    ctx.step()
-    ktimes = ctx.kernel_times()
-    ctx.set_profiling(False)
+    ktimes = []
+    if profile_step:
+        ctx.set_profiling(True)
+        ctx.step()
+        ktimes = ctx.kernel_times()
+        ctx.set_profiling(False)
     ctx.close()
     agg = {}
     for name, ms in ktimes:
@@ -138,7 +140,7 @@ def run_mode(fa, n, precond, weights, steps, warmup):
         "precond": precond, "steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
         "particles": n_particles, "n_fluid": stats[-1]["n_fluid_cells"],
         "iterations_mean": mean("iterations"), "learned_iterations_mean": mean("learned_iterations"),
-        "solve_ms_mean": mean("ms_solve"),
+        "solve_ms_mean": mean("ms_solve"), "repositioned_mean": mean("n_reposition"),
         "rel_residual_max": max((st["error2"] / max(st["bnorm2"], 1e-300)) ** 0.5 for st in stats),
         "agg": agg,
     }
@@ -176,6 +178,8 @@ def main():
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS,
                     help="coarse-level network weights for --precond learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile-step", action="store_true",
+                    help="skip the per-launch timed step (for runs under an external profiler)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -187,9 +191,10 @@ def main():
     import flip3d_amd as fa
 
     n = args.n
-    plain = run_mode(fa, n, "reference", None, args.steps, args.warmup)
+    prof = not args.no_profile_step
+    plain = run_mode(fa, n, "reference", None, args.steps, args.warmup, prof)
     main_run = plain if args.precond == "reference" else run_mode(fa, n, args.precond, args.weights, args.steps,
-                                                                 args.warmup)
+                                                                 args.warmup, prof)
     n_particles = main_run["particles"]
     roofline = roofline_of(main_run["agg"], n, n_particles, main_run["n_fluid"])
 

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <unordered_map>
 
 using namespace flipdev;
 
@@ -474,144 +475,55 @@ __global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const fl
 }
 
 // flip3D.cpp:445-450: resample each repositioned particle in list order with
-// re = h = 1/N (corrector.cpp:15-47), u starting from 0.  One block: weights
-// are evaluated in parallel, summed serially in window order, and each
-// particle's result is visible to the next (the reference's loop order).
-// A listed particle with no other listed particle inside its kernel support
-// (length2 < h^2) reads no velocity another listed particle writes, and (the
-// relation being symmetric) none reads its own, so those resample in parallel
-// here, one thread each with the reference's serial sum; dep[t] marks the
-// others, which k_resample_list then does in list order.
-__global__ __launch_bounds__(TB) void k_resample_list_indep(int m, const int32_t *__restrict__ ids,
-                                                            const int32_t *__restrict__ inv, int n,
-                                                            const float4 *__restrict__ pos, float4 *vel,
-                                                            const int32_t *__restrict__ cs, float hh, int32_t *dep)
+// re = h = 1/N (corrector.cpp:15-47), u starting from 0, each particle's
+// result visible to the later ones (the reference's loop order).  A listed
+// particle reads another listed particle's velocity exactly when the two lie
+// within the kernel support (length2 < h^2, a symmetric test), so the list
+// splits into dependency levels: level(t) = 1 + the highest level of an
+// earlier listed particle within t's support (the host forms them from the
+// new positions).  Particles of one level never read each other, and every
+// particle they read from the list is on a lower level (earlier in the list,
+// new value) or a higher one (later in the list, old value), exactly as in
+// the serial loop.  One block walks the levels; within a level each thread
+// resamples one particle with the reference's serial window sum.
+__global__ __launch_bounds__(1024) void k_resample_levels(int nlev, const int32_t *__restrict__ loff,
+                                                          const int32_t *__restrict__ order,
+                                                          const int32_t *__restrict__ ids,
+                                                          const int32_t *__restrict__ inv, int n,
+                                                          const float4 *__restrict__ pos, float4 *vel,
+                                                          const int32_t *__restrict__ cs, float hh)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    int32_t slot = inv[ids[t]];
-    float4 p = pos[slot];
-    int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
-    // the resample reads a neighbour's velocity only when length2 < hh (the
-    // weight test below); that relation is symmetric
-    bool d = false;
-    for (int u = 0; u < m && !d; ++u) {
-        if (u == t) continue;
-        float4 q = pos[inv[ids[u]]];
-        d = length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh;
-    }
-    dep[t] = d ? 1 : 0;
-    if (d) return;
-    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
-    float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
-    for (int si = ci - 1; si <= ci + 1; ++si) {
-        if (si < 0 || si > n - 1) continue;
-        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
-            if (sj < 0 || sj > n - 1) continue;
-            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for (int a = b; a < e; ++a) {
-                float4 q = pos[a];
-                if (is_wall_w(q.w)) continue;
-                float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
-                if (!(d2 < hh)) continue;
-                float w = q.w * sharp_kernel_hh(d2, hh);
-                float4 u = vel[a];
-                ax += w * u.x;
-                ay += w * u.y;
-                az += w * u.z;
-                wsum += w;
-            }
-        }
-    }
-    float4 v = vel[slot];
-    if (wsum != 0.0f) {
-        v.x = ax / wsum;
-        v.y = ay / wsum;
-        v.z = az / wsum;
-    } else {
-        v.x = v.y = v.z = 0.0f;
-    }
-    vel[slot] = v;
-}
-
-constexpr int RCH = 1024;
-__global__ __launch_bounds__(TB) void k_resample_list(int m, const int32_t *__restrict__ ids, const int32_t *inv,
-                                                      int n, const float4 *__restrict__ pos, float4 *vel,
-                                                      const int32_t *__restrict__ cs, float hh,
-                                                      const int32_t *__restrict__ dep)
-{
-    __shared__ float sw[RCH], sx[RCH], sy[RCH], sz[RCH];
-    __shared__ int rb[9], re_[9];
-    __shared__ float acc[4];
-    for (int t = 0; t < m; ++t) {
-        if (!dep[t]) continue;
-        int32_t slot = inv[ids[t]];
-        float4 p = pos[slot];
-        int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
-        int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
-        if (threadIdx.x < 9) {
-            int si = ci - 1 + threadIdx.x / 3, sj = cj - 1 + threadIdx.x % 3;
-            int b = 0, e = 0;
-            if (si >= 0 && si < n && sj >= 0 && sj < n) {
-                b = cs[cidx(n, si, sj, klo)];
-                e = cs[cidx(n, si, sj, khi) + 1];
-            }
-            rb[threadIdx.x] = b;
-            re_[threadIdx.x] = e;
-        }
-        if (threadIdx.x == 0) acc[0] = acc[1] = acc[2] = acc[3] = 0.0f;
-        __syncthreads();
-        int total = 0;
-        for (int r = 0; r < 9; ++r) total += re_[r] - rb[r];
-        for (int base = 0; base < total; base += RCH) {
-            for (int e = threadIdx.x; e < RCH && base + e < total; e += blockDim.x) {
-                int g = base + e, r = 0;
-                while (g >= re_[r] - rb[r]) {
-                    g -= re_[r] - rb[r];
-                    ++r;
+    for (int l = 0; l < nlev; ++l) {
+        for (int x = loff[l] + (int)threadIdx.x; x < loff[l + 1]; x += blockDim.x) {
+            const int32_t slot = inv[ids[order[x]]];
+            const float4 p = pos[slot];
+            const int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+            const int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+            float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
+            for (int si = ci - 1; si <= ci + 1; ++si) {
+                if (si < 0 || si > n - 1) continue;
+                for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+                    if (sj < 0 || sj > n - 1) continue;
+                    const int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+                    for (int a = b; a < e; ++a) {
+                        const float4 q = pos[a];
+                        if (is_wall_w(q.w)) continue;
+                        const float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
+                        if (!(d2 < hh)) continue;
+                        const float w = q.w * sharp_kernel_hh(d2, hh);
+                        const float4 u = vel[a];
+                        ax += w * u.x;
+                        ay += w * u.y;
+                        az += w * u.z;
+                        wsum += w;
+                    }
                 }
-                int a = rb[r] + g;
-                float4 q = pos[a];
-                float w = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
-                if (!is_wall_w(q.w)) {
-                    float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
-                    w = q.w * sharp_kernel_hh(d2, hh);
-                    float4 u = vel[a];
-                    wx = w * u.x;
-                    wy = w * u.y;
-                    wz = w * u.z;
-                } else {
-                    w = -1.0f;  // marks "not a FLUID neighbour"
-                }
-                sw[e] = w;
-                sx[e] = wx;
-                sy[e] = wy;
-                sz[e] = wz;
             }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                int lim = min(RCH, total - base);
-                float ax = acc[0], ay = acc[1], az = acc[2], aw = acc[3];
-                for (int e = 0; e < lim; ++e) {
-                    if (sw[e] < 0.0f) continue;
-                    ax += sx[e];
-                    ay += sy[e];
-                    az += sz[e];
-                    aw += sw[e];
-                }
-                acc[0] = ax;
-                acc[1] = ay;
-                acc[2] = az;
-                acc[3] = aw;
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
             float4 v = vel[slot];
-            if (acc[3] != 0.0f) {
-                v.x = acc[0] / acc[3];
-                v.y = acc[1] / acc[3];
-                v.z = acc[2] / acc[3];
+            if (wsum != 0.0f) {
+                v.x = ax / wsum;
+                v.y = ay / wsum;
+                v.z = az / wsum;
             } else {
                 v.x = v.y = v.z = 0.0f;
             }
@@ -1074,13 +986,47 @@ int k_stuck_and_reposition(FlipCtx *c)
                                       c->stream));
             hipLaunchKernelGGL(k_set_pos_by_id, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_rand, c->inv,
                                c->P4());
+            // dependency levels of the list (see k_resample_levels)
+            const float hh = h * h;
+            std::vector<int32_t> lev(m, 0);
+            std::unordered_map<long, std::vector<int32_t>> bucket;
+            int nlev = 0;
+            for (int t = 0; t < m; ++t) {
+                const float *pt = &xyz[3 * t];
+                const int bi = (int)(pt[0] * (float)n), bj = (int)(pt[1] * (float)n), bk = (int)(pt[2] * (float)n);
+                int l = 0;
+                for (int di = -1; di <= 1; ++di)
+                    for (int dj = -1; dj <= 1; ++dj)
+                        for (int dk = -1; dk <= 1; ++dk) {
+                            auto it = bucket.find(((long)(bi + di) * (n + 2) + (bj + dj)) * (n + 2) + (bk + dk));
+                            if (it == bucket.end()) continue;
+                            for (int32_t u : it->second) {
+                                const float *pu = &xyz[3 * u];
+                                const float dx = pt[0] - pu[0], dy = pt[1] - pu[1], dz = pt[2] - pu[2];
+                                const float d2 = dx * dx + dy * dy + dz * dz;
+                                if (d2 < hh) l = std::max(l, lev[u] + 1);
+                            }
+                        }
+                lev[t] = l;
+                nlev = std::max(nlev, l + 1);
+                bucket[((long)bi * (n + 2) + bj) * (n + 2) + bk].push_back(t);
+            }
+            std::vector<int32_t> loff(nlev + 1, 0), order(m);
+            for (int t = 0; t < m; ++t) loff[lev[t] + 1]++;
+            for (int l = 0; l < nlev; ++l) loff[l + 1] += loff[l];
+            {
+                std::vector<int32_t> fill(loff.begin(), loff.end() - 1);
+                for (int t = 0; t < m; ++t) order[fill[lev[t]]++] = t;
+            }
             k_launch_sort(c);
             t_begin(c, "reposition_resample");
-            int32_t *dep = c->ev_off;  // free again after the deep-cell list was read
-            hipLaunchKernelGGL(k_resample_list_indep, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->inv, n,
-                               c->P4(), c->V4(), c->cell_start, h * h, dep);
-            hipLaunchKernelGGL(k_resample_list, 1, TB, 0, c->stream, m, c->ev_list, c->inv, n, c->P4(), c->V4(),
-                               c->cell_start, h * h, dep);
+            // ev_off (free again after the deep-cell list was read) holds the
+            // level offsets, deep (the sort's scratch) the order
+            FLIP_CHECK(hipMemcpyAsync(c->ev_off, loff.data(), sizeof(int32_t) * (nlev + 1), hipMemcpyHostToDevice,
+                                      c->stream));
+            FLIP_CHECK(hipMemcpyAsync(c->deep, order.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(k_resample_levels, 1, 1024, 0, c->stream, nlev, c->ev_off, c->deep, c->ev_list, c->inv,
+                               n, c->P4(), c->V4(), c->cell_start, hh);
             t_end(c);
             FLIP_CHECK(hipStreamSynchronize(c->stream));
             moved = m;
