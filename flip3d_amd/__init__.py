@@ -18,7 +18,7 @@ GRID_U0, GRID_U1, GRID_U2, GRID_US0, GRID_US1, GRID_US2, GRID_A, GRID_L, GRID_PR
 PRECOND_REFERENCE, PRECOND_IC0, PRECOND_LEARNED = 0, 1, 2
 (STAGE_DENSITY, STAGE_EXT_FORCE, STAGE_P2G, STAGE_PROJECT, STAGE_EXTRAP, STAGE_G2P, STAGE_ADVECT,
  STAGE_CORRECT) = range(8)
-SCENE_REFERENCE, SCENE_BENCHMARK, SCENE_RANDOM = 0, 1, 2
+SCENE_REFERENCE, SCENE_BENCHMARK, SCENE_RANDOM, SCENE_RANDOM_BOXES = 0, 1, 2, 3
 BENCH_SEED = 20261203
 
 EXPORTS = [
@@ -28,7 +28,7 @@ EXPORTS = [
     "flip_gpu_set_dt", "flip_gpu_get_grid", "flip_gpu_set_grid", "flip_gpu_step", "flip_gpu_stage",
     "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
-    "flip_gpu_save_state", "flip_gpu_load_state",
+    "flip_gpu_save_state", "flip_gpu_load_state", "flip_gpu_step_many",
     "flip_gpu_set_learned_weights", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
     "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
     "flip_unet_predict", "flip_unet_save", "flip_unet_load",
@@ -90,6 +90,7 @@ def lib():
         L.flip_gpu_set_grid.argtypes = [P, C.c_int, P]
         L.flip_gpu_step.argtypes = [P, C.POINTER(Stats)]
         L.flip_gpu_stage.argtypes = [P, C.c_int, C.POINTER(Stats)]
+        L.flip_gpu_step_many.argtypes = [C.POINTER(P), C.c_int, C.c_int, C.POINTER(Stats), C.POINTER(C.c_double)]
         L.flip_gpu_map_p2g.argtypes = [P, P, P, P]
         L.flip_gpu_map_g2p.argtypes = [P, P, P, P, C.c_int64, P, P]
         L.flip_gpu_solve.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(Stats)]
@@ -302,6 +303,18 @@ class FlipContext:
 
     def load_state(self, path):
         _check(lib().flip_gpu_load_state(self.h, os.fsencode(path)))
+
+
+def step_many(contexts, threads=8):
+    """One step of every FlipContext in `contexts`, driven concurrently by
+    `threads` host threads (flip_gpu_step_many); returns (stats dicts,
+    per-context step wall-clock in ms)."""
+    n = len(contexts)
+    arr = (C.c_void_p * n)(*[c.h for c in contexts])
+    st = (Stats * n)()
+    ms = (C.c_double * n)()
+    _check(lib().flip_gpu_step_many(arr, n, threads, st, ms))
+    return [st[i].as_dict() for i in range(n)], [ms[i] for i in range(n)]
 
 
 class UNet:

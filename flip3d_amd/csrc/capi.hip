@@ -3,10 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "flip_internal.hpp"
@@ -703,6 +707,46 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         timing_collect(c);
         return FLIP_OK;
     });
+}
+
+int flip_gpu_step_many(flip_ctx **ctxs, int count, int threads, flip_step_stats *stats, double *ms)
+{
+    if (!ctxs || count <= 0) return fail(FLIP_E_ARG, "no contexts");
+    if (threads <= 0) return fail(FLIP_E_ARG, "threads must be >= 1");
+    for (int i = 0; i < count; ++i)
+        if (!ctxs[i]) return fail(FLIP_E_ARG, "null context");
+    for (int i = 0; i < count; ++i)
+        for (int j = i + 1; j < count; ++j)
+            if (ctxs[i] == ctxs[j]) return fail(FLIP_E_ARG, "a context appears twice");
+    threads = std::min(threads, count);
+    std::atomic<int> next{0};
+    std::mutex mu;
+    int first_rc = FLIP_OK, first_at = count;
+    std::string first_msg;
+    auto worker = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= count) return;
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = flip_gpu_step(ctxs[i], stats ? &stats[i] : nullptr);
+            const auto t1 = std::chrono::steady_clock::now();
+            if (ms) ms[i] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            if (rc < 0) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (i < first_at) {
+                    first_at = i;
+                    first_rc = rc;
+                    first_msg = flip_gpu_last_error();
+                }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    if (first_rc < 0) return fail(first_rc, ("context " + std::to_string(first_at) + ": " + first_msg).c_str());
+    return FLIP_OK;
 }
 
 int flip_gpu_map_p2g(flip_ctx *ctx, float *u0, float *u1, float *u2)

@@ -263,22 +263,25 @@ extern "C" int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int
                 rng.next();
                 rng.next();
             });
-        } else if (scene == FLIP_SCENE_RANDOM) {
-            // 1-4 liquid boxes (edges U(0.1, 0.6)) or spheres (radius U(0.05, 0.2)),
-            // centres uniform inside the walls; draws from SplitMix64(seed) at
-            // indices 1e9 + i so the jitter stream below stays independent
+        } else if (scene == FLIP_SCENE_RANDOM || scene == FLIP_SCENE_RANDOM_BOXES) {
+            // FLIP_SCENE_RANDOM: 1-4 liquid boxes (edges U(0.1, 0.6)) or spheres
+            // (radius U(0.05, 0.2)); FLIP_SCENE_RANDOM_BOXES: 1-3 boxes.  Centres
+            // uniform inside the walls; draws from SplitMix64(seed) at indices
+            // 1e9 + i so the jitter stream below stays independent
+            const bool boxes = scene == FLIP_SCENE_RANDOM_BOXES;
+            const int maxobj = boxes ? 3 : 4;
             uint64_t draw = 1000000000ull;
             auto uni = [&](double a, double b) {
                 double uu = (double)(splitmix_at(seed, draw++) >> 11) * (1.0 / 9007199254740992.0);
                 return a + (b - a) * uu;
             };
-            int nobj = 1 + (int)(uni(0.0, 1.0) * 4.0);
-            if (nobj > 4) nobj = 4;
+            int nobj = 1 + (int)(uni(0.0, 1.0) * (double)maxobj);
+            if (nobj > maxobj) nobj = maxobj;
             for (int m = 0; m < nobj; ++m) {
                 Obj b{};
                 b.type = FLIP_FLUID;
                 double cx = uni(wt, 1.0 - wt), cy = uni(wt, 1.0 - wt), cz = uni(wt, 1.0 - wt);
-                if (uni(0.0, 1.0) < 0.5) {
+                if (boxes || uni(0.0, 1.0) < 0.5) {
                     b.shape = 0;
                     double e[3] = {uni(0.1, 0.6), uni(0.1, 0.6), uni(0.1, 0.6)};
                     double c3[3] = {cx, cy, cz};

@@ -78,11 +78,11 @@ def main():
             "particle_u_err": rel(pa["u"][fl], pb["u"][fl]),
             "iterations": sa["iterations"], "refine_sweeps": sa["refine_sweeps"],
             "rel_residual": float(np.sqrt(sa["error2"] / max(sa["bnorm2"], 1e-300))),
-            "fp64_iterations": sb["iterations"],
+            "fp64_iterations": sb["iterations"], "ms_step": round(sa["ms_step"], 3), "ms_solve": round(sa["ms_solve"], 3),
         }
         recs.append(rec)
         print(json.dumps(rec), flush=True)
-    keys = ("press_err", "face_u_err", "particle_u_err", "rel_residual")
+    keys = ("press_err", "face_u_err", "particle_u_err", "rel_residual", "ms_step", "ms_solve", "iterations")
     summ = {k: {"max": max(r[k] for r in recs), "median": float(np.median([r[k] for r in recs]))} for k in keys}
     summ["steps"] = len(recs)
     summ["seconds"] = round(time.time() - t0, 1)

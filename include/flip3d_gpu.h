@@ -186,6 +186,7 @@ int flip_gpu_set_profiling(flip_ctx *ctx, int on);
 #define FLIP_SCENE_REFERENCE 0
 #define FLIP_SCENE_BENCHMARK 1
 #define FLIP_SCENE_RANDOM 2 /* 1-4 liquid boxes (edges U(0.1,0.6)) / spheres (r U(0.05,0.2)), seeded */
+#define FLIP_SCENE_RANDOM_BOXES 3 /* 1-3 liquid boxes (edges U(0.1,0.6)), seeded (ensemble scenes) */
 int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np);
 
 /* Checkpoints in the reference's render/save/<step>_scene.dat layout
@@ -205,6 +206,14 @@ int flip_gpu_load_state(flip_ctx *ctx, const char *path);
  * FLIP_PRECOND_LEARNED; the network runs on the (n/2)^3 coarse grid, so n
  * must be a multiple of 16.  No counterpart in the reference. */
 int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path);
+
+/* Ensembles of independent scenes.  Steps every context of ctxs once (each
+ * exactly as flip_gpu_step would), with `threads` host threads driving the
+ * contexts concurrently, each context on its own stream; returns when all
+ * are done.  stats and ms (the wall-clock of each context's step) may be
+ * NULL.  On failure returns the first failing context's code and message.
+ * No counterpart in the reference, which simulates one scene per process. */
+int flip_gpu_step_many(flip_ctx **ctxs, int count, int threads, flip_step_stats *stats, double *ms);
 
 /* Device math self-check for the ops whose rounding the parity argument
  * relies on: out[4*i+0] = x/y, +1 = sqrtf(x), +2 = (float)sqrt((double)x*x+(double)y*y),
