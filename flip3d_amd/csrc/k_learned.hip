@@ -611,7 +611,8 @@ static LearnedSolver *learned_get(FlipCtx *c)
         std::string err;
         if (!unet_read_weights(c->learned_weights.c_str(), ls->net->nparam, w, err))
             throw flip_arg_error{FLIP_E_ARG, err};
-        FLIP_CHECK(hipMemcpy(ls->net->pw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpyAsync(ls->net->pw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
         unet_refresh_bf16(ls->net);
         const long C = c->C;
         dmalloc(&ls->dinv0, C);

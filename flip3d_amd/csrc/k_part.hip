@@ -950,7 +950,8 @@ int k_stuck_and_reposition(FlipCtx *c)
     int moved = 0;
     if (m > 0) {
         ids.resize(m);
-        FLIP_CHECK(hipMemcpy(ids.data(), c->ev_list, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+        FLIP_CHECK(hipMemcpyAsync(ids.data(), c->ev_list, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
         std::sort(ids.begin(), ids.end());
         // deep-water cells in scan order (flip3D.cpp:411-430)
         hipLaunchKernelGGL(k_deep, nblk(c->C, TB), TB, 0, c->stream, c->C, n, c->A, c->cell_count);
@@ -963,7 +964,8 @@ int k_stuck_and_reposition(FlipCtx *c)
         if (nd > 0) {
             int take = std::min(nd, m);
             std::vector<int32_t> deep(take), waters(m);
-            FLIP_CHECK(hipMemcpy(deep.data(), c->ev_off, sizeof(int32_t) * take, hipMemcpyDeviceToHost));
+            FLIP_CHECK(hipMemcpyAsync(deep.data(), c->ev_off, sizeof(int32_t) * take, hipMemcpyDeviceToHost, c->stream));
+            FLIP_CHECK(hipStreamSynchronize(c->stream));
             for (int t = 0; t < m; ++t) waters[t] = deep[t % take];
             // libstdc++ random_shuffle: swap(a[i], a[rand() % (i+1)]) for i >= 1
             for (int i = 1; i < m; ++i) {
@@ -1055,12 +1057,13 @@ int k_correct(FlipCtx *c)
         // order the listed particles by creation index, count their events, and
         // draw 3 rand()%101 per event in that order (corrector.cpp:74-76 at one thread)
         std::vector<int32_t> pairs(2 * (size_t)m), slots(m), ev(m);
-        FLIP_CHECK(hipMemcpy(pairs.data(), c->ev_list, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost));
+        FLIP_CHECK(hipMemcpyAsync(pairs.data(), c->ev_list, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost, c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
         std::vector<std::pair<int32_t, int32_t>> byid(m);
         for (int t = 0; t < m; ++t) byid[t] = {pairs[2 * t], pairs[2 * t + 1]};
         std::sort(byid.begin(), byid.end());
         for (int t = 0; t < m; ++t) slots[t] = byid[t].second;
-        FLIP_CHECK(hipMemcpy(c->ev_list, slots.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpyAsync(c->ev_list, slots.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_spring_count, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, n, c->P4(), c->cell_start,
                            re, c->ev_off);
         FLIP_CHECK(hipMemcpyAsync(ev.data(), c->ev_off, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
@@ -1073,10 +1076,11 @@ int k_correct(FlipCtx *c)
         std::vector<float> rv(3 * (size_t)events + 1);
         for (int e = 0; e < 3 * events; ++e) rv[e] = (float)(c->rng.next() % 101);
         if ((int64_t)rv.size() > c->ev_cap * 3) throw flip_arg_error{FLIP_E_STATE, "too many corrector jitter events"};
-        FLIP_CHECK(hipMemcpy(c->ev_rand, rv.data(), sizeof(float) * rv.size(), hipMemcpyHostToDevice));
-        FLIP_CHECK(hipMemcpy(c->ev_off, off.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+        FLIP_CHECK(hipMemcpyAsync(c->ev_rand, rv.data(), sizeof(float) * rv.size(), hipMemcpyHostToDevice, c->stream));
+        FLIP_CHECK(hipMemcpyAsync(c->ev_off, off.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_spring_fix, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_off, n, c->P4(), c->M(),
                            c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
     }
     launch_timed(c, "resample", k_resample, nblk(c->np, TB), TB, c->np, n, c->P4(), c->V4(), c->cell_start, c->tmp0,
                  c->tmp1, re * re);
