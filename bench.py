@@ -29,17 +29,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-5
 METRIC = "FLIP steps/s at 128^3 (~4M particles), learned vs plain PCG, rel resid 1e-5"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_a_n128_traffic.txt")
+PROFILES = os.path.join(ROOT, "profiles")
 DEFAULT_WEIGHTS = os.path.join(ROOT, "flip3d_amd", "weights", "unet_coarse.bin")
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, n=128):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a copy of
-    known size); None when no summary for it exists."""
-    if not os.path.exists(TRAFFIC_FILE):
+    profiles/<kernel>_n<n>_traffic.txt (FETCH_SIZE + WRITE_SIZE, each scaled
+    by the same counter on a copy of known size); None when there is none."""
+    path = os.path.join(PROFILES, f"{kernel}_n{n}_traffic.txt")
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         parts = line.split()
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
             return int(float(parts[2]))
@@ -159,7 +160,7 @@ def roofline_of(agg, n, n_particles, n_fluid):
     b = kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid)
     achieved = b / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, n),
             "bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt}
 
 
