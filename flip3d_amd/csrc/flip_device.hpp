@@ -34,6 +34,15 @@ __device__ __forceinline__ float smooth_kernel(float r2, float h)
     return v > 0.0f ? v : 0.0f;
 }
 
+// smooth_kernel when h*h is a power of two and inv_hh = 1 / (h*h): the
+// quotient r2 / (h*h) is then exactly r2 * inv_hh, so the result is the same.
+__device__ __forceinline__ float smooth_kernel_scaled(float r2, float inv_hh)
+{
+    float q = r2 * inv_hh;
+    float v = 1.0f - q;
+    return v > 0.0f ? v : 0.0f;
+}
+
 // kernel.cpp:8-10 sharp_kernel: (float)fmax((double)(h*h)/fmax((double)r2,1e-5) - 1.0, 0.0).
 // r2 >= h*h gives exactly 0 (quotient <= 1); NaN r2 takes the full path like fmax does.
 __device__ __forceinline__ float sharp_kernel_hh(float r2, float hh)

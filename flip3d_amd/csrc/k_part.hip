@@ -27,9 +27,12 @@ __device__ __forceinline__ float mass_of(float w) { return __builtin_fabsf(w); }
 // ------------------------------------------------------------- density
 // flip3D.cpp:166-190 computeDensity (+ add_ExtForce when gdt != 0 is fused
 // by the step driver through a separate call).
+// POW2: h*h is a power of two, so r2 / (h*h) == r2 * (1 / (h*h)) exactly
+// (scaling by a power of two rounds nothing while the result stays finite).
+template <bool POW2>
 __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, const int32_t *__restrict__ cs, float h,
-                                                float max_dens)
+                                                float inv_hh, float max_dens)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
@@ -48,7 +51,8 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
             for_range4(pos, b, e, [&](const float4 &q, int) {
                 if (is_wall_w(q.w)) return;
-                float w = q.w * smooth_kernel(length2(q.x, q.y, q.z, p.x, p.y, p.z), h);
+                const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
+                float w = q.w * (POW2 ? smooth_kernel_scaled(r2, inv_hh) : smooth_kernel(r2, h));
                 wsum += w;
             });
         }
@@ -141,68 +145,37 @@ __global__ void k_p2g_prep_unit(int64_t np, int n, const float4 *__restrict__ po
     pz[i] = make_float4(x, y, z, v.z);
 }
 
-// Faces of one component in 16x16 tiles of (a, k), one layer of the third
-// axis per block row (a = j, layer i for X and Z faces; a = i, layer j for Y
-// faces).  Each wave takes an 8x8 (a, k) patch: its windows' particle ranges
-// run along k, so neighbouring lanes read neighbouring memory.
+// mapper.cpp:17-88 window sum of one face with unit masses and packed
+// (x, y, z, u_c) input read from `src` (global memory, or a block's LDS copy
+// through loff).  The window is the one described at k_p2g.
 template <int COMP>
-__host__ __device__ __forceinline__ void p2g_dims(int n, int &layers, int &na, int &nk)
+__device__ __forceinline__ void p2g_face_coord(int i, int j, int k, float &fx, float &fy, float &fz)
 {
-    layers = COMP == 1 ? n + 1 : (COMP == 0 ? n + 1 : n);
-    na = n;
-    nk = COMP == 2 ? n + 1 : n;
+    fx = COMP == 0 ? (float)i : (float)(i + 0.5);
+    fy = COMP == 1 ? (float)j : (float)(j + 0.5);
+    fz = COMP == 2 ? (float)k : (float)(k + 0.5);
 }
 
-template <int COMP>
-__device__ __forceinline__ bool p2g_face_of(int n, int &i, int &j, int &k)
+__device__ __forceinline__ void p2g_accum(const float4 *__restrict__ src, int b, int e, float fx, float fy, float fz,
+                                          float hh, float &sumw, float &sumx)
 {
-    int layers, na, nk;
-    p2g_dims<COMP>(n, layers, na, nk);
-    int tiles_k = (nk + 15) / 16;
-    int bx = blockIdx.x % tiles_k, by = blockIdx.x / tiles_k;
-    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    int ta = by * 16 + (w >> 1) * 8 + (l >> 3);
-    int tk = bx * 16 + (w & 1) * 8 + (l & 7);
-    int layer = blockIdx.y;
-    if (ta >= na || tk >= nk || layer >= layers) return false;
-    if (COMP == 1) { j = layer; i = ta; }
-    else { i = layer; j = ta; }
-    k = tk;
-    return true;
+    for_range4(src, b, e, [&](const float4 &q, int) {
+        float r2 = length2(q.x, q.y, q.z, fx, fy, fz);
+        if (!(r2 < hh)) return;
+        float w = sharp_kernel_hh(r2, hh);
+        sumx += w * q.w;
+        sumw += w;
+    });
 }
 
-// mapper.cpp:17-88 with unit masses and packed input (see k_p2g for the window)
+// u_save = raw value (save_grid, flip3D.cpp:971) and u = value with
+// enforce_boundary (flip3D.cpp:783-805)
 template <int COMP>
-__global__ __launch_bounds__(TB) void k_p2g_unit(int n, const float4 *__restrict__ pk,
-                                                 const int32_t *__restrict__ cs, const int8_t *__restrict__ A,
-                                                 float *__restrict__ ug, float *__restrict__ us, float hh)
+__device__ __forceinline__ void p2g_store(int n, int i, int j, int k, float sumw, float sumx,
+                                          const int8_t *__restrict__ A, float *__restrict__ ug, float *__restrict__ us)
 {
-    int i, j, k;
-    if (!p2g_face_of<COMP>(n, i, j, k)) return;
     const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
     const long f = ((long)i * nj + j) * nk + k;
-    float fx = COMP == 0 ? (float)i : (float)(i + 0.5);
-    float fy = COMP == 1 ? (float)j : (float)(j + 0.5);
-    float fz = COMP == 2 ? (float)k : (float)(k + 0.5);
-    const int ihi = COMP == 0 ? 0 : 1, jhi = COMP == 1 ? 0 : 1, khi_off = COMP == 2 ? 0 : 1;
-    int klo = max(k - 1, 0), khi = min(k + khi_off, n - 1);
-    float sumw = 0.0f, sumx = 0.0f;
-    if (klo <= khi) {
-        for (int si = i - 1; si <= i + ihi; ++si) {
-            if (si < 0 || si > n - 1) continue;
-            for (int sj = j - 1; sj <= j + jhi; ++sj) {
-                if (sj < 0 || sj > n - 1) continue;
-                int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-                for_range4(pk, b, e, [&](const float4 &q, int) {
-                    float r2 = length2(q.x, q.y, q.z, fx, fy, fz);
-                    if (!(r2 < hh)) return;
-                    float w = sharp_kernel_hh(r2, hh);
-                    sumx += w * q.w;
-                    sumw += w;
-                });
-            }
-        }
-    }
     float val = sumw != 0.0f ? sumx / sumw : 0.0f;
     us[f] = val;
     int a0 = COMP == 0 ? i : (COMP == 1 ? j : k);
@@ -214,6 +187,130 @@ __global__ __launch_bounds__(TB) void k_p2g_unit(int n, const float4 *__restrict
         zero = wall_chk(A[c1]) * wall_chk(A[c0]) < 0;
     }
     ug[f] = zero ? 0.0f : val;
+}
+
+template <int COMP>
+__device__ __forceinline__ void p2g_face_global(int n, int i, int j, int k, const float4 *__restrict__ pk,
+                                                const int32_t *__restrict__ cs, float hh, float &sumw, float &sumx)
+{
+    float fx, fy, fz;
+    p2g_face_coord<COMP>(i, j, k, fx, fy, fz);
+    const int ihi = COMP == 0 ? 0 : 1, jhi = COMP == 1 ? 0 : 1, khi_off = COMP == 2 ? 0 : 1;
+    int klo = max(k - 1, 0), khi = min(k + khi_off, n - 1);
+    sumw = 0.0f;
+    sumx = 0.0f;
+    if (klo > khi) return;
+    for (int si = i - 1; si <= i + ihi; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = j - 1; sj <= j + jhi; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            p2g_accum(pk, b, e, fx, fy, fz, hh, sumw, sumx);
+        }
+    }
+}
+
+// mapper.cpp:17-88 with unit masses and packed input, over bricks of
+// P2G_BI x P2G_BJ x P2G_BK faces (i, j, k; k fastest, one wave per i layer).  The block first copies the particles of
+// the brick's whole cell window into LDS: one contiguous range per (si, sj)
+// row, rows back to back in row order, with loff[row][c] the LDS offset of
+// the row's c-th cell from sklo.  Every face then walks its own window out
+// of LDS in the reference's order, so the sums are those of p2g_face_global
+// bit for bit.  A brick whose window holds more than P2G_CAP particles reads
+// global memory instead.
+typedef __attribute__((address_space(3))) void *lds_ptr;
+constexpr int P2G_BI = 4, P2G_BJ = 8, P2G_BK = 8;
+constexpr int P2G_TB = P2G_BI * P2G_BJ * P2G_BK;
+constexpr int P2G_ROWS = (P2G_BI + 2) * (P2G_BJ + 2);
+constexpr int P2G_RK = P2G_BK + 2;
+constexpr int P2G_CAP = 4608;  // 72 KiB of float4: two blocks per CU
+
+template <int COMP>
+__host__ __device__ __forceinline__ void p2g_bricks(int n, int &nbi, int &nbj, int &nbk)
+{
+    nbi = (n + (COMP == 0) + P2G_BI - 1) / P2G_BI;
+    nbj = (n + (COMP == 1) + P2G_BJ - 1) / P2G_BJ;
+    nbk = (n + (COMP == 2) + P2G_BK - 1) / P2G_BK;
+}
+
+template <int COMP>
+__global__ __launch_bounds__(P2G_TB) void k_p2g_lds(int n, const float4 *__restrict__ pk,
+                                                    const int32_t *__restrict__ cs, const int8_t *__restrict__ A,
+                                                    float *__restrict__ ug, float *__restrict__ us, float hh)
+{
+    __shared__ float4 sp[P2G_CAP];
+    __shared__ int loff[P2G_ROWS][P2G_RK + 1];
+    __shared__ int rbase[P2G_ROWS];
+    __shared__ int total;
+    const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
+    const int ihi = COMP == 0 ? 0 : 1, jhi = COMP == 1 ? 0 : 1, khi_off = COMP == 2 ? 0 : 1;
+    int nbi, nbj, nbk;
+    p2g_bricks<COMP>(n, nbi, nbj, nbk);
+    const int bk = blockIdx.x % nbk, bj = (blockIdx.x / nbk) % nbj, bi = blockIdx.x / (nbk * nbj);
+    const int i0 = bi * P2G_BI, j0 = bj * P2G_BJ, k0 = bk * P2G_BK;
+    // rows (si, sj) = (i0 - 1 + r / RJ, j0 - 1 + r % RJ); cells sklo..skhi
+    const int RI = P2G_BI + 1 + ihi, RJ = P2G_BJ + 1 + jhi;
+    const int sklo = max(k0 - 1, 0), skhi = min(k0 + P2G_BK - 1 + khi_off, n - 1);
+    const int nkc = skhi - sklo + 1;  // 1..P2G_RK
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int idx = tid; idx < RI * RJ * (nkc + 1); idx += P2G_TB) {
+        const int r = idx / (nkc + 1), c = idx - r * (nkc + 1);
+        const int si = i0 - 1 + r / RJ, sj = j0 - 1 + r % RJ;
+        const bool ok = si >= 0 && si <= n - 1 && sj >= 0 && sj <= n - 1;
+        loff[r][c] = ok ? cs[cidx(n, si, sj, sklo) + c] : 0;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive scan of the row counts (RI * RJ <= 60 rows, one per lane)
+        int cnt = lane < RI * RJ ? loff[lane][nkc] - loff[lane][0] : 0;
+        int inc = cnt;
+        for (int d = 1; d < 64; d <<= 1) {
+            int v = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += v;
+        }
+        if (lane < RI * RJ) rbase[lane] = inc - cnt;
+        if (lane == 63) total = inc;
+    }
+    __syncthreads();
+    const bool staged = total <= P2G_CAP;
+    if (staged) {
+        // Direct global -> LDS copies (global_load_lds_dwordx4: lane l of the
+        // wave writes the 16 bytes at the wave-uniform LDS address + 16 l), so
+        // a wave issues all of its rows' loads without waiting on any.
+        for (int r = wave; r < RI * RJ; r += P2G_TB / 64) {
+            const int g0 = loff[r][0], cnt = loff[r][nkc] - g0, base = rbase[r];
+            for (int e0 = 0; e0 < cnt; e0 += 64)
+                if (e0 + lane < cnt)
+                    __builtin_amdgcn_global_load_lds(pk + g0 + e0 + lane, (lds_ptr)(sp + base + e0), 16, 0, 0);
+            // every lane has read loff[r][*] above; rebase the row to LDS offsets
+            if (lane <= nkc) loff[r][lane] = loff[r][lane] - g0 + base;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    __syncthreads();
+    const int i = i0 + (tid >> 6), j = j0 + ((tid >> 3) & 7), k = k0 + (tid & 7);
+    if (i >= ni || j >= nj || k >= nk) return;
+    float sumw, sumx;
+    if (!staged) {
+        p2g_face_global<COMP>(n, i, j, k, pk, cs, hh, sumw, sumx);
+    } else {
+        float fx, fy, fz;
+        p2g_face_coord<COMP>(i, j, k, fx, fy, fz);
+        const int klo = max(k - 1, 0), khi = min(k + khi_off, n - 1);
+        sumw = 0.0f;
+        sumx = 0.0f;
+        if (klo <= khi) {
+            for (int si = i - 1; si <= i + ihi; ++si) {
+                if (si < 0 || si > n - 1) continue;
+                for (int sj = j - 1; sj <= j + jhi; ++sj) {
+                    if (sj < 0 || sj > n - 1) continue;
+                    const int r = (si - (i0 - 1)) * RJ + (sj - (j0 - 1));
+                    p2g_accum(sp, loff[r][klo - sklo], loff[r][khi - sklo + 1], fx, fy, fz, hh, sumw, sumx);
+                }
+            }
+        }
+    }
+    p2g_store<COMP>(n, i, j, k, sumw, sumx, A, ug, us);
 }
 
 // mapper.cpp:17-88 one face component per launch.  COMP selects the face
@@ -535,6 +632,120 @@ __global__ __launch_bounds__(1024) void k_resample_levels(int nlev, const int32_
 }
 
 // ------------------------------------------------------------- corrector
+// Conservative culling of a 3x3x3 neighbour window (the corrector's radius is
+// half a cell, so most of the window is out of reach).  For a particle at
+// g*p = (gx, gy, gz) with window centre cell (ci, cj, ck), the squared
+// distance in cell units from g*p to the lower and upper neighbour cells
+// along x is at least max(gx - ci - 1e-3, 0)^2 and max(ci + 1 - gx - 1e-3,
+// 0)^2; the margin covers the float rounding of every particle's cell
+// assignment.  A centre past the last cell (the corrector's clamp to g
+// allows ci == n) culls nothing below it, since the last cell reaches to
+// infinity.  A row or cell whose bound reaches r2 g^2 1.001 (r2: squared
+// radius in world units) holds only particles beyond the radius, whose
+// contributions are exactly zero and skipped by the callers anyway, so the
+// particles left are visited in the same order with the same sums.
+struct WinCull {
+    float xl, xh, yl, yh, zl, zh, thr;
+    __device__ __forceinline__ static float sq0(float v) { return v > 0.0f ? v * v : 0.0f; }
+    __device__ __forceinline__ WinCull(int n, float gx, float gy, float gz, int ci, int cj, int ck, float r2)
+    {
+        const float m = 1e-3f, g = (float)n;
+        xl = ci < n ? sq0(gx - (float)ci - m) : 0.0f;
+        yl = cj < n ? sq0(gy - (float)cj - m) : 0.0f;
+        zl = ck < n ? sq0(gz - (float)ck - m) : 0.0f;
+        xh = sq0((float)(ci + 1) - gx - m);
+        yh = sq0((float)(cj + 1) - gy - m);
+        zh = sq0((float)(ck + 1) - gz - m);
+        thr = r2 * g * g * 1.001f;
+    }
+    // bound of row (si, sj); false when the whole row is out of reach
+    __device__ __forceinline__ bool row(int ci, int cj, int si, int sj, float &dij) const
+    {
+        float dx = si < ci ? xl : (si > ci ? xh : 0.0f);
+        float dy = sj < cj ? yl : (sj > cj ? yh : 0.0f);
+        dij = dx + dy;
+        return dij < thr;
+    }
+    // cells [klo, khi] of a row within reach (klo > khi: none)
+    __device__ __forceinline__ void cells(int n, int ck, float dij, int &klo, int &khi) const
+    {
+        klo = (ck >= 1 && dij + zl < thr) ? ck - 1 : ck;
+        khi = (ck + 1 <= n - 1 && dij + zh < thr) ? ck + 1 : min(ck, n - 1);
+    }
+};
+
+// Block-local regrouping of a block's B particles by the octant of the cell
+// that centres their window.  With the corrector's half-cell radius the
+// octant (which half of the cell along each axis) fixes the neighbour rows
+// and cells within reach, so a wave whose lanes share an octant walks only
+// those.  Group 8 collects idle threads.  Stable within a group (index
+// order); built from wave ballots and one wave scan.  Returns the
+// block-relative particle index this thread handles.  The mapping changes
+// only which thread computes a particle, never what it computes.
+template <int B>
+__device__ __forceinline__ int octant_regroup(int grp, int32_t *s_off, int32_t *s_idx)
+{
+    constexpr int W = B / 64, NG = 9, E = (NG * W + 63) / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int rank = 0;
+    for (int o = 0; o < NG; ++o) {
+        const unsigned long long m = __ballot(grp == o);
+        if (grp == o) rank = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_off[o * W + wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive scan of the (group, wave) counts in that order
+        int v[E], sum = 0;
+        for (int e = 0; e < E; ++e) {
+            const int j = lane * E + e;
+            v[e] = j < NG * W ? s_off[j] : 0;
+            sum += v[e];
+        }
+        int inc = sum;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += u;
+        }
+        int acc = inc - sum;
+        for (int e = 0; e < E; ++e) {
+            const int j = lane * E + e;
+            if (j < NG * W) s_off[j] = acc;
+            acc += v[e];
+        }
+    }
+    __syncthreads();
+    s_idx[s_off[grp * W + wave] + rank] = tid;
+    __syncthreads();
+    return s_idx[tid];
+}
+
+// octant of g*p relative to its cell c (per axis: upper half or not)
+__device__ __forceinline__ int octant_of(float gx, float gy, float gz, int ci, int cj, int ck)
+{
+    return ((gx - (float)ci >= 0.5f) << 2) | ((gy - (float)cj >= 0.5f) << 1) | (gz - (float)ck >= 0.5f);
+}
+
+// window centre of the spring pass: max(0, min(g, g*p)) per axis with the
+// reference's macros (utility.h:16-17), truncated
+__device__ __forceinline__ void spring_centre(int n, const float4 &p, float &cx, float &cy, float &cz, int &ci,
+                                              int &cj, int &ck)
+{
+    const float g = (float)n;
+    cx = g * p.x;
+    cy = g * p.y;
+    cz = g * p.z;
+    float x = g > cx ? cx : g;
+    x = 0.0f > x ? 0.0f : x;
+    float y = g > cy ? cy : g;
+    y = 0.0f > y ? 0.0f : y;
+    float z = g > cz ? cz : g;
+    z = 0.0f > z ? 0.0f : z;
+    ci = (int)x;
+    cj = (int)y;
+    ck = (int)z;
+}
+
 // corrector.cpp:55-88 spring for one particle.  RAND == false: stops at the
 // first random-jitter event and reports it; RAND == true: consumes the
 // host-drawn rand()%101 values in window order.
@@ -545,17 +756,9 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
                                           float4 *tmp0)
 {
     float4 p = pos[i];
-    float g = (float)n;
-    float cx = g * p.x, cy = g * p.y, cz = g * p.z;
-    // max(0, min(cs, cs*p)) with the reference's macros (utility.h:16-17)
-    float x = g > cx ? cx : g;
-    x = 0.0f > x ? 0.0f : x;
-    float y = g > cy ? cy : g;
-    y = 0.0f > y ? 0.0f : y;
-    float z = g > cz ? cz : g;
-    z = 0.0f > z ? 0.0f : z;
-    int ci = (int)x, cj = (int)y, ck = (int)z;
-    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    float cx, cy, cz;
+    int ci, cj, ck;
+    spring_centre(n, p, cx, cy, cz, ci, cj, ck);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     int events = 0;
     const double rre = 0.1 * (double)re;
@@ -566,10 +769,15 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
     // squared distance is within a few ulp of dist*dist, so the 1e-4 margin
     // keeps the test conservative.
     const float far2 = re * re * 1.0001f;
+    const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
             if (sj < 0 || sj > n - 1) continue;
+            float dij;
+            int klo, khi;
+            if (!wc.row(ci, cj, si, sj, dij)) continue;
+            wc.cells(n, ck, dij, klo, khi);
             if (klo > khi) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
             bool bail = false;
@@ -613,7 +821,7 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
 // than SPRING_NEAR survivors (never seen in practice) falls back to
 // spring_one itself.
 constexpr int SPRING_TB = 256;
-constexpr int SPRING_NEAR = 32;
+constexpr int SPRING_NEAR = 16;
 
 __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const float4 *__restrict__ pos,
                                                       const int32_t *__restrict__ meta,
@@ -622,27 +830,40 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
                                                       int32_t *list, int32_t *count)
 {
     __shared__ int32_t near[SPRING_NEAR][SPRING_TB];
+    __shared__ int32_t s_off[9 * SPRING_TB / 64], s_idx[SPRING_TB];
+    const int64_t base = (int64_t)blockIdx.x * SPRING_TB;
+    float cx, cy, cz;
+    int ci, cj, ck;
+    {
+        const int64_t i = base + threadIdx.x;
+        int grp = 8;
+        if (i < np) {
+            const float4 p = pos[i];
+            if (!is_wall_w(p.w)) {
+                spring_centre(n, p, cx, cy, cz, ci, cj, ck);
+                grp = octant_of(cx, cy, cz, ci, cj, ck);
+            }
+        }
+        const int t = octant_regroup<SPRING_TB>(grp, s_off, s_idx);
+        // the thread that handles particle base + t
+        if (base + t >= np) return;
+    }
     const int tid = threadIdx.x;
-    int64_t i = (int64_t)blockIdx.x * SPRING_TB + tid;
-    if (i >= np) return;
+    const int64_t i = base + s_idx[tid];
     const float4 p = pos[i];
     if (is_wall_w(p.w)) return;
-    float g = (float)n;
-    float cx = g * p.x, cy = g * p.y, cz = g * p.z;
-    float x = g > cx ? cx : g;
-    x = 0.0f > x ? 0.0f : x;
-    float y = g > cy ? cy : g;
-    y = 0.0f > y ? 0.0f : y;
-    float z = g > cz ? cz : g;
-    z = 0.0f > z ? 0.0f : z;
-    int ci = (int)x, cj = (int)y, ck = (int)z;
-    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    spring_centre(n, p, cx, cy, cz, ci, cj, ck);
     const float far2 = re * re * 1.0001f;  // as in spring_one
+    const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
     int cnt = 0;
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
             if (sj < 0 || sj > n - 1) continue;
+            float dij;
+            int klo, khi;
+            if (!wc.row(ci, cj, si, sj, dij)) continue;
+            wc.cells(n, ck, dij, klo, khi);
             if (klo > khi) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
             for_range4(pos, b, e, [&](const float4 &q, int a) {
@@ -696,23 +917,22 @@ __global__ void k_spring_count(int m, const int32_t *__restrict__ list, int n, c
     if (t >= m) return;
     int64_t i = list[t];
     float4 p = pos[i];
-    float g = (float)n;
-    float cx = g * p.x, cy = g * p.y, cz = g * p.z;
-    float x = g > cx ? cx : g;
-    x = 0.0f > x ? 0.0f : x;
-    float y = g > cy ? cy : g;
-    y = 0.0f > y ? 0.0f : y;
-    float z = g > cz ? cz : g;
-    z = 0.0f > z ? 0.0f : z;
-    int ci = (int)x, cj = (int)y, ck = (int)z;
-    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    float cx, cy, cz;
+    int ci, cj, ck;
+    spring_centre(n, p, cx, cy, cz, ci, cj, ck);
     const double rre = 0.1 * (double)re;
     const float far2 = re * re * 1.0001f;  // as in spring_one
+    const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
     int ev = 0;
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
-            if (sj < 0 || sj > n - 1 || klo > khi) continue;
+            if (sj < 0 || sj > n - 1) continue;
+            float dij;
+            int klo, khi;
+            if (!wc.row(ci, cj, si, sj, dij)) continue;
+            wc.cells(n, ck, dij, klo, khi);
+            if (klo > khi) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
             for (int a = b; a < e; ++a) {
                 if (a == i) continue;
@@ -738,22 +958,42 @@ __global__ void k_spring_fix(int m, const int32_t *__restrict__ list, const int3
 
 // corrector.cpp:15-47 resample at the pseudo-moved position with the old
 // positions and velocities (re = DENSITY/N); keeps u when no weight.
-__global__ __launch_bounds__(TB) void k_resample(int64_t np, int n, const float4 *__restrict__ pos,
-                                                 const float4 *__restrict__ vel, const int32_t *__restrict__ cs,
-                                                 const float4 *__restrict__ tmp0, float4 *tmp1, float hh)
+constexpr int RESAMPLE_TB = 256;
+
+__global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, const float4 *__restrict__ pos,
+                                                          const float4 *__restrict__ vel,
+                                                          const int32_t *__restrict__ cs,
+                                                          const float4 *__restrict__ tmp0, float4 *tmp1, float hh)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
+    __shared__ int32_t s_off[9 * RESAMPLE_TB / 64], s_idx[RESAMPLE_TB];
+    const int64_t base = (int64_t)blockIdx.x * RESAMPLE_TB;
+    const float g = (float)n;
+    {
+        const int64_t i = base + threadIdx.x;
+        int grp = 8;
+        if (i < np && !is_wall_w(pos[i].w)) {
+            const float4 p = tmp0[i];
+            grp = octant_of(g * p.x, g * p.y, g * p.z, cell_of(n, p.x), cell_of(n, p.y), cell_of(n, p.z));
+        }
+        const int t = octant_regroup<RESAMPLE_TB>(grp, s_off, s_idx);
+        if (base + t >= np) return;
+    }
+    const int64_t i = base + s_idx[threadIdx.x];
     if (is_wall_w(pos[i].w)) return;
     float4 p = tmp0[i];
     float4 v0 = vel[i];
     int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
-    int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    const WinCull wc(n, g * p.x, g * p.y, g * p.z, ci, cj, ck, hh);
     float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
             if (sj < 0 || sj > n - 1) continue;
+            float dij;
+            int klo, khi;
+            if (!wc.row(ci, cj, si, sj, dij)) continue;
+            wc.cells(n, ck, dij, klo, khi);
+            if (klo > khi) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
             for_range4(pos, b, e, [&](const float4 &q, int a) {
                 if (is_wall_w(q.w)) return;
@@ -824,9 +1064,17 @@ __global__ void k_selftest(int64_t m, const float *x, const float *y, float *out
 // ------------------------------------------------------------- launchers
 void k_launch_density(FlipCtx *c)
 {
-    if (c->np > 0)
-        launch_timed(c, "density", k_density, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->cell_start,
-                     (float)(4.0 * c->prm.density / c->n), c->max_dens);
+    if (c->np == 0) return;
+    const float h = (float)(4.0 * c->prm.density / c->n), hh = h * h;
+    int e = 0;
+    const bool pow2 = std::isnormal(hh) && std::frexp(hh, &e) == 0.5f;
+    const float inv_hh = pow2 ? std::ldexp(1.0f, 1 - e) : 0.0f;  // hh = 2^(e-1)
+    if (pow2)
+        launch_timed(c, "density", k_density<true>, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(),
+                     c->cell_start, h, inv_hh, c->max_dens);
+    else
+        launch_timed(c, "density", k_density<false>, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(),
+                     c->cell_start, h, inv_hh, c->max_dens);
 }
 
 void k_launch_ext_force(FlipCtx *c)
@@ -857,19 +1105,16 @@ void k_launch_p2g(FlipCtx *c)
                                c->spos, c->tmp0, c->tmp1);
         t_end(c);
         t_begin(c, "p2g");
-        int layers, na, nk;
-        p2g_dims<0>(c->n, layers, na, nk);
-        dim3 g0(((na + 15) / 16) * ((nk + 15) / 16), layers);
-        hipLaunchKernelGGL(k_p2g_unit<0>, g0, TB, 0, c->stream, c->n, c->spos, c->cell_start, c->A, c->ug[0],
-                           c->us[0], hh);
-        p2g_dims<1>(c->n, layers, na, nk);
-        dim3 g1(((na + 15) / 16) * ((nk + 15) / 16), layers);
-        hipLaunchKernelGGL(k_p2g_unit<1>, g1, TB, 0, c->stream, c->n, c->tmp0, c->cell_start, c->A, c->ug[1],
-                           c->us[1], hh);
-        p2g_dims<2>(c->n, layers, na, nk);
-        dim3 g2(((na + 15) / 16) * ((nk + 15) / 16), layers);
-        hipLaunchKernelGGL(k_p2g_unit<2>, g2, TB, 0, c->stream, c->n, c->tmp1, c->cell_start, c->A, c->ug[2],
-                           c->us[2], hh);
+        int bi, bj, bk;
+        p2g_bricks<0>(c->n, bi, bj, bk);
+        hipLaunchKernelGGL(k_p2g_lds<0>, bi * bj * bk, P2G_TB, 0, c->stream, c->n, c->spos, c->cell_start, c->A,
+                           c->ug[0], c->us[0], hh);
+        p2g_bricks<1>(c->n, bi, bj, bk);
+        hipLaunchKernelGGL(k_p2g_lds<1>, bi * bj * bk, P2G_TB, 0, c->stream, c->n, c->tmp0, c->cell_start, c->A,
+                           c->ug[1], c->us[1], hh);
+        p2g_bricks<2>(c->n, bi, bj, bk);
+        hipLaunchKernelGGL(k_p2g_lds<2>, bi * bj * bk, P2G_TB, 0, c->stream, c->n, c->tmp1, c->cell_start, c->A,
+                           c->ug[2], c->us[2], hh);
         t_end(c);
         return;
     }
@@ -1082,7 +1327,7 @@ int k_correct(FlipCtx *c)
                            c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
         FLIP_CHECK(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
     }
-    launch_timed(c, "resample", k_resample, nblk(c->np, TB), TB, c->np, n, c->P4(), c->V4(), c->cell_start, c->tmp0,
+    launch_timed(c, "resample", k_resample, nblk(c->np, RESAMPLE_TB), RESAMPLE_TB, c->np, n, c->P4(), c->V4(), c->cell_start, c->tmp0,
                  c->tmp1, re * re);
     launch_timed(c, "correct_update", k_correct_update, nblk(c->np, TB), TB, c->np, c->P4(), c->V4(), c->tmp0,
                  c->tmp1);
