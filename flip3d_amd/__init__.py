@@ -19,6 +19,7 @@ PRECOND_REFERENCE, PRECOND_IC0, PRECOND_LEARNED = 0, 1, 2
 (STAGE_DENSITY, STAGE_EXT_FORCE, STAGE_P2G, STAGE_PROJECT, STAGE_EXTRAP, STAGE_G2P, STAGE_ADVECT,
  STAGE_CORRECT) = range(8)
 SCENE_REFERENCE, SCENE_BENCHMARK, SCENE_RANDOM, SCENE_RANDOM_BOXES = 0, 1, 2, 3
+SCENE_SPHERE_POUR, SCENE_CLIFF_POUR, SCENE_WATER_DROP = 4, 5, 6
 BENCH_SEED = 20261203
 
 EXPORTS = [
@@ -29,6 +30,7 @@ EXPORTS = [
     "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
     "flip_gpu_save_state", "flip_gpu_load_state", "flip_gpu_step_many",
+    "flip_gpu_set_emitter", "flip_gpu_set_step", "flip_gpu_get_step",
     "flip_gpu_set_learned_weights", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
     "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
     "flip_unet_predict", "flip_unet_save", "flip_unet_load",
@@ -82,6 +84,9 @@ def lib():
         L.flip_gpu_upload_particles.argtypes = [P, C.c_int64, P, P, P, P, P, P]
         L.flip_gpu_download_particles.argtypes = [P, P, P, P, P, P]
         L.flip_gpu_num_particles.argtypes = [P, C.POINTER(C.c_int64)]
+        L.flip_gpu_set_emitter.argtypes = [P, C.c_float, C.c_float, C.c_float, C.c_int]
+        L.flip_gpu_set_step.argtypes = [P, C.c_int]
+        L.flip_gpu_get_step.argtypes = [P, C.POINTER(C.c_int)]
         L.flip_gpu_set_max_dens.argtypes = [P, C.c_float]
         L.flip_gpu_get_max_dens.argtypes = [P, C.POINTER(C.c_float)]
         L.flip_gpu_srand.argtypes = [P, C.c_uint]
@@ -211,6 +216,20 @@ class FlipContext:
 
     def srand(self, seed):
         _check(lib().flip_gpu_srand(self.h, seed))
+
+    def set_emitter(self, x, z, r, pour_time):
+        """pourWater disc (flip3D.cpp:375-402); pour_time < 0 switches it off."""
+        _check(lib().flip_gpu_set_emitter(self.h, x, z, r, pour_time))
+
+    @property
+    def step_count(self):
+        v = C.c_int()
+        _check(lib().flip_gpu_get_step(self.h, C.byref(v)))
+        return v.value
+
+    @step_count.setter
+    def step_count(self, v):
+        _check(lib().flip_gpu_set_step(self.h, v))
 
     def set_dt(self, dt):
         _check(lib().flip_gpu_set_dt(self.h, dt))

@@ -79,8 +79,6 @@ void free_particles(FlipCtx *c)
     dfree(c->ev_rand);
     dfree(c->partials);
     dfree(c->scan_blocks);
-    dfree(c->wall_pos);
-    dfree(c->wall_nrm);
     c->cap = 0;
 }
 
@@ -111,6 +109,66 @@ void alloc_particles(FlipCtx *c, int64_t cap)
     dalloc(&c->scan_blocks, big / 2048 + 8);
     c->cap = cap;
 }
+
+}  // namespace
+
+// Room for `need` particles.  Growing reallocates every per-particle buffer;
+// the persistent state (current positions, velocities, ids and the normals
+// by creation index) is carried over through the host.  The next sort
+// rebuilds everything else.
+void ensure_particle_capacity(FlipCtx *c, int64_t need)
+{
+    if (need <= c->cap) return;
+    require(need < (int64_t(1) << 29), "too many particles (limit 2^29)");
+    const int64_t np = c->np;
+    std::vector<float4> p4(np), v4(np), n4(np);
+    std::vector<int32_t> mt(np);
+    hipStream_t st = c->stream;
+    if (np > 0) {
+        FLIP_CHECK(hipMemcpyAsync(p4.data(), c->P4(), np * sizeof(float4), hipMemcpyDeviceToHost, st));
+        FLIP_CHECK(hipMemcpyAsync(v4.data(), c->V4(), np * sizeof(float4), hipMemcpyDeviceToHost, st));
+        FLIP_CHECK(hipMemcpyAsync(mt.data(), c->M(), np * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FLIP_CHECK(hipMemcpyAsync(n4.data(), c->nrm_id, np * sizeof(float4), hipMemcpyDeviceToHost, st));
+    }
+    FLIP_CHECK(hipStreamSynchronize(st));
+    alloc_particles(c, std::max<int64_t>(need, 2 * c->cap));
+    c->cur = 0;
+    if (np > 0) {
+        FLIP_CHECK(hipMemcpyAsync(c->pos[0], p4.data(), np * sizeof(float4), hipMemcpyHostToDevice, st));
+        FLIP_CHECK(hipMemcpyAsync(c->vel[0], v4.data(), np * sizeof(float4), hipMemcpyHostToDevice, st));
+        FLIP_CHECK(hipMemcpyAsync(c->meta[0], mt.data(), np * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        FLIP_CHECK(hipMemcpyAsync(c->nrm_id, n4.data(), np * sizeof(float4), hipMemcpyHostToDevice, st));
+    }
+    FLIP_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
+}
+
+// pourWater's disc (flip3D.cpp:375-402): lattice points at spacing DENSITY/N
+// in x and z, the float coordinate stepped by the double spacing, kept when
+// hypotf(x - px, z - pz) < r; y just under the ceiling wall.
+void set_emitter(FlipCtx *c, float x, float z, float r, int pour_time)
+{
+    c->pour_pos[0] = x;
+    c->pour_pos[1] = z;
+    c->pour_rad = r;
+    c->pour_time = pour_time;
+    const int N = c->n;
+    const double w = c->prm.density / N;
+    const float y = (float)(1.0 - 1.0 / N - 2.5 * c->prm.density / N);
+    std::vector<float4> disc;
+    if (pour_time >= 0)
+        for (float px = (float)(w + w / 2.0); px < 1.0 - w / 2.0; px = (float)((double)px + w))
+            for (float pz = (float)(w + w / 2.0); pz < 1.0 - w / 2.0; pz = (float)((double)pz + w))
+                if (hypotf(px - x, pz - z) < r) disc.push_back(make_float4(px, y, pz, 1.0f));
+    dalloc(&c->emit_pos, std::max<size_t>(disc.size(), 1));
+    if (!disc.empty()) {
+        FLIP_CHECK(hipMemcpyAsync(c->emit_pos, disc.data(), disc.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                  c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    c->emit_n = (int)disc.size();
+}
+
+namespace {
 
 void alloc_grids(FlipCtx *c)
 {
@@ -156,6 +214,9 @@ void alloc_grids(FlipCtx *c)
 void free_all(FlipCtx *c)
 {
     free_particles(c);
+    dfree(c->wall_pos);
+    dfree(c->wall_nrm);
+    dfree(c->emit_pos);
     for (int k = 0; k < 3; ++k) {
         dfree(c->ug[k]);
         dfree(c->us[k]);
@@ -570,6 +631,33 @@ int flip_gpu_download_particles(flip_ctx *ctx, float *pos, float *vel, float *nr
     });
 }
 
+int flip_gpu_set_emitter(flip_ctx *ctx, float x, float z, float r, int pour_time)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        require(std::isfinite(x) && std::isfinite(z) && std::isfinite(r) && r >= 0.0f, "bad emitter disc");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        set_emitter(c, x, z, r, pour_time);
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_set_step(flip_ctx *ctx, int step)
+{
+    if (!ctx) return fail(FLIP_E_ARG, "null context");
+    if (step < 0) return fail(FLIP_E_ARG, "negative step");
+    reinterpret_cast<FlipCtx *>(ctx)->step = step;
+    return FLIP_OK;
+}
+
+int flip_gpu_get_step(flip_ctx *ctx, int *step)
+{
+    if (!ctx || !step) return fail(FLIP_E_ARG, "null argument");
+    *step = reinterpret_cast<FlipCtx *>(ctx)->step;
+    return FLIP_OK;
+}
+
 int flip_gpu_num_particles(flip_ctx *ctx, int64_t *np)
 {
     if (!ctx || !np) return fail(FLIP_E_ARG, "null argument");
@@ -684,7 +772,11 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
             FLIP_CHECK(hipEventCreate(&c->ev_a));
             FLIP_CHECK(hipEventCreate(&c->ev_b));
         }
+        // flip3D.cpp:1196-1199: the step counter, then pourWater
+        c->step++;
+        if (c->step <= c->pour_time && c->emit_n > 0) ensure_particle_capacity(c, c->np + c->emit_n);
         FLIP_CHECK(hipEventRecord(c->ev_a, c->stream));
+        if (c->step <= c->pour_time) k_launch_emit(c);
         choose_dt(c);
         st->dt = c->dt;
         // flip3D.cpp:1221-1222 sort + computeDensity

@@ -142,8 +142,8 @@ extern "C" int flip_gpu_save_state(flip_ctx *ctx, const char *path)
         File f;
         f.f = fopen(path, "wb");
         if (!f.f) return flip_internal_fail(FLIP_E_ARG, "cannot open checkpoint for writing");
-        int count = (int)np, pour_time = -1;
-        float md = c->max_dens, pour_pos[2] = {0.0f, 0.0f}, pour_rad = 0.12f;  // flip3D.cpp:40-42 defaults
+        int count = (int)np, pour_time = c->pour_time;
+        float md = c->max_dens, pour_pos[2] = {c->pour_pos[0], c->pour_pos[1]}, pour_rad = c->pour_rad;
         bool ok = fwrite(&count, 4, 1, f.f) == 1 && fwrite(&md, 4, 1, f.f) == 1 && fwrite(&pour_time, 4, 1, f.f) == 1 &&
                   fwrite(pour_pos, 8, 1, f.f) == 1 && fwrite(&pour_rad, 4, 1, f.f) == 1;
         ok = ok && (np == 0 || fwrite(rec.data(), sizeof(RefParticle), np, f.f) == (size_t)np);
@@ -201,6 +201,9 @@ extern "C" int flip_gpu_load_state(flip_ctx *ctx, const char *path)
         if (rc) return rc;
         c->max_dens = md;
         FLIP_CHECK(hipMemcpy(c->press, press.data(), c->C * sizeof(float), hipMemcpyHostToDevice));
+        // the emitter travels with the state (flip3D.cpp:536-538); the caller
+        // sets the step number, which the reference takes from the file name
+        set_emitter(c, pour_pos[0], pour_pos[1], pour_rad, pour_time);
         return FLIP_OK;
     } catch (const flip_hip_error &e) {
         return flip_internal_fail(FLIP_E_HIP, e.what.c_str());

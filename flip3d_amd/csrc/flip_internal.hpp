@@ -108,6 +108,12 @@ struct FlipCtx {
 
     float max_dens = 1.0f;
     double dt = 6e-3;
+    // pourWater emitter (flip3D.cpp:41-43,375-402) and the step counter it
+    // reads (flip3D.cpp:59,1196); emit_pos holds the disc's particles
+    int step = 0, pour_time = -1;
+    float pour_pos[2] = {0.0f, 0.0f}, pour_rad = 0.12f;
+    float4 *emit_pos = nullptr;
+    int emit_n = 0;
     GlibcRand rng;
     bool uploaded = false;
     bool sorted = false;  // storage order matches the current positions
@@ -137,6 +143,9 @@ void k_launch_g2p(FlipCtx *c);                  // FLIP/PIC blend
 void k_launch_advect(FlipCtx *c);               // advect + clamp + wall push-out
 int k_stuck_and_reposition(FlipCtx *c);         // returns repositioned count (host syncs)
 int k_correct(FlipCtx *c);                      // corrector; returns jitter events
+void k_launch_emit(FlipCtx *c);                 // append the emitter's disc (capacity already ensured)
+void ensure_particle_capacity(FlipCtx *c, int64_t need);  // grow, keeping the particles
+void set_emitter(FlipCtx *c, float x, float z, float r, int pour_time);
 void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n);  // exclusive scan, out[n] = total
 int k_build_fluid_list(FlipCtx *c);             // returns nfl (host sync)
 int k_solve(FlipCtx *c, flip_step_stats *st);   // PCG on (A, L, press, div)

@@ -1059,9 +1059,34 @@ __global__ void k_selftest(int64_t m, const float *x, const float *y, float *out
     out[4 * i + 3] = (float)((double)a / (double)b);
 }
 
+// flip3D.cpp:375-402 pourWater: the disc's particles appended in disc order
+// after the current ones, storage slot = creation index = np + t (the
+// reference appends to its particle vector); velocity (0, uy, 0), dens =
+// max_dens, m = 1, zero normal.
+__global__ void k_emit(int m, int64_t np, const float4 *__restrict__ disc, float uy, float dens, float4 *pos,
+                       float4 *vel, int32_t *meta, float4 *nrm_id)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    int64_t s = np + t;
+    pos[s] = disc[t];
+    vel[s] = make_float4(0.0f, uy, 0.0f, dens);
+    meta[s] = (int32_t)((uint32_t)s << 2 | (uint32_t)FLIP_FLUID);
+    nrm_id[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- launchers
+void k_launch_emit(FlipCtx *c)
+{
+    if (c->emit_n <= 0) return;
+    const float uy = (float)(-0.5 * c->prm.density / c->n / c->prm.dt);
+    hipLaunchKernelGGL(k_emit, nblk(c->emit_n, TB), TB, 0, c->stream, c->emit_n, c->np, c->emit_pos, uy, c->max_dens,
+                       c->P4(), c->V4(), c->M(), c->nrm_id);
+    c->np += c->emit_n;
+    c->sorted = false;
+}
 void k_launch_density(FlipCtx *c)
 {
     if (c->np == 0) return;

@@ -149,9 +149,17 @@ void finish(Scene &s, int N)
     s = t;
     s.nrm.assign(3 * s.size(), 0.0f);
     double wt = 1.0 / N;
-    std::vector<size_t> walls;
+    // wall particles by (cell, index): a cell's bucket in bucket order
+    auto cell_key = [&](const float *q) {
+        int qi = (int)fmax(0.0, fmin((double)(N - 1), (double)((float)N * q[0])));
+        int qj = (int)fmax(0.0, fmin((double)(N - 1), (double)((float)N * q[1])));
+        int qk = (int)fmax(0.0, fmin((double)(N - 1), (double)((float)N * q[2])));
+        return ((long)qi * N + qj) * N + qk;
+    };
+    std::vector<std::pair<long, size_t>> walls;
     for (size_t i = 0; i < s.size(); ++i)
-        if (s.type[i] == FLIP_WALL) walls.push_back(i);
+        if (s.type[i] == FLIP_WALL) walls.push_back({cell_key(&s.p[3 * i]), i});
+    std::sort(walls.begin(), walls.end());
     for (size_t i = 0; i < s.size(); ++i) {
         if (s.type[i] != FLIP_WALL) continue;
         float *p = &s.p[3 * i], *nr = &s.nrm[3 * i];
@@ -165,19 +173,17 @@ void finish(Scene &s, int N)
             // interior wall: inverse-distance sum over wall particles in the
             // 7x7x7 cell window, visited in bucket order (flip3D.cpp:107-118)
             int ci = cell_b(N, p[0]), cj = cell_b(N, p[1]), ck = cell_b(N, p[2]);
-            std::vector<std::pair<long, size_t>> cand;
-            for (size_t t2 : walls) {
-                if (t2 == i) continue;
+            std::vector<size_t> cand;
+            for (int si = std::max(ci - 3, 0); si <= std::min(ci + 3, N - 1); ++si)
+                for (int sj = std::max(cj - 3, 0); sj <= std::min(cj + 3, N - 1); ++sj) {
+                    long k0 = ((long)si * N + sj) * N + std::max(ck - 3, 0);
+                    long k1 = ((long)si * N + sj) * N + std::min(ck + 3, N - 1);
+                    auto a = std::lower_bound(walls.begin(), walls.end(), std::make_pair(k0, (size_t)0));
+                    for (; a != walls.end() && a->first <= k1; ++a)
+                        if (a->second != i) cand.push_back(a->second);
+                }
+            for (size_t t2 : cand) {
                 const float *q = &s.p[3 * t2];
-                int qi = (int)fmax(0.0, fmin((double)(N - 1), (double)((float)N * q[0])));
-                int qj = (int)fmax(0.0, fmin((double)(N - 1), (double)((float)N * q[1])));
-                int qk = (int)fmax(0.0, fmin((double)(N - 1), (double)((float)N * q[2])));
-                if (std::abs(qi - ci) <= 3 && std::abs(qj - cj) <= 3 && std::abs(qk - ck) <= 3)
-                    cand.push_back({((long)qi * N + qj) * N + qk, t2});
-            }
-            std::sort(cand.begin(), cand.end());
-            for (auto &cq : cand) {
-                const float *q = &s.p[3 * cq.second];
                 float d = length3(p, q);
                 float w = (float)(1.0 / d);
                 nr[0] += w * (p[0] - q[0]) / d;
@@ -198,6 +204,57 @@ void finish(Scene &s, int N)
 
 extern "C" int flip_gpu_upload_particles(flip_ctx *ctx, int64_t np, const float *pos, const float *vel,
                                          const float *nrm, const int8_t *type, const float *mass, const float *dens);
+
+static Obj box_obj(int type, double x0, double x1, double y0, double y1, double z0, double z1)
+{
+    Obj b{};
+    b.type = type;
+    b.shape = 0;
+    b.lo[0] = (float)x0; b.hi[0] = (float)x1;
+    b.lo[1] = (float)y0; b.hi[1] = (float)y1;
+    b.lo[2] = (float)z0; b.hi[2] = (float)z1;
+    return b;
+}
+
+static Obj sphere_obj(int type, double cx, double cy, double cz, double r)
+{
+    Obj b{};
+    b.type = type;
+    b.shape = 1;
+    b.c[0] = (float)cx; b.c[1] = (float)cy; b.c[2] = (float)cz;
+    b.r = (float)r;
+    return b;
+}
+
+// The reference's scene functions, appended after placeWalls' six walls:
+// damBreakTest (flip3D.cpp:277-297), spherePourTest (:203-228, a wall
+// sphere over a thin pool, pouring for 200 steps), cliffPourTest (:249-275,
+// a wall box cliff beside a thin pool, pouring for 200 steps) and
+// waterDropTest (:230-247).  The pour scenes set the emitter as they do.
+static void test_scene_objects(FlipCtx *c, int scene, std::vector<Obj> &o)
+{
+    const double wt = 1.0 / c->n;
+    switch (scene) {
+    case FLIP_SCENE_REFERENCE:
+        o.push_back(box_obj(FLIP_FLUID, 0.2, 0.4, wt, 0.4, 0.2, 0.8));
+        o.push_back(box_obj(FLIP_FLUID, wt, 1.0 - wt, wt, 0.06, wt, 1.0 - wt));
+        break;
+    case FLIP_SCENE_SPHERE_POUR:
+        o.push_back(sphere_obj(FLIP_WALL, 0.5, 0.4, 0.47, 0.15));
+        o.push_back(box_obj(FLIP_FLUID, wt, 1.0 - wt, wt, 0.05, wt, 1.0 - wt));
+        set_emitter(c, 0.5f, 0.5f, 0.1f, 200);
+        break;
+    case FLIP_SCENE_CLIFF_POUR:
+        o.push_back(box_obj(FLIP_WALL, 0.0, 0.35, 0.0, 0.4, -0.002, 1.002));
+        o.push_back(box_obj(FLIP_FLUID, 0.35, 1.0 - wt, wt, 0.05, wt, 1.0 - wt));
+        set_emitter(c, 0.2f, 0.2f, 0.1f, 200);
+        break;
+    case FLIP_SCENE_WATER_DROP:
+        o.push_back(box_obj(FLIP_FLUID, wt, 1.0 - wt, wt, 0.2, wt, 1.0 - wt));
+        o.push_back(sphere_obj(FLIP_FLUID, 0.5, 0.6, 0.5, 0.12));
+        break;
+    }
+}
 
 // max_dens (flip3D.cpp:689-709): densities of a 10^3 lattice at
 // (i+0.5)*DENSITY/N with max_dens = 1, maximum over the lattice
@@ -241,16 +298,13 @@ extern "C" int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int
         add_walls(o, N);
         Scene s;
         float wt = (float)(1.0 / N);
-        if (scene == FLIP_SCENE_REFERENCE) {
-            // damBreakTest (flip3D.cpp:277-297); 3 rand() per placed particle
-            Obj a{}, b{};
-            a.type = FLIP_FLUID;
-            a.lo[0] = 0.2f; a.hi[0] = 0.4f; a.lo[1] = wt; a.hi[1] = 0.4f; a.lo[2] = 0.2f; a.hi[2] = 0.8f;
-            b.type = FLIP_FLUID;
-            b.lo[0] = wt; b.hi[0] = (float)(1.0 - wt); b.lo[1] = wt; b.hi[1] = 0.06f; b.lo[2] = wt;
-            b.hi[2] = (float)(1.0 - wt);
-            o.push_back(a);
-            o.push_back(b);
+        c->step = 0;
+        set_emitter(c, 0.0f, 0.0f, 0.12f, -1);  // flip3D.cpp:41-43 defaults: no pour
+        if (scene == FLIP_SCENE_REFERENCE || scene == FLIP_SCENE_SPHERE_POUR || scene == FLIP_SCENE_CLIFF_POUR ||
+            scene == FLIP_SCENE_WATER_DROP) {
+            // the reference's test scenes after placeWalls, placed through
+            // pushParticle: 3 rand() per placed particle
+            test_scene_objects(c, scene, o);
             GlibcRand &rng = c->rng;
             place_fluid(s, o, N, dens, [&](Scene &sc, double x, double y, double z) {
                 float px = (float)(x + 0.01 * 1 * 0.2 * ((rng.next() % 101) / 50.0 - 1.0) / N);

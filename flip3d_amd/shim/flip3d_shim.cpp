@@ -27,6 +27,7 @@ struct Shim {
     std::map<long, flip_ctx *> ctx;  // (grid size, density key) -> context
     flip_ctx *sim = nullptr;         // flip3D::init / simulateStep state
     int sim_n = 32;                  // the reference's #define N (flip3D.cpp:25)
+    int sim_scene = FLIP_SCENE_REFERENCE;  // placeObjects' damBreakTest (flip3D.cpp:365-373)
     char subcell = 0;                // solver.cpp:13 default
     ~Shim() { release(); }
     void release()
@@ -222,10 +223,11 @@ void flip3D::init(int load_step)
         char path[256];
         std::snprintf(path, sizeof(path), "render/save/%d_scene.dat", load_step);
         check(flip_gpu_load_state(s.sim, path), "flip_gpu_load_state");
+        check(flip_gpu_set_step(s.sim, load_step), "flip_gpu_set_step");  // flip3D.cpp:577
         return;
     }
     int64_t np = 0;
-    check(flip_gpu_build_scene(s.sim, FLIP_SCENE_REFERENCE, 0, &np), "flip_gpu_build_scene");
+    check(flip_gpu_build_scene(s.sim, s.sim_scene, 0, &np), "flip_gpu_build_scene");
 }
 
 void flip3D::simulateStep()
@@ -237,6 +239,8 @@ void flip3D::simulateStep()
 
 // ------------------------------------------------------------- shim extras
 void flip3d_shim::set_grid_size(int n) { shim().sim_n = n; }
+
+void flip3d_shim::set_scene(int scene) { shim().sim_scene = scene; }
 
 flip_ctx *flip3d_shim::context() { return shim().sim; }
 

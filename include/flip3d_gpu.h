@@ -122,6 +122,17 @@ int flip_gpu_upload_particles(flip_ctx *ctx, int64_t np, const float *pos, const
 int flip_gpu_download_particles(flip_ctx *ctx, float *pos, float *vel, float *nrm, int8_t *type,
                                 float *dens);
 int flip_gpu_num_particles(flip_ctx *ctx, int64_t *np);
+
+/* pourWater (flip3D.cpp:375-402) and the step counter it reads (flip3D.cpp:59,
+ * 1196).  flip_gpu_step first advances the counter; while it is <= pour_time
+ * the step then appends a disc of FLUID particles (centre (x, z), radius r, on
+ * the DENSITY/N lattice just under the ceiling wall, velocity -0.5 DENSITY/N/dt
+ * along y, dens = max_dens) before anything else, as simulateStep does.
+ * pour_time < 0 switches the emitter off.  Particle storage grows as needed.
+ * set_step is loadState's step = load_step (flip3D.cpp:577). */
+int flip_gpu_set_emitter(flip_ctx *ctx, float x, float z, float r, int pour_time);
+int flip_gpu_set_step(flip_ctx *ctx, int step);
+int flip_gpu_get_step(flip_ctx *ctx, int *step);
 int flip_gpu_set_max_dens(flip_ctx *ctx, float max_dens); /* flip3D.cpp:58 */
 int flip_gpu_get_max_dens(flip_ctx *ctx, float *max_dens);
 int flip_gpu_srand(flip_ctx *ctx, unsigned seed);          /* stdlib srand */
@@ -187,6 +198,15 @@ int flip_gpu_set_profiling(flip_ctx *ctx, int on);
 #define FLIP_SCENE_BENCHMARK 1
 #define FLIP_SCENE_RANDOM 2 /* 1-4 liquid boxes (edges U(0.1,0.6)) / spheres (r U(0.05,0.2)), seeded */
 #define FLIP_SCENE_RANDOM_BOXES 3 /* 1-3 liquid boxes (edges U(0.1,0.6)), seeded (ensemble scenes) */
+/* The reference's other test scenes, placed like scene 0 (rand() jitter):
+ * spherePourTest (flip3D.cpp:203-228): a wall sphere over a thin pool, pouring;
+ * cliffPourTest (flip3D.cpp:249-275): a wall box cliff beside a thin pool, pouring;
+ * waterDropTest (flip3D.cpp:230-247): a pool and a liquid sphere above it.
+ * Every scene resets the step counter; the pour scenes set the emitter as
+ * their reference functions do (200 steps), the others switch it off. */
+#define FLIP_SCENE_SPHERE_POUR 4
+#define FLIP_SCENE_CLIFF_POUR 5
+#define FLIP_SCENE_WATER_DROP 6
 int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np);
 
 /* Checkpoints in the reference's render/save/<step>_scene.dat layout

@@ -79,6 +79,10 @@ void correct(sorter *sort, std::vector<particle *> &particle, FLOAT dt, FLOAT re
 namespace flip3d_shim {
 // Grid size used by flip3D::init (default 32, the reference's #define N).
 void set_grid_size(int n);
+// Scene flip3D::init(0) builds: FLIP_SCENE_REFERENCE (damBreakTest, the
+// default, as placeObjects has it), FLIP_SCENE_SPHERE_POUR, _CLIFF_POUR or
+// _WATER_DROP (the scene functions placeObjects leaves commented out).
+void set_scene(int scene);
 // The context flip3D::init created, for callers that render or export.
 flip_ctx *context();
 // Copies the GPU-resident particles out in reference record layout.

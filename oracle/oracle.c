@@ -52,6 +52,9 @@ struct orc_sim {
     real *r, *z, *s, *tmp;
     real max_dens;
     double cur_dt;
+    /* step counter and pourWater emitter (flip3D.cpp:41-43,59,375-402) */
+    int step, pour_time;
+    real pour_pos[2], pour_rad;
     /* glibc random_r TYPE_3 state: 31 words, front/rear taps 3 apart */
     int32_t rs[31];
     int rf, rr;
@@ -164,6 +167,8 @@ orc_sim *orc_create(const orc_params *prm)
     s->cell_start = (int32_t *)calloc(s->C + 1, sizeof(int32_t));
     s->cell_fill = (int32_t *)calloc(s->C, sizeof(int32_t));
     s->cur_dt = prm->dt;
+    s->pour_time = -1;
+    s->pour_rad = (real)0.12;
     orc_srand(s, 1);
     return s;
 }
@@ -192,6 +197,25 @@ static void reserve_particles(orc_sim *s, int cap)
     s->type = (int8_t *)calloc((size_t)cap + 1, 1);
     s->remove = (int8_t *)calloc((size_t)cap + 1, 1);
     s->cell_pid = (int32_t *)calloc((size_t)cap + 1, sizeof(int32_t));
+}
+
+/* capacity for at least `need` particles, keeping the first np */
+static void grow_particles(orc_sim *s, int need)
+{
+    if (need <= s->cap) return;
+    int cap = need > 2 * s->cap ? need : 2 * s->cap;
+    size_t v3 = (3 * (size_t)cap + 3) * sizeof(real), v1 = ((size_t)cap + 1) * sizeof(real);
+    s->p = (real *)realloc(s->p, v3);
+    s->u = (real *)realloc(s->u, v3);
+    s->nrm = (real *)realloc(s->nrm, v3);
+    s->tmp0 = (real *)realloc(s->tmp0, v3);
+    s->tmp1 = (real *)realloc(s->tmp1, v3);
+    s->m = (real *)realloc(s->m, v1);
+    s->dens = (real *)realloc(s->dens, v1);
+    s->type = (int8_t *)realloc(s->type, (size_t)cap + 1);
+    s->remove = (int8_t *)realloc(s->remove, (size_t)cap + 1);
+    s->cell_pid = (int32_t *)realloc(s->cell_pid, ((size_t)cap + 1) * sizeof(int32_t));
+    s->cap = cap;
 }
 
 void orc_destroy(orc_sim *s)
@@ -1215,11 +1239,56 @@ static void choose_dt(orc_sim *s)
 }
 
 /* flip3D.cpp:1194-1264 simulateStep, timed region :1216 -> :1247 */
+void orc_set_emitter(orc_sim *s, real x, real z, real r, int pour_time)
+{
+    s->pour_pos[0] = x;
+    s->pour_pos[1] = z;
+    s->pour_rad = r;
+    s->pour_time = pour_time;
+}
+
+void orc_set_step(orc_sim *s, int step) { s->step = step; }
+int orc_get_step(const orc_sim *s) { return s->step; }
+
+/* flip3D.cpp:375-402 pourWater(limit): while step <= limit, a disc of FLUID
+ * particles on the DENSITY/N lattice in x and z (float coordinates stepped by
+ * the double spacing), just under the ceiling wall, moving down half a
+ * lattice spacing per DT.  Returns the number added. */
+int orc_pour(orc_sim *s)
+{
+    if (s->step > s->pour_time) return 0;
+    int N = s->n;
+    double w = s->prm.density / N;
+    int cnt = 0;
+    for (real x = (real)(w + w / 2.0); x < 1.0 - w / 2.0; x += w) {
+        for (real z = (real)(w + w / 2.0); z < 1.0 - w / 2.0; z += w) {
+            if (R_HYPOT(x - s->pour_pos[0], z - s->pour_pos[1]) < s->pour_rad) {
+                grow_particles(s, s->np + 1);
+                int q = s->np++;
+                s->p[3 * q + 0] = x;
+                s->p[3 * q + 1] = (real)(1.0 - 1.0 / N - 2.5 * s->prm.density / N);
+                s->p[3 * q + 2] = z;
+                s->u[3 * q + 0] = 0.0;
+                s->u[3 * q + 1] = (real)(-0.5 * s->prm.density / N / s->prm.dt);
+                s->u[3 * q + 2] = 0.0;
+                s->nrm[3 * q] = s->nrm[3 * q + 1] = s->nrm[3 * q + 2] = 0.0;
+                s->type[q] = FLUID;
+                s->dens[q] = s->max_dens;
+                s->m[q] = 1.0;
+                cnt++;
+            }
+        }
+    }
+    return cnt;
+}
+
 void orc_step(orc_sim *s, orc_stats *st)
 {
     orc_stats tmp;
     if (!st) st = &tmp;
     memset(st, 0, sizeof(*st));
+    s->step++; /* flip3D.cpp:1196 */
+    orc_pour(s); /* :1199 */
     choose_dt(s);
     st->dt = s->cur_dt;
     orc_sort(s);
@@ -1392,28 +1461,74 @@ static void scene_finish(orc_sim *s)
     memset(s->A, AIR, s->C);
 }
 
-/* flip3D.cpp:649-764 init(0) with the damBreakTest scene (:277-297): fluid
- * lattice with rand() jitter, then wall particles at wall-cell centres.
- * Consumes 3 rand() per placed particle, as the reference does. */
-int orc_scene_reference(orc_sim *s)
+static void box_obj(scene_obj *o, int type, double x0, double x1, double y0, double y1, double z0, double z1)
+{
+    memset(o, 0, sizeof(*o));
+    o->type = type;
+    o->shape = 0;
+    o->lo[0] = (real)x0; o->hi[0] = (real)x1;
+    o->lo[1] = (real)y0; o->hi[1] = (real)y1;
+    o->lo[2] = (real)z0; o->hi[2] = (real)z1;
+}
+
+static void sphere_obj(scene_obj *o, int type, double cx, double cy, double cz, double r)
+{
+    memset(o, 0, sizeof(*o));
+    o->type = type;
+    o->shape = 1;
+    o->c[0] = (real)cx; o->c[1] = (real)cy; o->c[2] = (real)cz;
+    o->r = (real)r;
+}
+
+/* The reference's scene functions after placeWalls (flip3D.cpp:300-362):
+ * 0 damBreakTest (:277-297), 1 spherePourTest (:203-228), 2 cliffPourTest
+ * (:249-275), 3 waterDropTest (:230-247).  Sets the emitter of the pour
+ * scenes.  Returns the object count, -1 for an unknown scene. */
+static int scene_objects(orc_sim *s, int which, scene_obj *o)
+{
+    int N = s->n;
+    double wt = 1.0 / N;
+    int no = scene_walls(o, N);
+    switch (which) {
+    case 0:
+        box_obj(&o[no++], FLUID, 0.2, 0.4, wt, 0.4, 0.2, 0.8);
+        box_obj(&o[no++], FLUID, wt, 1.0 - wt, wt, 0.06, wt, 1.0 - wt);
+        break;
+    case 1:
+        sphere_obj(&o[no++], WALL, 0.5, 0.4, 0.47, 0.15);
+        box_obj(&o[no++], FLUID, wt, 1.0 - wt, wt, 0.05, wt, 1.0 - wt);
+        orc_set_emitter(s, (real)0.5, (real)0.5, (real)0.1, 200);
+        break;
+    case 2:
+        box_obj(&o[no++], WALL, 0.0, 0.35, 0.0, 0.4, -0.002, 1.002);
+        box_obj(&o[no++], FLUID, 0.35, 1.0 - wt, wt, 0.05, wt, 1.0 - wt);
+        orc_set_emitter(s, (real)0.2, (real)0.2, (real)0.1, 200);
+        break;
+    case 3:
+        box_obj(&o[no++], FLUID, wt, 1.0 - wt, wt, 0.2, wt, 1.0 - wt);
+        sphere_obj(&o[no++], FLUID, 0.5, 0.6, 0.5, 0.12);
+        break;
+    default:
+        return -1;
+    }
+    return no;
+}
+
+/* flip3D.cpp:649-764 init(0) with one of the reference's test scenes: fluid
+ * lattice with rand() jitter (pushParticle, :580-647), then wall particles
+ * at wall-cell centres.  Consumes 3 rand() per placed particle, as the
+ * reference does. */
+int orc_scene_test(orc_sim *s, int which)
 {
     int N = s->n;
     double dens = s->prm.density;
+    scene_obj o[10];
+    s->pour_time = -1;
+    int no = scene_objects(s, which, o);
+    if (no < 0) return -1;
     reserve_particles(s, (int)orc_scene_capacity(N));
     s->np = 0;
-    scene_obj o[8];
-    int no = scene_walls(o, N);
-    real wt = (real)(1.0 / N);
-    o[no].type = FLUID; o[no].shape = 0;
-    o[no].lo[0] = (real)0.2; o[no].hi[0] = (real)0.4;
-    o[no].lo[1] = wt; o[no].hi[1] = (real)0.4;
-    o[no].lo[2] = (real)0.2; o[no].hi[2] = (real)0.8;
-    no++;
-    o[no].type = FLUID; o[no].shape = 0;
-    o[no].lo[0] = wt; o[no].hi[0] = (real)(1.0 - wt);
-    o[no].lo[1] = wt; o[no].hi[1] = (real)0.06;
-    o[no].lo[2] = wt; o[no].hi[2] = (real)(1.0 - wt);
-    no++;
+    s->step = 0;
     s->max_dens = orc_calibrate_max_dens(N, dens);
     double w = dens * (1.0 / N);
     double wthick = 1.0 / N;
@@ -1449,6 +1564,8 @@ int orc_scene_reference(orc_sim *s)
     scene_finish(s);
     return s->np;
 }
+
+int orc_scene_reference(orc_sim *s) { return orc_scene_test(s, 0); }
 
 /* SplitMix64 finaliser; value k of the stream seeded with `seed` is
  * mix(seed + (k+1)*golden). */

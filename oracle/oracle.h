@@ -135,6 +135,15 @@ void orc_apply_precond(const orc_params *prm, const int8_t *A, const double *P,
 long orc_scene_capacity(int n);
 real orc_calibrate_max_dens(int n, double density);
 int orc_scene_reference(orc_sim *s);
+/* the reference's test scenes: 0 damBreakTest, 1 spherePourTest, 2
+ * cliffPourTest, 3 waterDropTest (pour scenes also set the emitter) */
+int orc_scene_test(orc_sim *s, int which);
+/* pourWater emitter (flip3D.cpp:375-402): disc centre (x, z), radius, last
+ * step; orc_step advances the step counter and pours first */
+void orc_set_emitter(orc_sim *s, real x, real z, real r, int pour_time);
+void orc_set_step(orc_sim *s, int step);
+int orc_get_step(const orc_sim *s);
+int orc_pour(orc_sim *s);
 int orc_scene_benchmark(orc_sim *s, uint64_t seed);
 void orc_compute_wall_normals(orc_sim *s);
 

@@ -102,6 +102,11 @@ def _lib(f64):
     lib.orc_calibrate_max_dens.restype = rt
     lib.orc_scene_reference.argtypes = [P]
     lib.orc_scene_benchmark.argtypes = [P, C.c_uint64]
+    lib.orc_scene_test.argtypes = [P, C.c_int]
+    lib.orc_set_emitter.argtypes = [P, rt, rt, rt, C.c_int]
+    lib.orc_set_step.argtypes = [P, C.c_int]
+    lib.orc_get_step.argtypes = [P]
+    lib.orc_pour.argtypes = [P]
     return lib
 
 
@@ -238,6 +243,22 @@ class Sim:
     def scene_benchmark(self, seed=20261203):
         return self.L.orc_scene_benchmark(self.h, seed)
 
+    def scene_test(self, which):
+        """The reference's test scenes: 0 damBreak, 1 spherePour, 2 cliffPour,
+        3 waterDrop (flip3D.cpp:203-297); pour scenes set the emitter."""
+        return self.L.orc_scene_test(self.h, which)
+
+    def set_emitter(self, x, z, r, pour_time):
+        self.L.orc_set_emitter(self.h, x, z, r, pour_time)
+
+    @property
+    def step_count(self):
+        return self.L.orc_get_step(self.h)
+
+    @step_count.setter
+    def step_count(self, v):
+        self.L.orc_set_step(self.h, v)
+
     def fetch_velocity(self, grid_set, pts):
         pts = np.ascontiguousarray(pts, self.dtype).reshape(-1, 3)
         out = np.empty_like(pts)
@@ -339,4 +360,6 @@ def ref_lib():
     lib.ref_kat_kernels.argtypes = [C.c_int, P, P, P, P]
     lib.ref_kat_length.argtypes = [C.c_int, P, P, P, P]
     lib.ref_save_state.argtypes = [C.c_int]
+    lib.ref_init_test.argtypes = [C.c_int, C.c_uint]
+    lib.ref_set_step.argtypes = [C.c_int]
     return lib

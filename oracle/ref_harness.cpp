@@ -93,6 +93,75 @@ void ref_set_particles(int np, const float *p, const float *u, const float *nrm,
 
 void ref_compute_wall_normal() { compute_wall_normal(); }
 
+// flip3D::init(0) with one of the reference's test scenes in place of the
+// damBreakTest that placeObjects (flip3D.cpp:365-373) hard-wires:
+// 0 damBreakTest, 1 spherePourTest, 2 cliffPourTest, 3 waterDropTest.
+// init(0) runs first for its allocation and max_dens calibration (no rand()
+// draws); its particles are then discarded and the placement of
+// flip3D.cpp:711-764 is driven again through the reference's own
+// pushParticle, sorter, markWater and compute_wall_normal, with the rand()
+// stream reseeded to `seed` before the placement.
+int ref_init_test(int which, unsigned seed)
+{
+    // init's calibration runs over whatever the particle vector holds
+    clear_particles();
+    objects.clear();
+    flip3D::init(0);
+    clear_particles();
+    objects.clear();
+    pourTime = -1;
+    step = 0;
+    placeWalls();
+    switch (which) {
+    case 0: damBreakTest(); break;
+    case 1: spherePourTest(); break;
+    case 2: cliffPourTest(); break;
+    case 3: waterDropTest(); break;
+    default: return -1;
+    }
+    srand(seed);
+    const double lw = DENSITY * WALL_THICK;
+    const int lim = (int)(N / DENSITY);
+    for (int a = 0; a < lim; a++)
+        for (int b = 0; b < lim; b++)
+            for (int c = 0; c < lim; c++) {
+                double x = a * lw + lw / 2.0, y = b * lw + lw / 2.0, z = c * lw + lw / 2.0;
+                bool in = x > WALL_THICK && x < 1.0 - WALL_THICK && y > WALL_THICK && y < 1.0 - WALL_THICK &&
+                          z > WALL_THICK && z < 1.0 - WALL_THICK;
+                if (in) pushParticle(x, y, z, FLUID);
+            }
+    const double cw = 1.0 / N;
+    for (int a = 0; a < N; a++)
+        for (int b = 0; b < N; b++)
+            for (int c = 0; c < N; c++) pushParticle(a * cw + cw / 2.0, b * cw + cw / 2.0, c * cw + cw / 2.0, WALL);
+    sort->sort(particles);
+    sort->markWater(A, wall_normal[3], DENSITY);
+    std::vector<particle *> kept;
+    for (size_t n = 0; n < particles.size(); n++) {
+        particle *q = particles[n];
+        int ci = fmin(N - 1, fmax(0, q->p[0] * N)), cj = fmin(N - 1, fmax(0, q->p[1] * N)),
+            ck = fmin(N - 1, fmax(0, q->p[2] * N));
+        if (q->type != WALL && A[ci][cj][ck] == WALL) {
+            delete q;
+            continue;
+        }
+        kept.push_back(q);
+    }
+    particles = kept;
+    compute_wall_normal();
+    return (int)particles.size();
+}
+
+// simulateStep's step counter and pourWater (flip3D.cpp:1196-1199)
+void ref_set_step(int s) { step = s; }
+int ref_get_step() { return step; }
+int ref_pour()
+{
+    size_t before = particles.size();
+    pourWater(pourTime);
+    return (int)(particles.size() - before);
+}
+
 // grids: 0..2 u, 3..5 u_save, 6 A, 7 L, 8 press
 void ref_get_grid(int which, void *out)
 {

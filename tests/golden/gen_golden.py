@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 SECTIONS = ["kernels", "sort", "p2g", "g2p", "correct", "solve", "scene_ref", "traj_ref", "traj_bench",
-            "checkpoint"]
+            "checkpoint", "scenes_test"]
 
 
 def digest(a):
@@ -287,6 +287,39 @@ def sec_traj_bench(out):
     p, u, nr, t, d = _ref_particles(R)
     np.savez_compressed(out + ".init.npz", p=p, n=nr, t=t, max_dens=np.float32(R.ref_get_max_dens()))
     _trajectory(R, 100, (1, 10), out)
+
+
+def sec_scenes_test(out):
+    """The reference's other test scenes (spherePourTest, cliffPourTest,
+    waterDropTest: obstacles, interior wall normals and the pourWater
+    emitter) at N=32: the initial particles, then 12 steps of simulateStep
+    order (step counter, pourWater, the five stages) with per-step digests
+    and full arrays at steps 1 and 12."""
+    R = ref()
+    rec = {}
+    for which, name, steps in ((1, "sphere_pour", 12), (2, "cliff_pour", 12), (3, "water_drop", 0)):
+        with Quiet():
+            R.ref_init_test(which, 1)
+        p, u, nr, t, d = _ref_particles(R)
+        rec.update({f"{name}_init_p": p, f"{name}_init_n": nr, f"{name}_init_t": t,
+                    f"{name}_max_dens": np.float32(R.ref_get_max_dens())})
+        R.ref_srand(1)
+        for s in range(1, steps + 1):
+            R.ref_set_step(s)
+            added = R.ref_pour()
+            with Quiet():
+                R.ref_step()
+            p, u, nr, t, d = _ref_particles(R)
+            g = _ref_grids(R)
+            fl = t == 1
+            rec[f"{name}_dig{s}"] = np.array([digest(p), digest(u[fl]), digest(d), digest(g["press"]),
+                                              digest(g["A"]), digest(g["L"])])
+            rec[f"{name}_added{s}"] = np.int32(added)
+            if s in (1, steps):
+                rec.update({f"{name}_full{s}_p": p, f"{name}_full{s}_u": u, f"{name}_full{s}_t": t,
+                            f"{name}_full{s}_press": g["press"], f"{name}_full{s}_A": g["A"]})
+        rec[f"{name}_steps"] = np.int32(steps)
+    np.savez_compressed(out, **rec)
 
 
 def sec_checkpoint(out):

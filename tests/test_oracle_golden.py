@@ -200,3 +200,43 @@ def test_benchmark_trajectory():
     assert np.array_equal(q["p"], g["full10_p"])
     assert np.array_equal(q["dens"], g["full10_dens"])
     assert np.array_equal(full.grid(po.G_PRESS).reshape(-1), g["full10_press"])
+
+
+TEST_SCENES = [(1, "sphere_pour"), (2, "cliff_pour"), (3, "water_drop")]
+
+
+@pytest.mark.parametrize("which,name", TEST_SCENES)
+def test_reference_test_scenes(which, name):
+    """spherePourTest / cliffPourTest / waterDropTest placement: obstacles
+    (a wall sphere, a wall box), the interior-wall normals of
+    compute_wall_normal (flip3D.cpp:107-119) and the calibrated max_dens."""
+    g = gold("scenes_test")
+    s = po.Sim(32)
+    assert s.scene_test(which) == len(g[f"{name}_init_t"])
+    q = s.particles()
+    assert np.array_equal(q["type"], g[f"{name}_init_t"])
+    assert np.array_equal(q["p"], g[f"{name}_init_p"])
+    assert np.array_equal(q["n"], g[f"{name}_init_n"])
+    assert np.float32(s.max_dens) == g[f"{name}_max_dens"]
+
+
+@pytest.mark.parametrize("which,name", TEST_SCENES[:2])
+def test_pour_trajectories(which, name):
+    """Steps of the pour scenes in simulateStep order: step counter, pourWater
+    (flip3D.cpp:375-402) appending a disc of particles, then the step; every
+    step bit-exact against the reference's digests."""
+    g = gold("scenes_test")
+    s = po.Sim(32)
+    s.scene_test(which)
+    s.srand(1)
+    steps = int(g[f"{name}_steps"])
+    for k in range(1, steps + 1):
+        n0 = s.num_particles()
+        st = s.step()
+        q = s.particles()
+        assert s.num_particles() - n0 == int(g[f"{name}_added{k}"])
+        fl = q["type"] == po.FLUID
+        mine = [digest(q["p"]), digest(q["u"][fl]), digest(q["dens"]), digest(s.grid(po.G_PRESS)),
+                digest(s.grid(po.G_A)), digest(s.grid(po.G_L))]
+        bad = [nm for nm, a, b in zip(["p", "u", "dens", "press", "A", "L"], mine, g[f"{name}_dig{k}"]) if a != b]
+        assert not bad, f"{name} step {k}: mismatch in {bad} ({st})"
