@@ -30,7 +30,7 @@ EXPORTS = [
     "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
     "flip_gpu_save_state", "flip_gpu_load_state", "flip_gpu_step_many",
-    "flip_gpu_set_emitter", "flip_gpu_set_step", "flip_gpu_get_step",
+    "flip_gpu_set_emitter", "flip_gpu_set_step", "flip_gpu_get_step", "flip_gpu_mesh", "flip_gpu_mesh_get",
     "flip_gpu_set_learned_weights", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
     "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
     "flip_unet_predict", "flip_unet_save", "flip_unet_load",
@@ -86,6 +86,8 @@ def lib():
         L.flip_gpu_num_particles.argtypes = [P, C.POINTER(C.c_int64)]
         L.flip_gpu_set_emitter.argtypes = [P, C.c_float, C.c_float, C.c_float, C.c_int]
         L.flip_gpu_set_step.argtypes = [P, C.c_int]
+        L.flip_gpu_mesh.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.flip_gpu_mesh_get.argtypes = [P, P, P, P]
         L.flip_gpu_get_step.argtypes = [P, C.POINTER(C.c_int)]
         L.flip_gpu_set_max_dens.argtypes = [P, C.c_float]
         L.flip_gpu_get_max_dens.argtypes = [P, C.POINTER(C.c_float)]
@@ -220,6 +222,17 @@ class FlipContext:
     def set_emitter(self, x, z, r, pour_time):
         """pourWater disc (flip3D.cpp:375-402); pour_time < 0 switches it off."""
         _check(lib().flip_gpu_set_emitter(self.h, x, z, r, pour_time))
+
+    def mesh(self, mg=0):
+        """mesher::generateMesh through the last sort's buckets (mg <= 0:
+        N + 1): vertices and normals float64 (nv, 3), faces int32 (nf, 3)."""
+        nv, nf = C.c_int64(), C.c_int64()
+        _check(lib().flip_gpu_mesh(self.h, mg, C.byref(nv), C.byref(nf)))
+        v = np.empty((nv.value, 3), np.float64)
+        nr = np.empty((nv.value, 3), np.float64)
+        f = np.empty((nf.value, 3), np.int32)
+        _check(lib().flip_gpu_mesh_get(self.h, _ptr(v), _ptr(nr), _ptr(f)))
+        return v, nr, f
 
     @property
     def step_count(self):

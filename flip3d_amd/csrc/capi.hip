@@ -251,6 +251,8 @@ void free_all(FlipCtx *c)
     c->pcg_graph = nullptr;
     learned_free(c->learned);
     c->learned = nullptr;
+    mesh_free(c->mesh);
+    c->mesh = nullptr;
     dfree(c->scal);
     dfree(c->iflags);
     dfree(c->tickets);
@@ -585,6 +587,7 @@ int flip_gpu_upload_particles(flip_ctx *ctx, int64_t np, const float *pos, const
         build_wall_bins(c, p4, mt, n4);
         c->unit_mass = unit;
         c->sorted = false;
+        c->bucketed = false;
         c->uploaded = true;
         return FLIP_OK;
     });
@@ -639,6 +642,33 @@ int flip_gpu_set_emitter(flip_ctx *ctx, float x, float z, float r, int pour_time
         FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
         FLIP_CHECK(hipStreamSynchronize(c->stream));
         set_emitter(c, x, z, r, pour_time);
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_mesh(flip_ctx *ctx, int mg, int64_t *nvert, int64_t *nface)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        if (mg <= 0) mg = c->n + 1;
+        require(mg >= 2 && (int64_t)mg * mg * mg * 3 < (int64_t(1) << 31), "lattice size out of range");
+        long nv = 0, nf = 0;
+        mesh_build(c, mg, &nv, &nf);
+        if (nvert) *nvert = nv;
+        if (nface) *nface = nf;
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_mesh_get(flip_ctx *ctx, double *vertices, double *normals, int32_t *faces)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->mesh != nullptr, "no mesh built", FLIP_E_STATE);
+        mesh_copy_out(c, vertices, normals, faces);
         return FLIP_OK;
     });
 }

@@ -36,6 +36,7 @@ struct GlibcRand {
 };
 
 struct LearnedSolver;  // learned multilevel flexible-CG state (k_learned.hip)
+struct MeshState;      // buffers of the last surface mesh (k_mesh.hip)
 void learned_free(LearnedSolver *ls);
 
 // Device buffers of one context.  Particles live in cell-sorted order and are
@@ -117,6 +118,8 @@ struct FlipCtx {
     GlibcRand rng;
     bool uploaded = false;
     bool sorted = false;  // storage order matches the current positions
+    bool bucketed = false;  // cell_start describes the storage (the last sort), maybe of older positions
+    MeshState *mesh = nullptr;
     bool unit_mass = false;  // every particle has m == 1 (the reference's only mass)
 
     // timing
@@ -146,7 +149,13 @@ int k_correct(FlipCtx *c);                      // corrector; returns jitter eve
 void k_launch_emit(FlipCtx *c);                 // append the emitter's disc (capacity already ensured)
 void ensure_particle_capacity(FlipCtx *c, int64_t need);  // grow, keeping the particles
 void set_emitter(FlipCtx *c, float x, float z, float r, int pour_time);
-void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n);  // exclusive scan, out[n] = total
+// exclusive scan, out[n] = total; `blocks` holds n / 2048 + 1 block sums
+// (default: the context's scratch, sized for max(particles, cells))
+void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks = nullptr);
+// surface mesher (k_mesh.hip)
+void mesh_build(FlipCtx *c, int mg, long *nv, long *nf);
+void mesh_copy_out(FlipCtx *c, double *v, double *nrm, int32_t *faces);
+void mesh_free(MeshState *m);
 int k_build_fluid_list(FlipCtx *c);             // returns nfl (host sync)
 int k_solve(FlipCtx *c, flip_step_stats *st);   // PCG on (A, L, press, div)
 // FLIP_PRECOND_LEARNED: flexible CG from the current press and r (after the

@@ -1086,6 +1086,7 @@ void k_launch_emit(FlipCtx *c)
                        c->P4(), c->V4(), c->M(), c->nrm_id);
     c->np += c->emit_n;
     c->sorted = false;
+    c->bucketed = false;
 }
 void k_launch_density(FlipCtx *c)
 {

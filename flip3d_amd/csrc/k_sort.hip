@@ -189,16 +189,17 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
-void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n)
+void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks)
 {
     if (n <= 0) {
         hipLaunchKernelGGL(k_fill_i32, 1, 1, 0, c->stream, out, 1L, 0);
         return;
     }
+    if (!blocks) blocks = c->scan_blocks;
     int nb = (int)((n + TILE - 1) / TILE);
-    hipLaunchKernelGGL(k_scan_reduce, nb, SB, 0, c->stream, in, n, c->scan_blocks);
-    hipLaunchKernelGGL(k_scan_top, 1, SB, 0, c->stream, c->scan_blocks, nb, out + n);
-    hipLaunchKernelGGL(k_scan_apply, nb, SB, 0, c->stream, in, n, c->scan_blocks, out);
+    hipLaunchKernelGGL(k_scan_reduce, nb, SB, 0, c->stream, in, n, blocks);
+    hipLaunchKernelGGL(k_scan_top, 1, SB, 0, c->stream, blocks, nb, out + n);
+    hipLaunchKernelGGL(k_scan_apply, nb, SB, 0, c->stream, in, n, blocks, out);
 }
 
 void k_launch_sort(FlipCtx *c)
@@ -213,6 +214,7 @@ void k_launch_sort(FlipCtx *c)
     t_begin(c, "sort_scan");
     k_launch_scan(c, c->cell_count, c->cell_start, c->C);
     t_end(c);
+    c->bucketed = true;
     if (np == 0) return;
     t_begin(c, "sort_scatter");
     int32_t *pid = c->deep;  // scratch sized max(C, np) + 1: creation index per bucket slot

@@ -131,6 +131,20 @@ int flip_gpu_num_particles(flip_ctx *ctx, int64_t *np);
  * pour_time < 0 switches the emitter off.  Particle storage grows as needed.
  * set_step is loadState's step = load_step (flip3D.cpp:577). */
 int flip_gpu_set_emitter(flip_ctx *ctx, float x, float z, float r, int pour_time);
+
+/* Surface mesh.  Replaces mesher::generateMesh (src/mesher.cpp:137-195, with
+ * implicit.cpp:14-38 and rchandra/CIsoSurface.cpp:331-777) as write_mesh calls
+ * it (flip3D.cpp:1002): implicit distance field on an mg^3 lattice over the unit
+ * cube (mg <= 0: N + 1), marching cubes at iso level 0, faces with an edge of
+ * 0.1 or longer dropped, two smoothing passes.  The field reads the particles
+ * through the buckets of the last sort, as the reference's mesher reads its
+ * sorter (after a step: the corrector's sort, with the corrected positions).
+ * flip_gpu_mesh builds the mesh on the device and returns its counts;
+ * flip_gpu_mesh_get copies it out: vertices and normals as 3*nvert doubles
+ * (the reference's vector<double>), faces as 3*nface vertex indices (any
+ * pointer may be NULL). */
+int flip_gpu_mesh(flip_ctx *ctx, int mg, int64_t *nvert, int64_t *nface);
+int flip_gpu_mesh_get(flip_ctx *ctx, double *vertices, double *normals, int32_t *faces);
 int flip_gpu_set_step(flip_ctx *ctx, int step);
 int flip_gpu_get_step(flip_ctx *ctx, int *step);
 int flip_gpu_set_max_dens(flip_ctx *ctx, float max_dens); /* flip3D.cpp:58 */

@@ -1631,3 +1631,282 @@ int orc_scene_benchmark(orc_sim *s, uint64_t seed)
     scene_finish(s);
     return s->np;
 }
+
+/* ---------------------------------------------------------------- mesher */
+#include "../flip3d_amd/csrc/mc_table.h"
+
+static const char mc_cases[256][16] = {MC_CASE_LIST};
+static const unsigned char mc_edge_corners[12] = {MC_EDGE_CORNERS};
+
+/* implicit.cpp:14-38: distance to the nearest FLUID particle in the 5^3-cell
+ * window (capped at 8 DENSITY/gn) less DENSITY/gn; 4.5 DENSITY/gn when a WALL
+ * particle lies within DENSITY/gn.  The window is read through the sorter's
+ * current buckets. */
+static double implicit_at(const orc_sim *s, const real p[3], real density)
+{
+    int gn = s->n;
+    int ci = cell_of_b(gn, p[0]), cj = cell_of_b(gn, p[1]), ck = cell_of_b(gn, p[2]);
+    double phi = 8.0 * density / gn;
+    WINDOW_BEGIN(s, ci - 2, ci + 2, cj - 2, cj + 2, ck - 2, ck + 2, q)
+        const real *np = s->p + 3 * (size_t)q;
+        if (s->type[q] == WALL) {
+            if (orc_length(np, p) < density / gn) return 4.5 * density / gn;
+            continue;
+        }
+        double d = orc_length(np, p);
+        if (d < phi) phi = d;
+    WINDOW_END
+    return phi - density / gn;
+}
+
+typedef struct {
+    int set;
+    float x, y, z;
+} mc_vert;
+
+/* CIsoSurface.cpp:587-682: the point on edge e of cell (x,y,z), interpolated
+ * from the edge's first to its second end as the class orders them */
+static mc_vert mc_intersect(const float *f, int nc, float len, int x, int y, int z, int e)
+{
+    static const signed char d1[12][3] = {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 1}, {0, 1, 1},
+                                          {1, 1, 1}, {1, 0, 1}, {0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}};
+    static const signed char d2[12][3] = {{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0}, {0, 1, 1}, {1, 1, 1},
+                                          {1, 0, 1}, {0, 0, 1}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}, {1, 0, 1}};
+    unsigned v1x = x + d1[e][0], v1y = y + d1[e][1], v1z = z + d1[e][2];
+    unsigned v2x = x + d2[e][0], v2y = y + d2[e][1], v2z = z + d2[e][2];
+    float x1 = v1x * len, y1 = v1y * len, z1 = v1z * len, x2 = v2x * len, y2 = v2y * len, z2 = v2z * len;
+    unsigned np1 = nc + 1, slice = np1 * np1;
+    float a = f[v1z * slice + v1y * np1 + v1x], b = f[v2z * slice + v2y * np1 + v2x];
+    float mu = (0.0f - a) / (b - a);
+    mc_vert r = {1, x1 + mu * (x2 - x1), y1 + mu * (y2 - y1), z1 + mu * (z2 - z1)};
+    return r;
+}
+
+/* CIsoSurface.cpp:550-585 GetEdgeID */
+static unsigned mc_edge_id(int nc, int x, int y, int z, int e)
+{
+    static const signed char o[12][4] = {{0, 0, 0, 1}, {0, 1, 0, 0}, {1, 0, 0, 1}, {0, 0, 0, 0},
+                                         {0, 0, 1, 1}, {0, 1, 1, 0}, {1, 0, 1, 1}, {0, 0, 1, 0},
+                                         {0, 0, 0, 2}, {0, 1, 0, 2}, {1, 1, 0, 2}, {1, 0, 0, 2}};
+    unsigned np1 = nc + 1;
+    unsigned vx = x + o[e][0], vy = y + o[e][1], vz = z + o[e][2];
+    return 3 * (vz * np1 * np1 + vy * np1 + vx) + o[e][3];
+}
+
+static int hexval(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
+
+static int orc_isnormal(double v) { return isnormal(v) != 0; }
+
+struct orc_mesh_out {
+    int nv, nf;
+    double *v, *n;
+    int32_t *f;
+};
+static struct orc_mesh_out g_mesh;
+
+/* mesher.cpp:137-195 generateMesh(A, sort, particles, DENSITY, mg): implicit
+ * field on the mg^3 lattice (x fastest, negated), CIsoSurface<float> at iso
+ * 0 with cell length 1/(mg-1) (CIsoSurface.cpp:331-503: per-cell vertex
+ * inserts on the owned edges, triangles cell by cell in x-major order,
+ * vertices renumbered by edge id, CalculateNormals :722-777), the 0.1 edge
+ * length filter, buildConnection and two smoothMesh passes (mesher.cpp:13-101).
+ * Returns the vertex count. */
+int orc_mesh(orc_sim *s, int mg)
+{
+    int nc = mg - 1;
+    real density = (real)s->prm.density;
+    size_t npts = (size_t)mg * mg * mg;
+    float *f = (float *)malloc(npts * sizeof(float));
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < mg; ++i)
+        for (int j = 0; j < mg; ++j)
+            for (int k = 0; k < mg; ++k) {
+                double h = 1.0 / (double)(mg - 1);
+                double x = i * h, y = j * h, z = k * h;
+                real p[3] = {(real)x, (real)y, (real)z};
+                double value = implicit_at(s, p, density);
+                if (i == 0 || i == mg - 1 || j == 0 || j == mg - 1 || k == 0 || k == mg - 1) value = fmax(value, 0.01);
+                f[(size_t)(mg * mg) * k + mg * j + i] = (float)-value;
+            }
+    float len = (float)(1.0 / (mg - 1));
+    size_t nid = 3 * npts;
+    mc_vert *verts = (mc_vert *)calloc(nid, sizeof(mc_vert));
+    size_t ncell = (size_t)nc * nc * nc;
+    unsigned char *tcount = (unsigned char *)calloc(ncell, 1);
+    unsigned *tri = (unsigned *)malloc(ncell * 15 * sizeof(unsigned));
+    unsigned np1 = nc + 1, slice = np1 * np1;
+    for (int n = 0; n < nc * nc * nc; n++) {
+        int z = n / (nc * nc), y = (n % (nc * nc)) / nc, x = (n % (nc * nc)) % nc;
+        static const signed char co[8][3] = {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0},
+                                             {0, 0, 1}, {0, 1, 1}, {1, 1, 1}, {1, 0, 1}};
+        unsigned ti = 0;
+        for (int b = 0; b < 8; ++b)
+            if (f[(z + co[b][2]) * slice + (y + co[b][1]) * np1 + (x + co[b][0])] < 0.0f) ti |= 1u << b;
+        unsigned mask = 0;
+        for (int e = 0; e < 12; ++e) {
+            int a = mc_edge_corners[e] >> 4, b = mc_edge_corners[e] & 15;
+            if (((ti >> a) ^ (ti >> b)) & 1) mask |= 1u << e;
+        }
+        if (!mask) continue;
+        int own[12] = {0};
+        own[3] = own[0] = own[8] = 1;
+        if (x == nc - 1) own[2] = own[11] = 1;
+        if (y == nc - 1) own[1] = own[9] = 1;
+        if (z == nc - 1) own[4] = own[7] = 1;
+        if (x == nc - 1 && y == nc - 1) own[10] = 1;
+        if (x == nc - 1 && z == nc - 1) own[6] = 1;
+        if (y == nc - 1 && z == nc - 1) own[5] = 1;
+        for (int e = 0; e < 12; ++e)
+            if (own[e] && (mask >> e & 1)) {
+                unsigned id = mc_edge_id(nc, x, y, z, e);
+                if (!verts[id].set) verts[id] = mc_intersect(f, nc, len, x, y, z, e);
+            }
+        size_t L = ((size_t)x * nc + y) * nc + z; /* the merge order of :486-505 */
+        const char *cs = mc_cases[ti];
+        int t = 0;
+        for (int i = 0; cs[i]; i += 3, ++t)
+            for (int c = 0; c < 3; ++c) tri[L * 15 + 3 * t + c] = mc_edge_id(nc, x, y, z, hexval(cs[i + c]));
+        tcount[L] = (unsigned char)t;
+    }
+    /* renumber by edge id (RenameVerticesAndTriangles, :684-720) */
+    int32_t *rid = (int32_t *)malloc(nid * sizeof(int32_t));
+    int nv = 0;
+    for (size_t id = 0; id < nid; ++id) rid[id] = verts[id].set ? nv++ : -1;
+    float *vx = (float *)malloc(3 * (size_t)(nv + 1) * sizeof(float));
+    for (size_t id = 0; id < nid; ++id)
+        if (verts[id].set) {
+            vx[3 * rid[id]] = verts[id].x;
+            vx[3 * rid[id] + 1] = verts[id].y;
+            vx[3 * rid[id] + 2] = verts[id].z;
+        }
+    size_t ntri = 0;
+    for (size_t L = 0; L < ncell; ++L) ntri += tcount[L];
+    int32_t *tv = (int32_t *)malloc(3 * (ntri + 1) * sizeof(int32_t));
+    size_t t = 0;
+    for (size_t L = 0; L < ncell; ++L)
+        for (int k = 0; k < tcount[L]; ++k, ++t)
+            for (int c = 0; c < 3; ++c) tv[3 * t + c] = rid[tri[L * 15 + 3 * k + c]];
+    /* CalculateNormals: triangle normals summed in triangle order, then
+     * divided by their float length */
+    float *nm = (float *)calloc(3 * (size_t)(nv + 1), sizeof(float));
+    for (size_t i = 0; i < ntri; ++i) {
+        int id0 = tv[3 * i], id1 = tv[3 * i + 1], id2 = tv[3 * i + 2];
+        float v1[3], v2[3], no[3];
+        for (int c = 0; c < 3; ++c) {
+            v1[c] = vx[3 * id1 + c] - vx[3 * id0 + c];
+            v2[c] = vx[3 * id2 + c] - vx[3 * id0 + c];
+        }
+        no[0] = v1[2] * v2[1] - v1[1] * v2[2];
+        no[1] = v1[0] * v2[2] - v1[2] * v2[0];
+        no[2] = v1[1] * v2[0] - v1[0] * v2[1];
+        for (int c = 0; c < 3; ++c) {
+            nm[3 * id0 + c] += no[c];
+            nm[3 * id1 + c] += no[c];
+            nm[3 * id2 + c] += no[c];
+        }
+    }
+    for (int i = 0; i < nv; ++i) {
+        float l = sqrtf(nm[3 * i] * nm[3 * i] + nm[3 * i + 1] * nm[3 * i + 1] + nm[3 * i + 2] * nm[3 * i + 2]);
+        nm[3 * i] /= l;
+        nm[3 * i + 1] /= l;
+        nm[3 * i + 2] /= l;
+    }
+    /* mesher.cpp:167-185 faces with every edge shorter than 0.1 */
+    int32_t *faces = (int32_t *)malloc(3 * (ntri + 1) * sizeof(int32_t));
+    int nf = 0;
+    double maxlen2 = 0.1 * 0.1;
+    for (size_t i = 0; i < ntri; ++i) {
+        const int32_t *q = tv + 3 * i;
+        real a[3], b[3], c[3];
+        for (int d = 0; d < 3; ++d) {
+            a[d] = (real)vx[3 * q[0] + d];
+            b[d] = (real)vx[3 * q[1] + d];
+            c[d] = (real)vx[3 * q[2] + d];
+        }
+        if (orc_length2(a, b) < maxlen2 && orc_length2(b, c) < maxlen2 && orc_length2(a, c) < maxlen2) {
+            faces[3 * nf] = q[0];
+            faces[3 * nf + 1] = q[1];
+            faces[3 * nf + 2] = q[2];
+            nf++;
+        }
+    }
+    double *V = (double *)malloc(3 * (size_t)(nv + 1) * sizeof(double));
+    double *Nn = (double *)malloc(3 * (size_t)(nv + 1) * sizeof(double));
+    for (size_t i = 0; i < 3 * (size_t)nv; ++i) {
+        V[i] = vx[i];
+        Nn[i] = nm[i];
+    }
+    /* buildConnection (mesher.cpp:88-101): per vertex, the other two corners
+     * of each face in face order */
+    int *cnt = (int *)calloc((size_t)nv + 1, sizeof(int));
+    for (int i = 0; i < nf; ++i)
+        for (int c = 0; c < 3; ++c) cnt[faces[3 * i + c]] += 2;
+    size_t *off = (size_t *)malloc(((size_t)nv + 1) * sizeof(size_t));
+    size_t acc = 0;
+    for (int i = 0; i < nv; ++i) {
+        off[i] = acc;
+        acc += cnt[i];
+    }
+    off[nv] = acc;
+    int32_t *nb = (int32_t *)malloc((acc + 1) * sizeof(int32_t));
+    memset(cnt, 0, ((size_t)nv + 1) * sizeof(int));
+    for (int i = 0; i < nf; ++i) {
+        int a = faces[3 * i], b = faces[3 * i + 1], c = faces[3 * i + 2];
+        nb[off[a] + cnt[a]++] = b;
+        nb[off[a] + cnt[a]++] = c;
+        nb[off[b] + cnt[b]++] = a;
+        nb[off[b] + cnt[b]++] = c;
+        nb[off[c] + cnt[c]++] = a;
+        nb[off[c] + cnt[c]++] = b;
+    }
+    /* smoothMesh(.., 2, 1, 1), mesher.cpp:13-86: the copies start at zero and
+     * keep their last value where a vertex has no usable sample */
+    double *CV = (double *)calloc(3 * (size_t)(nv + 1), sizeof(double));
+    double *CN = (double *)calloc(3 * (size_t)(nv + 1), sizeof(double));
+    for (int it = 0; it < 2; ++it) {
+        for (int vi = 0; vi < nv; ++vi) {
+            float wsum = 0.0f;
+            double v[3] = {0.0, 0.0, 0.0}, vn[3] = {0.0, 0.0, 0.0};
+            for (int k = -1; k < (int)(off[vi + 1] - off[vi]); ++k) {
+                int ni = k < 0 ? vi : nb[off[vi] + k];
+                int ok = 1;
+                for (int d = 0; d < 3; ++d)
+                    if (!orc_isnormal(V[3 * ni + d]) || !orc_isnormal(Nn[3 * ni + d])) ok = 0;
+                if (!ok) continue;
+                float w = 1.0f;
+                for (int d = 0; d < 3; ++d) {
+                    v[d] += w * V[3 * ni + d];
+                    vn[d] += w * Nn[3 * ni + d];
+                }
+                wsum += w;
+            }
+            float l = hypotf((float)vn[0], hypotf((float)vn[1], (float)vn[2]));
+            for (int d = 0; d < 3; ++d) {
+                if (wsum != 0.0f) CV[3 * vi + d] = v[d] / (double)wsum;
+                if (l != 0.0f) CN[3 * vi + d] = vn[d] / l;
+            }
+        }
+        memcpy(V, CV, 3 * (size_t)nv * sizeof(double));
+        memcpy(Nn, CN, 3 * (size_t)nv * sizeof(double));
+    }
+    free(g_mesh.v);
+    free(g_mesh.n);
+    free(g_mesh.f);
+    g_mesh.nv = nv;
+    g_mesh.nf = nf;
+    g_mesh.v = V;
+    g_mesh.n = Nn;
+    g_mesh.f = faces;
+    free(f); free(verts); free(tcount); free(tri); free(rid); free(vx); free(tv); free(nm);
+    free(cnt); free(off); free(nb); free(CV); free(CN);
+    return nv;
+}
+
+int orc_mesh_faces(void) { return g_mesh.nf; }
+
+void orc_mesh_get(double *v, double *n, int32_t *f)
+{
+    if (v) memcpy(v, g_mesh.v, 3 * (size_t)g_mesh.nv * sizeof(double));
+    if (n) memcpy(n, g_mesh.n, 3 * (size_t)g_mesh.nv * sizeof(double));
+    if (f) memcpy(f, g_mesh.f, 3 * (size_t)g_mesh.nf * sizeof(int32_t));
+}

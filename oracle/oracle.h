@@ -144,6 +144,12 @@ void orc_set_emitter(orc_sim *s, real x, real z, real r, int pour_time);
 void orc_set_step(orc_sim *s, int step);
 int orc_get_step(const orc_sim *s);
 int orc_pour(orc_sim *s);
+/* mesher::generateMesh on the mg^3 lattice through the current buckets;
+ * returns the vertex count, keeps the mesh for orc_mesh_get (one mesh per
+ * process: the reference's mesher keeps static state too) */
+int orc_mesh(orc_sim *s, int mg);
+int orc_mesh_faces(void);
+void orc_mesh_get(double *v, double *n, int32_t *f);
 int orc_scene_benchmark(orc_sim *s, uint64_t seed);
 void orc_compute_wall_normals(orc_sim *s);
 

@@ -107,6 +107,8 @@ def _lib(f64):
     lib.orc_set_step.argtypes = [P, C.c_int]
     lib.orc_get_step.argtypes = [P]
     lib.orc_pour.argtypes = [P]
+    lib.orc_mesh.argtypes = [P, C.c_int]
+    lib.orc_mesh_get.argtypes = [P, P, P]
     return lib
 
 
@@ -251,6 +253,19 @@ class Sim:
     def set_emitter(self, x, z, r, pour_time):
         self.L.orc_set_emitter(self.h, x, z, r, pour_time)
 
+    def mesh(self, mg=None):
+        """mesher::generateMesh through the current buckets (mg = N + 1 as
+        write_mesh uses): (vertices, normals) as float64 (nv, 3), faces as
+        int32 (nf, 3)."""
+        mg = self.n + 1 if mg is None else mg
+        nv = self.L.orc_mesh(self.h, mg)
+        nf = self.L.orc_mesh_faces()
+        v = np.empty((nv, 3), np.float64)
+        nr = np.empty((nv, 3), np.float64)
+        f = np.empty((nf, 3), np.int32)
+        self.L.orc_mesh_get(_ptr(v), _ptr(nr), _ptr(f))
+        return v, nr, f
+
     @property
     def step_count(self):
         return self.L.orc_get_step(self.h)
@@ -361,5 +376,7 @@ def ref_lib():
     lib.ref_kat_length.argtypes = [C.c_int, P, P, P, P]
     lib.ref_save_state.argtypes = [C.c_int]
     lib.ref_init_test.argtypes = [C.c_int, C.c_uint]
+    lib.ref_mesh.argtypes = [C.c_int]
+    lib.ref_mesh_get.argtypes = [P, P, P]
     lib.ref_set_step.argtypes = [C.c_int]
     return lib

@@ -152,6 +152,22 @@ int ref_init_test(int which, unsigned seed)
     return (int)particles.size();
 }
 
+// mesher::generateMesh as write_mesh calls it (flip3D.cpp:1002), through the
+// sorter's current buckets; mg stays fixed per process (the mesher and
+// CIsoSurface keep static arrays sized at their first call)
+int ref_mesh(int mg)
+{
+    mesher::generateMesh(A, sort, particles, DENSITY, mg, vertices, normals, faces);
+    return (int)(vertices.size() / 3);
+}
+int ref_mesh_faces() { return (int)(faces.size() / 3); }
+void ref_mesh_get(double *v, double *nr, int32_t *f)
+{
+    for (size_t i = 0; i < vertices.size(); i++) v[i] = vertices[i];
+    for (size_t i = 0; i < normals.size(); i++) nr[i] = normals[i];
+    for (size_t i = 0; i < faces.size(); i++) f[i] = faces[i];
+}
+
 // simulateStep's step counter and pourWater (flip3D.cpp:1196-1199)
 void ref_set_step(int s) { step = s; }
 int ref_get_step() { return step; }

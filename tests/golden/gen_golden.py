@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 SECTIONS = ["kernels", "sort", "p2g", "g2p", "correct", "solve", "scene_ref", "traj_ref", "traj_bench",
-            "checkpoint", "scenes_test"]
+            "checkpoint", "scenes_test", "mesh"]
 
 
 def digest(a):
@@ -320,6 +320,25 @@ def sec_scenes_test(out):
                             f"{name}_full{s}_press": g["press"], f"{name}_full{s}_A": g["A"]})
         rec[f"{name}_steps"] = np.int32(steps)
     np.savez_compressed(out, **rec)
+
+
+def sec_mesh(out):
+    """mesher::generateMesh (mg = N + 1, as write_mesh calls it) after 5 steps
+    of the reference dam-break scene: smoothed vertices and normals, faces."""
+    R = ref()
+    with Quiet():
+        R.ref_init()
+    R.ref_srand(1)
+    with Quiet():
+        for _ in range(5):
+            R.ref_step()
+        nv = R.ref_mesh(R.ref_grid_n() + 1)
+    nf = R.ref_mesh_faces()
+    v = np.empty((nv, 3), np.float64)
+    nr = np.empty((nv, 3), np.float64)
+    f = np.empty((nf, 3), np.int32)
+    R.ref_mesh_get(P(v), P(nr), P(f))
+    np.savez_compressed(out, v=v, n=nr, f=f)
 
 
 def sec_checkpoint(out):

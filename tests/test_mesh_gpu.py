@@ -1,0 +1,73 @@
+"""The GPU surface mesher against the oracle's restatement of
+mesher::generateMesh (itself bit-exact against the reference's mesher,
+tests/test_oracle_golden.py): vertices, normals and faces bit for bit, on
+identical particle states and buckets."""
+import numpy as np
+import pytest
+
+import flip3d_amd as fa
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_mesh(a, b):
+    (va, na, fa_), (vb, nb, fb) = a, b
+    assert va.shape == vb.shape and fa_.shape == fb.shape
+    assert np.array_equal(fa_, fb)
+    assert np.array_equal(va.view(np.uint64), vb.view(np.uint64))
+    assert np.array_equal(na.view(np.uint64), nb.view(np.uint64))
+
+
+def test_mesh_after_corrector_buckets():
+    """The state write_mesh sees: the corrector's buckets with the corrected
+    positions.  The oracle runs 4 steps and step 5 up to the corrector; the
+    GPU takes that state and runs the corrector itself."""
+    s = po.Sim(32)
+    s.scene_reference()
+    s.srand(1)
+    for _ in range(4):
+        s.step()
+    s.call("sort")
+    s.call("compute_density")
+    s.call("add_ext_force")
+    s.call_stats("solve_picflip")
+    s.call_stats("advect")
+    q = s.particles()
+    with fa.FlipContext(32) as g:
+        g.upload(q["p"], q["u"], q["n"], q["type"], dens=q["dens"])
+        g.max_dens = s.max_dens
+        g.srand(7)
+        s.srand(7)
+        g.stage(fa.STAGE_CORRECT)
+        s.call_stats("correct")
+        assert np.array_equal(g.download()["p"], s.particles()["p"])
+        mesh_g = g.mesh()
+        mesh_s = s.mesh()
+        assert len(mesh_s[2]) > 1000
+        _same_mesh(mesh_g, mesh_s)
+    s.close()
+
+
+@pytest.mark.parametrize("scene", [fa.SCENE_BENCHMARK, fa.SCENE_SPHERE_POUR])
+def test_mesh_of_fresh_buckets(scene):
+    """A GPU trajectory's state meshed by both after sorting the current
+    positions; N=48, a lattice that is not a power of two."""
+    with fa.FlipContext(48, tol_policy=1, tol=1e-5) as g:
+        g.build_scene(scene)
+        for _ in range(3):
+            g.step()
+        q = g.download()
+        s = po.Sim(48)
+        s.set_particles(q["p"], q["u"], q["n"], q["type"], dens=q["dens"])
+        s.call("sort")
+        g.upload(q["p"], q["u"], q["n"], q["type"], dens=q["dens"])
+        v, n, f = g.mesh()
+        _same_mesh((v, n, f), s.mesh())
+        assert f.min() >= 0 and f.max() < len(v)
+        # smoothed normals are unit vectors, or zero where a vertex never had
+        # a usable sample (smoothMesh keeps the zero-initialised copy)
+        norms = np.linalg.norm(n[np.isfinite(n).all(axis=1)], axis=1)
+        assert np.all((np.abs(norms - 1.0) < 1e-6) | (norms == 0.0))
+        assert np.mean(np.abs(norms - 1.0) < 1e-6) > 0.99
+        s.close()

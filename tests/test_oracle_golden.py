@@ -240,3 +240,34 @@ def test_pour_trajectories(which, name):
                 digest(s.grid(po.G_A)), digest(s.grid(po.G_L))]
         bad = [nm for nm, a, b in zip(["p", "u", "dens", "press", "A", "L"], mine, g[f"{name}_dig{k}"]) if a != b]
         assert not bad, f"{name} step {k}: mismatch in {bad} ({st})"
+
+
+def test_mesh_after_five_steps():
+    """mesher::generateMesh (implicit field, marching cubes, edge filter, two
+    smoothing passes) on the dam-break scene after 5 steps, through the
+    buckets of the corrector's sort: vertices, normals and faces bit-exact."""
+    g = gold("mesh")
+    s = po.Sim(32)
+    s.scene_reference()
+    s.srand(1)
+    for _ in range(5):
+        s.step()
+    v, n, f = s.mesh()
+    assert np.array_equal(f, g["f"])
+    assert np.array_equal(v.view(np.uint64), g["v"].view(np.uint64))
+    assert np.array_equal(n.view(np.uint64), g["n"].view(np.uint64))
+
+
+def test_mc_table_edges_follow_the_corners():
+    """Every edge a case's triangles use is cut (its corners lie on opposite
+    sides of the iso level), and every cut edge is used."""
+    import re
+    src = open(os.path.join(os.path.dirname(GOLD), "..", "flip3d_amd", "csrc", "mc_table.h")).read()
+    cases = re.findall(r'"([0-9a-b]*)"', src[src.index("MC_CASE_LIST"):src.index("MC_EDGE_CORNERS")])
+    assert len(cases) == 256
+    corners = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+    for c, tri in enumerate(cases):
+        assert len(tri) % 3 == 0
+        used = {int(ch, 16) for ch in tri}
+        cut = {e for e, (a, b) in enumerate(corners) if ((c >> a) ^ (c >> b)) & 1}
+        assert used == cut, c
