@@ -31,6 +31,8 @@ EXPORTS = [
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
     "flip_gpu_save_state", "flip_gpu_load_state", "flip_gpu_step_many",
     "flip_gpu_set_emitter", "flip_gpu_set_step", "flip_gpu_get_step", "flip_gpu_mesh", "flip_gpu_mesh_get",
+    "flip_gpu_render_preview", "flip_gpu_write_preview", "flip_gpu_write_obj", "flip_gpu_write_sunflow",
+    "flip_gpu_step_begin",
     "flip_gpu_set_learned_weights", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
     "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
     "flip_unet_predict", "flip_unet_save", "flip_unet_load",
@@ -88,6 +90,11 @@ def lib():
         L.flip_gpu_set_step.argtypes = [P, C.c_int]
         L.flip_gpu_mesh.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.flip_gpu_mesh_get.argtypes = [P, P, P, P]
+        L.flip_gpu_render_preview.argtypes = [P, P]
+        L.flip_gpu_write_preview.argtypes = [P, C.c_char_p]
+        L.flip_gpu_write_obj.argtypes = [P, C.c_char_p]
+        L.flip_gpu_write_sunflow.argtypes = [P, C.c_char_p, C.c_char_p]
+        L.flip_gpu_step_begin.argtypes = [P]
         L.flip_gpu_get_step.argtypes = [P, C.POINTER(C.c_int)]
         L.flip_gpu_set_max_dens.argtypes = [P, C.c_float]
         L.flip_gpu_get_max_dens.argtypes = [P, C.POINTER(C.c_float)]
@@ -233,6 +240,25 @@ class FlipContext:
         f = np.empty((nf.value, 3), np.int32)
         _check(lib().flip_gpu_mesh_get(self.h, _ptr(v), _ptr(nr), _ptr(f)))
         return v, nr, f
+
+    def render_preview(self):
+        """write_frame's preview image: uint8 (700, 1024, 4) RGBA."""
+        img = np.empty((700, 1024, 4), np.uint8)
+        _check(lib().flip_gpu_render_preview(self.h, _ptr(img)))
+        return img
+
+    def write_preview(self, path):
+        _check(lib().flip_gpu_write_preview(self.h, os.fsencode(path)))
+
+    def write_obj(self, path):
+        _check(lib().flip_gpu_write_obj(self.h, os.fsencode(path)))
+
+    def write_sunflow(self, path, template_path=None):
+        _check(lib().flip_gpu_write_sunflow(self.h, os.fsencode(path),
+                                            None if template_path is None else os.fsencode(template_path)))
+
+    def step_begin(self):
+        _check(lib().flip_gpu_step_begin(self.h))
 
     @property
     def step_count(self):

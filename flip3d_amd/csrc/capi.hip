@@ -788,6 +788,92 @@ int flip_gpu_stage(flip_ctx *ctx, int stage, flip_step_stats *st)
     });
 }
 
+// flip3D.cpp:1196-1199: the step counter, then pourWater
+static void begin_step(FlipCtx *c)
+{
+    c->step++;
+    if (c->step <= c->pour_time && c->emit_n > 0) {
+        ensure_particle_capacity(c, c->np + c->emit_n);
+        k_launch_emit(c);
+    }
+    c->begun = true;
+}
+
+int flip_gpu_step_begin(flip_ctx *ctx)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        require(!c->begun, "flip_gpu_step_begin twice before a step", FLIP_E_STATE);
+        begin_step(c);
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_render_preview(flip_ctx *ctx, uint8_t *rgba)
+{
+    return guarded([&]() -> int {
+        require(ctx && rgba, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        preview_render(c, rgba);
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_write_preview(flip_ctx *ctx, const char *path)
+{
+    return guarded([&]() -> int {
+        require(ctx && path, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+        std::vector<uint8_t> img(4 * (size_t)preview_width() * preview_height());
+        preview_render(c, img.data());
+        require(write_bmp_file(path, img.data(), preview_width(), preview_height()), "cannot write the preview file");
+        return FLIP_OK;
+    });
+}
+
+static void mesh_host(FlipCtx *c, std::vector<double> &v, std::vector<double> &n, std::vector<int32_t> &f)
+{
+    require(c->mesh != nullptr, "no mesh built (flip_gpu_mesh)", FLIP_E_STATE);
+    long nv = 0, nf = 0;
+    mesh_counts(c, &nv, &nf);
+    v.resize(3 * nv);
+    n.resize(3 * nv);
+    f.resize(3 * nf);
+    mesh_copy_out(c, v.data(), n.data(), f.data());
+}
+
+int flip_gpu_write_obj(flip_ctx *ctx, const char *path)
+{
+    return guarded([&]() -> int {
+        require(ctx && path, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        std::vector<double> v, n;
+        std::vector<int32_t> f;
+        mesh_host(c, v, n, f);
+        require(write_obj_file(path, v, n, f, (float)(1.0 / c->n)), "cannot write the OBJ file");
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_write_sunflow(flip_ctx *ctx, const char *path, const char *template_path)
+{
+    return guarded([&]() -> int {
+        require(ctx && path, "null argument");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        std::vector<double> v, n;
+        std::vector<int32_t> f;
+        mesh_host(c, v, n, f);
+        if (template_path) require(write_sunflow_template(template_path), "cannot write the Sunflow template");
+        require(write_sunflow_file(path, v, n, f, (float)(1.0 / c->n)), "cannot write the Sunflow scene file");
+        return FLIP_OK;
+    });
+}
+
 int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
 {
     return guarded([&]() -> int {
@@ -802,11 +888,9 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
             FLIP_CHECK(hipEventCreate(&c->ev_a));
             FLIP_CHECK(hipEventCreate(&c->ev_b));
         }
-        // flip3D.cpp:1196-1199: the step counter, then pourWater
-        c->step++;
-        if (c->step <= c->pour_time && c->emit_n > 0) ensure_particle_capacity(c, c->np + c->emit_n);
+        if (!c->begun) begin_step(c);
+        c->begun = false;
         FLIP_CHECK(hipEventRecord(c->ev_a, c->stream));
-        if (c->step <= c->pour_time) k_launch_emit(c);
         choose_dt(c);
         st->dt = c->dt;
         // flip3D.cpp:1221-1222 sort + computeDensity

@@ -119,6 +119,7 @@ struct FlipCtx {
     bool uploaded = false;
     bool sorted = false;  // storage order matches the current positions
     bool bucketed = false;  // cell_start describes the storage (the last sort), maybe of older positions
+    bool begun = false;     // flip_gpu_step_begin ran for the coming step
     MeshState *mesh = nullptr;
     bool unit_mass = false;  // every particle has m == 1 (the reference's only mass)
 
@@ -156,6 +157,17 @@ void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t 
 void mesh_build(FlipCtx *c, int mg, long *nv, long *nf);
 void mesh_copy_out(FlipCtx *c, double *v, double *nrm, int32_t *faces);
 void mesh_free(MeshState *m);
+void mesh_counts(const FlipCtx *c, long *nv, long *nf);
+// output files (export.hip)
+void preview_render(FlipCtx *c, uint8_t *rgba_host);  // write_frame's RGBA image
+int preview_width();
+int preview_height();
+bool write_bmp_file(const char *path, const uint8_t *rgba, int width, int height);
+bool write_obj_file(const char *path, const std::vector<double> &v, const std::vector<double> &n,
+                    const std::vector<int32_t> &f, float wall_thick);
+bool write_sunflow_file(const char *path, const std::vector<double> &v, const std::vector<double> &n,
+                        const std::vector<int32_t> &f, float wall_thick);
+bool write_sunflow_template(const char *path);
 int k_build_fluid_list(FlipCtx *c);             // returns nfl (host sync)
 int k_solve(FlipCtx *c, flip_step_stats *st);   // PCG on (A, L, press, div)
 // FLIP_PRECOND_LEARNED: flexible CG from the current press and r (after the

@@ -450,6 +450,12 @@ void mesh_build(FlipCtx *c, int mg, long *nv_out, long *nf_out)
     *nf_out = nf;
 }
 
+void mesh_counts(const FlipCtx *c, long *nv, long *nf)
+{
+    *nv = c->mesh ? c->mesh->nv : 0;
+    *nf = c->mesh ? c->mesh->nf : 0;
+}
+
 void mesh_copy_out(FlipCtx *c, double *v, double *nrm, int32_t *faces)
 {
     MeshState &m = *c->mesh;

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -28,6 +29,7 @@ struct Shim {
     flip_ctx *sim = nullptr;         // flip3D::init / simulateStep state
     int sim_n = 32;                  // the reference's #define N (flip3D.cpp:25)
     int sim_scene = FLIP_SCENE_REFERENCE;  // placeObjects' damBreakTest (flip3D.cpp:365-373)
+    bool out_preview = false, out_mesh = false;  // write_frame / write_mesh files per step
     char subcell = 0;                // solver.cpp:13 default
     ~Shim() { release(); }
     void release()
@@ -234,13 +236,45 @@ void flip3D::simulateStep()
 {
     Shim &s = shim();
     if (!s.sim) throw std::runtime_error("flip3D::simulateStep before flip3D::init");
+    if (!s.out_preview && !s.out_mesh) {
+        check(flip_gpu_step(s.sim, nullptr), "flip_gpu_step");
+        return;
+    }
+    // flip3D.cpp:1194-1257: counter and pour, the preview frame, the step,
+    // then the mesh files, under render/ as the reference names them
+    check(flip_gpu_step_begin(s.sim), "flip_gpu_step_begin");
+    int step = 0;
+    check(flip_gpu_get_step(s.sim, &step), "flip_gpu_get_step");
+    char path[256], tmpl[256];
+    if (s.out_preview) {
+        std::filesystem::create_directories("render/preview");
+        std::snprintf(path, sizeof(path), "render/preview/frame_%d.bmp", step);
+        check(flip_gpu_write_preview(s.sim, path), "flip_gpu_write_preview");
+    }
     check(flip_gpu_step(s.sim, nullptr), "flip_gpu_step");
+    if (s.out_mesh) {
+        int64_t nv = 0, nf = 0;
+        check(flip_gpu_mesh(s.sim, 0, &nv, &nf), "flip_gpu_mesh");
+        std::filesystem::create_directories("render/obj");
+        std::filesystem::create_directories("render/sunflow");
+        std::snprintf(path, sizeof(path), "render/obj/%d_scene.obj", step);
+        check(flip_gpu_write_obj(s.sim, path), "flip_gpu_write_obj");
+        std::snprintf(path, sizeof(path), "render/sunflow/%d_scene.sc", step);
+        std::snprintf(tmpl, sizeof(tmpl), "render/sunflow/template.sc");
+        check(flip_gpu_write_sunflow(s.sim, path, tmpl), "flip_gpu_write_sunflow");
+    }
 }
 
 // ------------------------------------------------------------- shim extras
 void flip3d_shim::set_grid_size(int n) { shim().sim_n = n; }
 
 void flip3d_shim::set_scene(int scene) { shim().sim_scene = scene; }
+
+void flip3d_shim::set_outputs(bool preview, bool mesh)
+{
+    shim().out_preview = preview;
+    shim().out_mesh = mesh;
+}
 
 flip_ctx *flip3d_shim::context() { return shim().sim; }
 

@@ -145,6 +145,25 @@ int flip_gpu_set_emitter(flip_ctx *ctx, float x, float z, float r, int pour_time
  * pointer may be NULL). */
 int flip_gpu_mesh(flip_ctx *ctx, int mg, int64_t *nvert, int64_t *nface);
 int flip_gpu_mesh_get(flip_ctx *ctx, double *vertices, double *normals, int32_t *faces);
+
+/* The reference's per-frame output files, byte for byte.
+ * flip_gpu_render_preview: write_frame's point-splat preview
+ *   (src/flip3D.cpp:1105-1192) of the current FLUID particles, rendered on the
+ *   GPU, as 1024 x 700 RGBA bytes (rows in write_frame's order);
+ * flip_gpu_write_preview: the same written by write_bmp (write_bmp.cpp:52-118);
+ * flip_gpu_write_obj / flip_gpu_write_sunflow: the last flip_gpu_mesh as
+ *   write_obj (exporter.cpp:127-164) / writeSunflow (exporter.cpp:72-125) write
+ *   it, with wall thickness 1/N; template_path (may be NULL) also receives the
+ *   template.sc the scene file includes.
+ * flip_gpu_step_begin: the first part of simulateStep (flip3D.cpp:1196-1199:
+ *   the step counter and pourWater) on its own, so that a caller can write the
+ *   preview where the reference does, before the rest of the step; the next
+ *   flip_gpu_step then skips that part. */
+int flip_gpu_render_preview(flip_ctx *ctx, uint8_t *rgba);
+int flip_gpu_write_preview(flip_ctx *ctx, const char *path);
+int flip_gpu_write_obj(flip_ctx *ctx, const char *path);
+int flip_gpu_write_sunflow(flip_ctx *ctx, const char *path, const char *template_path);
+int flip_gpu_step_begin(flip_ctx *ctx);
 int flip_gpu_set_step(flip_ctx *ctx, int step);
 int flip_gpu_get_step(flip_ctx *ctx, int *step);
 int flip_gpu_set_max_dens(flip_ctx *ctx, float max_dens); /* flip3D.cpp:58 */

@@ -83,6 +83,11 @@ void set_grid_size(int n);
 // default, as placeObjects has it), FLIP_SCENE_SPHERE_POUR, _CLIFF_POUR or
 // _WATER_DROP (the scene functions placeObjects leaves commented out).
 void set_scene(int scene);
+// Per-step output files as simulateStep writes them (off by default): the
+// preview render/preview/frame_<step>.bmp (write_frame, flip3D.cpp:1133) and
+// the mesh files render/obj/<step>_scene.obj, render/sunflow/<step>_scene.sc
+// and template.sc (write_mesh, flip3D.cpp:999, WRITE_FILE).
+void set_outputs(bool preview, bool mesh);
 // The context flip3D::init created, for callers that render or export.
 flip_ctx *context();
 // Copies the GPU-resident particles out in reference record layout.

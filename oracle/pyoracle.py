@@ -377,6 +377,7 @@ def ref_lib():
     lib.ref_save_state.argtypes = [C.c_int]
     lib.ref_init_test.argtypes = [C.c_int, C.c_uint]
     lib.ref_mesh.argtypes = [C.c_int]
+    lib.ref_write_outputs.argtypes = [C.c_int]
     lib.ref_mesh_get.argtypes = [P, P, P]
     lib.ref_set_step.argtypes = [C.c_int]
     return lib

@@ -168,6 +168,17 @@ void ref_mesh_get(double *v, double *nr, int32_t *f)
     for (size_t i = 0; i < faces.size(); i++) f[i] = faces[i];
 }
 
+// The reference's per-step output files as simulateStep writes them, for
+// frame `at`: the preview BMP (write_frame, flip3D.cpp:1133-1192, into
+// render/preview/frame_<at>.bmp) and the mesh files (write_mesh, :999-1009:
+// render/obj/<at>_scene.obj, render/sunflow/<at>_scene.sc and template.sc)
+void ref_write_outputs(int at)
+{
+    step = at;
+    write_frame();
+    write_mesh();
+}
+
 // simulateStep's step counter and pourWater (flip3D.cpp:1196-1199)
 void ref_set_step(int s) { step = s; }
 int ref_get_step() { return step; }

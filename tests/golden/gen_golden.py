@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 SECTIONS = ["kernels", "sort", "p2g", "g2p", "correct", "solve", "scene_ref", "traj_ref", "traj_bench",
-            "checkpoint", "scenes_test", "mesh"]
+            "checkpoint", "scenes_test", "mesh", "exports"]
 
 
 def digest(a):
@@ -339,6 +339,33 @@ def sec_mesh(out):
     f = np.empty((nf, 3), np.int32)
     R.ref_mesh_get(P(v), P(nr), P(f))
     np.savez_compressed(out, v=v, n=nr, f=f)
+
+
+def sec_exports(out):
+    """The files simulateStep writes for frame 6 of the dam-break scene after
+    5 steps (preview BMP, OBJ, Sunflow scene and template): sizes and
+    SHA-256 digests, plus the BMP header and the first OBJ lines as text."""
+    R = ref()
+    with Quiet():
+        R.ref_init()
+    R.ref_srand(1)
+    with Quiet():
+        for _ in range(5):
+            R.ref_step()
+    os.makedirs(os.path.join("render", "preview"), exist_ok=True)
+    os.makedirs(os.path.join("render", "obj"), exist_ok=True)
+    os.makedirs(os.path.join("render", "sunflow"), exist_ok=True)
+    with Quiet():
+        R.ref_write_outputs(6)
+    rec = {}
+    for key, path in (("bmp", "render/preview/frame_6.bmp"), ("obj", "render/obj/6_scene.obj"),
+                      ("sc", "render/sunflow/6_scene.sc"), ("template", "render/sunflow/template.sc")):
+        raw = open(path, "rb").read()
+        rec[f"{key}_size"] = np.int64(len(raw))
+        rec[f"{key}_sha"] = np.array(hashlib.sha256(raw).hexdigest())
+    rec["bmp_header"] = np.frombuffer(open("render/preview/frame_6.bmp", "rb").read()[:54], np.uint8)
+    rec["obj_head"] = np.array(open("render/obj/6_scene.obj").read()[:400])
+    np.savez_compressed(out, **rec)
 
 
 def sec_checkpoint(out):

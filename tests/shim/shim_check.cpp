@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "flip3d_shim.h"
@@ -226,12 +227,74 @@ static void check_step_and_checkpoint()
     std::printf("step and checkpoint ok\n");
 }
 
+static std::string slurp(const char *path)
+{
+    std::string out;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return out;
+    char buf[65536];
+    size_t k;
+    while ((k = std::fread(buf, 1, sizeof(buf), f)) > 0) out.append(buf, k);
+    std::fclose(f);
+    return out;
+}
+
+// flip3d_shim::set_outputs: simulateStep writes the reference's per-step files
+// (preview at the start of the step, after the pour; mesh files at the end),
+// equal to what the C ABI writes for the same steps
+static void check_outputs()
+{
+    const int n = 16;
+    flip3d_shim::set_grid_size(n);
+    flip3d_shim::set_scene(FLIP_SCENE_SPHERE_POUR);
+    flip3d_shim::set_outputs(true, true);
+    flip3D::init();
+    flip_params p;
+    ok(flip_gpu_default_params(&p, n), "default_params");
+    flip_ctx *c = nullptr;
+    ok(flip_gpu_create(&p, &c), "create");
+    int64_t np = 0;
+    ok(flip_gpu_build_scene(c, FLIP_SCENE_SPHERE_POUR, 0, &np), "build_scene");
+    mkdir("abi", 0755);
+    for (int s = 1; s <= 2; ++s) {
+        flip3D::simulateStep();
+        ok(flip_gpu_step_begin(c), "step_begin");
+        char path[64];
+        std::snprintf(path, sizeof(path), "abi/frame_%d.bmp", s);
+        ok(flip_gpu_write_preview(c, path), "write_preview");
+        ok(flip_gpu_step(c, nullptr), "step");
+        int64_t nv = 0, nf = 0;
+        ok(flip_gpu_mesh(c, 0, &nv, &nf), "mesh");
+        std::snprintf(path, sizeof(path), "abi/%d_scene.obj", s);
+        ok(flip_gpu_write_obj(c, path), "write_obj");
+        std::snprintf(path, sizeof(path), "abi/%d_scene.sc", s);
+        ok(flip_gpu_write_sunflow(c, path, "abi/template.sc"), "write_sunflow");
+    }
+    int64_t np2 = 0;
+    ok(flip_gpu_num_particles(flip3d_shim::context(), &np2), "num_particles");
+    EXPECT(np2 > np, "the pour scene poured through simulateStep");
+    const char *pairs[][2] = {{"render/preview/frame_1.bmp", "abi/frame_1.bmp"},
+                              {"render/preview/frame_2.bmp", "abi/frame_2.bmp"},
+                              {"render/obj/2_scene.obj", "abi/2_scene.obj"},
+                              {"render/sunflow/2_scene.sc", "abi/2_scene.sc"},
+                              {"render/sunflow/template.sc", "abi/template.sc"}};
+    for (auto &pr : pairs) {
+        std::string a = slurp(pr[0]), b = slurp(pr[1]);
+        EXPECT(!a.empty() && a == b, pr[0]);
+    }
+    flip3d_shim::set_outputs(false, false);
+    flip3d_shim::set_scene(FLIP_SCENE_REFERENCE);
+    flip_gpu_destroy(c);
+    std::printf("outputs ok\n");
+}
+
 int main()
 {
     try {
         check_solver();
         check_mapper();
         check_step_and_checkpoint();
+        check_outputs();
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());
         return 2;

@@ -271,3 +271,24 @@ def test_mc_table_edges_follow_the_corners():
         used = {int(ch, 16) for ch in tri}
         cut = {e for e, (a, b) in enumerate(corners) if ((c >> a) ^ (c >> b)) & 1}
         assert used == cut, c
+
+
+def test_output_files_after_five_steps():
+    """write_obj, writeSunflow and write_frame + write_bmp restated
+    (oracle/exporters.py): the files of frame 6 of the dam-break scene match
+    the reference's byte for byte."""
+    from oracle import exporters as ex
+    g = gold("exports")
+    s = po.Sim(32)
+    s.scene_reference()
+    s.srand(1)
+    for _ in range(5):
+        s.step()
+    v, n, f = s.mesh()
+    q = s.particles()
+    files = {"obj": ex.obj_bytes(v, n, f, 32), "sc": ex.sunflow_bytes(v, n, f, 32),
+             "bmp": ex.bmp_bytes(ex.render_preview(q["p"], q["type"], 32))}
+    for key, raw in files.items():
+        assert len(raw) == int(g[f"{key}_size"]), key
+        assert hashlib.sha256(raw).hexdigest() == str(g[f"{key}_sha"]), key
+    assert files["bmp"][:54] == g["bmp_header"].tobytes()
