@@ -88,6 +88,54 @@ def kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid):
     return per.get(name)
 
 
+def mesh_leg(fa, n, weights, cpu=True):
+    """The surface mesher and the preview on the state after 3 steps of the
+    same scene (outside every timed step): device mesh time (median of 5
+    builds, counts read back), its kernels from one profiled build, the
+    preview render, and the oracle's mesher on the same particle state
+    (bucketed by the current positions) on this host's cores."""
+    ctx = fa.FlipContext(n, tol_policy=1, tol=TOL, dt_policy=1, precond=fa.PRECOND_LEARNED)
+    ctx.set_learned_weights(weights)
+    ctx.build_scene(fa.SCENE_BENCHMARK)
+    ctx.srand(1)
+    for _ in range(3):
+        ctx.step()
+    ctx.synchronize()
+    ms = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        v, nr, f = ctx.mesh()
+        ms.append(1e3 * (time.perf_counter() - t0))
+    ctx.set_profiling(True)
+    ctx.mesh()
+    kern = {}
+    for name, t in ctx.kernel_times():
+        kern[name] = round(kern.get(name, 0.0) + t, 4)
+    ctx.set_profiling(False)
+    pv = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ctx.render_preview()
+        pv.append(1e3 * (time.perf_counter() - t0))
+    q = ctx.download()
+    ctx.close()
+    out = {"n": n, "lattice": n + 1, "vertices": len(v), "faces": len(f),
+           "gpu_ms_median": round(sorted(ms)[2], 3), "kernels_ms": kern,
+           "preview_ms_median": round(sorted(pv)[2], 3)}
+    if cpu:
+        from oracle import pyoracle as po
+        s = po.Sim(n)
+        s.set_particles(q["p"], q["u"], q["n"], q["type"], dens=q["dens"])
+        s.call("sort")
+        t0 = time.perf_counter()
+        vo, _, fo = s.mesh()
+        out["cpu_oracle_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+        out["cpu_threads"] = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        out["cpu_sample"] = f"oracle mesher on the same particles, {len(vo)} vertices, {len(fo)} faces"
+        s.close()
+    return out
+
+
 def cpu_baseline(n, seconds_budget):
     """The oracle (C restatement of the reference path, OpenMP where the
     reference uses it) on this host: the first step of the same scene."""
@@ -181,6 +229,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile-step", action="store_true",
                     help="skip the per-launch timed step (for runs under an external profiler)")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the mesher / preview leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -205,6 +254,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n, 30)
+    mesh = None if args.no_mesh else mesh_leg(fa, n, args.weights, cpu=not args.no_cpu_baseline)
 
     value = main_run["steps_per_s"]
     out = {
@@ -223,6 +273,7 @@ def main():
         "plain": summary(plain),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mesh": mesh,
         "profile_step_ms": {k: round(v[0], 4) for k, v in sorted(main_run["agg"].items(), key=lambda kv: -kv[1][0])},
     }
     print(json.dumps(out))

@@ -655,7 +655,9 @@ int flip_gpu_mesh(flip_ctx *ctx, int mg, int64_t *nvert, int64_t *nface)
         if (mg <= 0) mg = c->n + 1;
         require(mg >= 2 && (int64_t)mg * mg * mg * 3 < (int64_t(1) << 31), "lattice size out of range");
         long nv = 0, nf = 0;
+        timing_reset(c);
         mesh_build(c, mg, &nv, &nf);
+        timing_collect(c);
         if (nvert) *nvert = nv;
         if (nface) *nface = nf;
         return FLIP_OK;

@@ -37,17 +37,48 @@ __constant__ char d_mc_cases[256][16] = {MC_CASE_LIST};
 
 __device__ __forceinline__ int hexval(char ch) { return ch <= '9' ? ch - '0' : ch - 'a' + 10; }
 
+// Bounding box of the current positions of each bucket's particles (the
+// buckets may be older than the positions: after a step they are the
+// corrector's, which moves particles afterwards).  Empty: lo = +inf.
+__global__ void k_mesh_cell_boxes(long C, const int32_t *__restrict__ cs, const float4 *__restrict__ pos,
+                                  float4 *lo, float4 *hi)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const float inf = __builtin_inff();
+    float4 l = make_float4(inf, inf, inf, 0.0f), h = make_float4(-inf, -inf, -inf, 0.0f);
+    for (int a = cs[c]; a < cs[c + 1]; ++a) {
+        const float4 q = pos[a];
+        l.x = fminf(l.x, q.x);
+        l.y = fminf(l.y, q.y);
+        l.z = fminf(l.z, q.z);
+        h.x = fmaxf(h.x, q.x);
+        h.y = fmaxf(h.y, q.y);
+        h.z = fmaxf(h.z, q.z);
+    }
+    lo[c] = l;
+    hi[c] = h;
+}
+
 // implicit.cpp:14-38 at lattice point (i, j, k) * 1/(mg-1): the distance to
 // the nearest FLUID particle of the 5^3-cell window, capped at 8 DENSITY/gn,
 // less DENSITY/gn; 4.5 DENSITY/gn when a WALL particle lies within
-// DENSITY/gn.  The exact distance (glibc hypotf twice) is formed only for
-// particles whose float squared distance is within 1e-4 of the current
-// minimum (or of the wall radius): the two agree to a few ulp, so every
-// particle that can decide the result is measured exactly, and the minimum
-// and the wall test do not depend on visiting order.  mesher.cpp:141-152
-// then clamps the boundary and stores -value with x fastest.
+// DENSITY/gn.  The result is a minimum plus an existence test, so the
+// window may be visited in any order and a cell may be skipped when the
+// box of its particles is provably no nearer than both the current minimum
+// and the wall radius:
+//   - the 2x2x2 cells nearest the point go first, to find a small minimum
+//     early;
+//   - the other cells follow, pruned by their particle boxes (squared box
+//     distance above the bound by a 1e-4 margin, which covers the few ulp
+//     between float and exact distances);
+//   - inside a cell the exact distance (glibc hypotf twice) is formed only
+//     when the float squared distance is within 1e-4 of the minimum (or of
+//     the wall radius).
+// mesher.cpp:141-152 then clamps the boundary and stores -value, x fastest.
 __global__ __launch_bounds__(TB) void k_mesh_field(int mg, int n, const float4 *__restrict__ pos,
-                                                   const int32_t *__restrict__ cs, float density, float *f)
+                                                   const int32_t *__restrict__ cs, const float4 *__restrict__ blo,
+                                                   const float4 *__restrict__ bhi, float density, float *f)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long m3 = (long)mg * mg * mg;
@@ -57,32 +88,57 @@ __global__ __launch_bounds__(TB) void k_mesh_field(int mg, int n, const float4 *
     const double h = 1.0 / (double)(mg - 1);
     const float px = (float)(i * h), py = (float)(j * h), pz = (float)(k * h);
     const int ci = cell_of_b(n, px), cj = cell_of_b(n, py), ck = cell_of_b(n, pz);
+    const float gn = (float)n;
     const float dw = density / (float)n;
     const double dw2 = (double)dw * (double)dw * (1.0 + 1e-4);
     double phi = 8.0 * (double)density / n;
     double phi2 = phi * phi * (1.0 + 1e-4);
     bool wall = false;
-    const int klo = max(ck - 2, 0), khi = min(ck + 2, n - 1);
+    auto scan = [&](long c0, long c1) {
+        const int b = cs[c0], e = cs[c1 + 1];
+        for (int a = b; a < e; ++a) {
+            const float4 q = pos[a];
+            const double r2 = (double)length2(q.x, q.y, q.z, px, py, pz);
+            if (__builtin_signbit(q.w)) {
+                if (r2 < dw2 && length3(q.x, q.y, q.z, px, py, pz) < dw) {
+                    wall = true;
+                    return;
+                }
+                continue;
+            }
+            if (r2 < phi2) {
+                const double d = (double)length3(q.x, q.y, q.z, px, py, pz);
+                if (d < phi) {
+                    phi = d;
+                    phi2 = d * d * (1.0 + 1e-4);
+                }
+            }
+        }
+    };
+    auto in = [&](int s) { return s >= 0 && s <= n - 1; };
+    // 1. the near cells: per axis the point's cell and its neighbour on the
+    // nearer side
+    const int ni = gn * px - (float)ci < 0.5f ? ci - 1 : ci + 1, nj = gn * py - (float)cj < 0.5f ? cj - 1 : cj + 1,
+              nk = gn * pz - (float)ck < 0.5f ? ck - 1 : ck + 1;
+    const int klo1 = max(min(ck, nk), 0), khi1 = min(max(ck, nk), n - 1);
+    for (int si = min(ci, ni); si <= max(ci, ni) && !wall; ++si)
+        for (int sj = min(cj, nj); sj <= max(cj, nj) && !wall; ++sj)
+            if (in(si) && in(sj)) scan(cidx(n, si, sj, klo1), cidx(n, si, sj, khi1));
+    // 2. the rest of the window
     for (int si = max(ci - 2, 0); si <= min(ci + 2, n - 1) && !wall; ++si)
         for (int sj = max(cj - 2, 0); sj <= min(cj + 2, n - 1) && !wall; ++sj) {
-            const int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for (int a = b; a < e; ++a) {
-                const float4 q = pos[a];
-                const double r2 = (double)length2(q.x, q.y, q.z, px, py, pz);
-                if (__builtin_signbit(q.w)) {
-                    if (r2 < dw2 && length3(q.x, q.y, q.z, px, py, pz) < dw) {
-                        wall = true;
-                        break;
-                    }
-                    continue;
-                }
-                if (r2 < phi2) {
-                    const double d = (double)length3(q.x, q.y, q.z, px, py, pz);
-                    if (d < phi) {
-                        phi = d;
-                        phi2 = d * d * (1.0 + 1e-4);
-                    }
-                }
+            const bool near_row = (si == ci || si == ni) && (sj == cj || sj == nj);
+            for (int sk = max(ck - 2, 0); sk <= min(ck + 2, n - 1) && !wall; ++sk) {
+                if (near_row && sk >= klo1 && sk <= khi1) continue;  // done in 1.
+                const long c = cidx(n, si, sj, sk);
+                const float4 l = blo[c], u = bhi[c];
+                // per-axis gaps, shortened by 1e-6 for their own rounding
+                const float gx = fmaxf(fmaxf(l.x - px, px - u.x) - 1e-6f, 0.0f),
+                            gy = fmaxf(fmaxf(l.y - py, py - u.y) - 1e-6f, 0.0f),
+                            gz = fmaxf(fmaxf(l.z - pz, pz - u.z) - 1e-6f, 0.0f);
+                const double d2 = ((double)gx * gx + (double)gy * gy + (double)gz * gz) * (1.0 - 1e-4);
+                if (!(d2 < phi2) && !(d2 < dw2)) continue;  // also skips empty cells (infinite gap)
+                scan(c, c);
             }
         }
     double value = wall ? 4.5 * (double)density / n : phi - (double)dw;
@@ -355,18 +411,19 @@ struct MeshState {
     int32_t *flag = nullptr, *vid = nullptr, *vedge = nullptr, *cnt = nullptr, *toff = nullptr, *keep = nullptr,
             *fidx = nullptr, *faces = nullptr, *blocks = nullptr;
     uint8_t *ccase = nullptr;
-    float4 *vpos = nullptr, *tn = nullptr;
+    float4 *vpos = nullptr, *tn = nullptr, *blo = nullptr, *bhi = nullptr;
     int4 *tri = nullptr;
     double *V = nullptr, *N = nullptr, *CV = nullptr, *CN = nullptr;
     size_t c_f = 0, c_flag = 0, c_vid = 0, c_vedge = 0, c_cnt = 0, c_toff = 0, c_keep = 0, c_fidx = 0, c_faces = 0,
-           c_blocks = 0, c_ccase = 0, c_vpos = 0, c_tn = 0, c_tri = 0, c_V = 0, c_N = 0, c_CV = 0, c_CN = 0;
+           c_blocks = 0, c_ccase = 0, c_vpos = 0, c_tn = 0, c_tri = 0, c_V = 0, c_N = 0, c_CV = 0, c_CN = 0, c_blo = 0,
+           c_bhi = 0;
 };
 
 void mesh_free(MeshState *m)
 {
     if (!m) return;
     void *ps[] = {m->f, m->flag, m->vid, m->vedge, m->cnt, m->toff, m->keep, m->fidx, m->faces, m->blocks,
-                  m->ccase, m->vpos, m->tn, m->tri, m->V, m->N, m->CV, m->CN};
+                  m->ccase, m->vpos, m->tn, m->tri, m->V, m->N, m->CV, m->CN, m->blo, m->bhi};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete m;
@@ -392,7 +449,10 @@ void mesh_build(FlipCtx *c, int mg, long *nv_out, long *nf_out)
     grow(&m.ccase, m.c_ccase, (size_t)ncell);
     grow(&m.cnt, m.c_cnt, (size_t)ncell);
     grow(&m.toff, m.c_toff, (size_t)ncell + 1);
-    launch_timed(c, "mesh_field", k_mesh_field, nblk(m3, TB), TB, mg, n, c->P4(), c->cell_start,
+    grow(&m.blo, m.c_blo, (size_t)c->C);
+    grow(&m.bhi, m.c_bhi, (size_t)c->C);
+    launch_timed(c, "mesh_boxes", k_mesh_cell_boxes, nblk(c->C, TB), TB, c->C, c->cell_start, c->P4(), m.blo, m.bhi);
+    launch_timed(c, "mesh_field", k_mesh_field, nblk(m3, TB), TB, mg, n, c->P4(), c->cell_start, m.blo, m.bhi,
                  (float)c->prm.density, m.f);
     launch_timed(c, "mesh_edges", k_mesh_edge_flags, nblk(m3, TB), TB, mg, m.f, m.flag);
     k_launch_scan(c, m.flag, m.vid, 3 * m3, m.blocks);
