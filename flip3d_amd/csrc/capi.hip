@@ -898,8 +898,7 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         // flip3D.cpp:1221-1222 sort + computeDensity
         k_launch_sort(c);
         c->sorted = true;
-        k_launch_density(c);
-        k_launch_ext_force(c);  // :1230
+        k_launch_density(c, true);  // with add_ExtForce (:1230) fused in
         // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
         k_launch_flags(c);
         k_launch_p2g(c);

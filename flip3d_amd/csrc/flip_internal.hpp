@@ -136,7 +136,7 @@ struct FlipCtx {
 
 // ---- kernels (launch wrappers; each enqueues on ctx->stream) ----
 void k_launch_sort(FlipCtx *c);                 // counting sort + per-cell id order + permute
-void k_launch_density(FlipCtx *c);
+void k_launch_density(FlipCtx *c, bool with_force = false);  // with_force: add_ExtForce fused in
 void k_launch_ext_force(FlipCtx *c);
 void k_launch_flags(FlipCtx *c);                // A and L from the sorted buckets
 void k_launch_p2g(FlipCtx *c);                  // P2G + save_grid + enforce_boundary

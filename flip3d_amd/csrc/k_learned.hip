@@ -423,8 +423,10 @@ __global__ __launch_bounds__(TB) void k_ml_post0_b(const int32_t *__restrict__ i
                                                    int n, const float *__restrict__ L, int subcell, double h2,
                                                    const float *__restrict__ r, const float *__restrict__ dinv,
                                                    const float *__restrict__ z4, const float *__restrict__ q,
-                                                   float2 *B0, float2 *B1, double *part)
+                                                   float2 *B0, float2 *B1, double *part, int32_t *hflag)
 {
+    // the stop flag into mapped host memory (null outside the iteration)
+    if (hflag && blockIdx.x == 0 && threadIdx.x == 0) hflag[0] = fi[FI_DONE];
     if (fi[FI_DONE]) return;
     const int nfl = iflags[I_NFL];
     float2 *B = ((fi[FI_K] + 1) & 1) ? B1 : B0;
@@ -510,12 +512,6 @@ __global__ __launch_bounds__(TB) void k_fcg_b(const int32_t *__restrict__ iflags
         rr += (double)(rc * rc);
     }
     store_partials(pb, 1, rr, 0.0, 0.0, 1);
-}
-
-// the stop flag into mapped host memory, at the end of every iteration
-__global__ void k_fcg_flag(const int32_t *__restrict__ fi, int32_t *hflag)
-{
-    if (threadIdx.x == 0) hflag[0] = fi[FI_DONE];
 }
 
 struct Level {
@@ -680,7 +676,7 @@ static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell,
     launch_timed(c, "ml_post0_a", k_ml_post0_a, G0, TBL, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
                        c->r, ls->dinv0, ls->z2, ls->lv[1].x, ls->z4);
     launch_timed(c, "ml_post0_b", k_ml_post0_b, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
-                       ls->dinv0, ls->z4, q, c->zs[0], c->zs[1], pz);
+                       ls->dinv0, ls->z4, q, c->zs[0], c->zs[1], pz, check ? ls->d_flag : nullptr);
 }
 
 // Flexible CG with the multilevel preconditioner from the current x (press)
@@ -722,8 +718,8 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
                      c->zs[1], c->q, pz, ls->fs, pa, G);
         launch_timed(c, "fcg_b", k_fcg_b, G, TB, c->iflags, ls->fi, c->fl, c->press, c->r, c->zs[0], c->zs[1], c->q,
                      pa, ls->fs, pb, G);
-        vcycle(c, ls, G, h2, subcell, c->q, pz, pb, true);  // its first kernel applies the stop test
-        hipLaunchKernelGGL(k_fcg_flag, 1, 64, 0, c->stream, ls->fi, ls->d_flag);
+        // its first kernel applies the stop test, its last writes the flag
+        vcycle(c, ls, G, h2, subcell, c->q, pz, pb, true);
     };
     if (!c->profiling && !ls->graph) {
         hipGraph_t g;

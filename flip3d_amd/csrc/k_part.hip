@@ -29,16 +29,25 @@ __device__ __forceinline__ float mass_of(float w) { return __builtin_fabsf(w); }
 // by the step driver through a separate call).
 // POW2: h*h is a power of two, so r2 / (h*h) == r2 * (1 / (h*h)) exactly
 // (scaling by a power of two rounds nothing while the result stays finite).
-template <bool POW2>
+// FORCE: add_ExtForce fused in (flip3D.cpp:870-875, u_y += gdt for every
+// particle); it reads and writes only velocities, which density does not use.
+template <bool POW2, bool FORCE>
 __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, const int32_t *__restrict__ cs, float h,
-                                                float inv_hh, float max_dens)
+                                                float inv_hh, float max_dens, double gdt)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     float4 p = pos[i];
     if (is_wall_w(p.w)) {
-        vel[i].w = 1.0f;
+        if (FORCE) {
+            float4 v = vel[i];
+            v.y = (float)((double)v.y + gdt);
+            v.w = 1.0f;
+            vel[i] = v;
+        } else {
+            vel[i].w = 1.0f;
+        }
         return;
     }
     int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
@@ -57,7 +66,14 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
             });
         }
     }
-    vel[i].w = wsum / max_dens;
+    if (FORCE) {
+        float4 v = vel[i];
+        v.y = (float)((double)v.y + gdt);
+        v.w = wsum / max_dens;
+        vel[i] = v;
+    } else {
+        vel[i].w = wsum / max_dens;
+    }
 }
 
 // flip3D.cpp:870-875 add_ExtForce: u_y = (float)((double)u_y + gdt), gdt = -dt*g
@@ -1088,19 +1104,18 @@ void k_launch_emit(FlipCtx *c)
     c->sorted = false;
     c->bucketed = false;
 }
-void k_launch_density(FlipCtx *c)
+void k_launch_density(FlipCtx *c, bool with_force)
 {
     if (c->np == 0) return;
     const float h = (float)(4.0 * c->prm.density / c->n), hh = h * h;
     int e = 0;
     const bool pow2 = std::isnormal(hh) && std::frexp(hh, &e) == 0.5f;
     const float inv_hh = pow2 ? std::ldexp(1.0f, 1 - e) : 0.0f;  // hh = 2^(e-1)
-    if (pow2)
-        launch_timed(c, "density", k_density<true>, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(),
-                     c->cell_start, h, inv_hh, c->max_dens);
-    else
-        launch_timed(c, "density", k_density<false>, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(),
-                     c->cell_start, h, inv_hh, c->max_dens);
+    const double gdt = -c->dt * c->prm.gravity;
+    auto kern = pow2 ? (with_force ? k_density<true, true> : k_density<true, false>)
+                     : (with_force ? k_density<false, true> : k_density<false, false>);
+    launch_timed(c, with_force ? "density_force" : "density", kern, nblk(c->np, TB), TB, c->np, c->n, c->P4(),
+                 c->V4(), c->cell_start, h, inv_hh, c->max_dens, gdt);
 }
 
 void k_launch_ext_force(FlipCtx *c)

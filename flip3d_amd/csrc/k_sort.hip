@@ -151,12 +151,42 @@ __global__ void k_sort_scatter(int64_t np, const int32_t *__restrict__ key, cons
     pid[d] = meta_id(meta[i]);
 }
 
-// 4. creation order inside every bucket
+// 4. creation order inside every bucket.  A bucket of at most SO_REG
+// particles is ranked in registers (ids are unique, so every rank is
+// distinct) and skipped when already in order, which the atomic slot
+// claim of step 1 mostly leaves it; larger buckets use an insertion sort.
+constexpr int SO_REG = 16;
+
 __global__ void k_sort_order(long C, const int32_t *__restrict__ start, int32_t *perm, int32_t *pid)
 {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     int b = start[c], e = start[c + 1];
+    const int m = e - b;
+    if (m <= 1) return;
+    if (m <= SO_REG) {
+        int key[SO_REG], val[SO_REG];
+#pragma unroll
+        for (int t = 0; t < SO_REG; ++t) {
+            key[t] = t < m ? pid[b + t] : 0x7fffffff;
+            val[t] = t < m ? perm[b + t] : 0;
+        }
+        bool ordered = true;
+#pragma unroll
+        for (int t = 1; t < SO_REG; ++t) ordered = ordered && key[t - 1] < key[t];
+        if (ordered) return;
+#pragma unroll
+        for (int t = 0; t < SO_REG; ++t) {
+            if (t < m) {
+                int r = 0;
+#pragma unroll
+                for (int u = 0; u < SO_REG; ++u) r += key[u] < key[t];
+                pid[b + r] = key[t];
+                perm[b + r] = val[t];
+            }
+        }
+        return;
+    }
     for (int a = b + 1; a < e; ++a) {
         int32_t kid = pid[a], kp = perm[a];
         int t = a - 1;
