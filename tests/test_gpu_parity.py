@@ -118,9 +118,11 @@ def _pair(n=32, scene="benchmark", **kw):
     return g, s
 
 
-def test_stage_parity_first_step():
-    """Each stage of step 1 from identical inputs (teacher forced)."""
-    g, s = _pair()
+@pytest.mark.parametrize("n", [32, 24])
+def test_stage_parity_first_step(n):
+    """Each stage of step 1 from identical inputs (teacher forced).  N=24
+    takes the paths for a kernel support h*h that is not a power of two."""
+    g, s = _pair(n=n)
     try:
         g.stage(fa.STAGE_DENSITY)
         s.call("sort")
@@ -181,11 +183,11 @@ def test_stage_parity_first_step():
         s.close()
 
 
-@pytest.mark.parametrize("scene", ["benchmark", "reference"])
-def test_trajectory_parity(scene):
+@pytest.mark.parametrize("scene,n", [("benchmark", 32), ("reference", 32), ("benchmark", 24)])
+def test_trajectory_parity(scene, n):
     """Free-running steps: flags bit-exact, fields within 1e-5 relative."""
     steps = int(os.environ.get("FLIP_PARITY_STEPS", "30"))
-    g, s = _pair(scene=scene)
+    g, s = _pair(n=n, scene=scene)
     exact = 0
     try:
         for k in range(steps):
