@@ -34,15 +34,6 @@ __device__ __forceinline__ float smooth_kernel(float r2, float h)
     return v > 0.0f ? v : 0.0f;
 }
 
-// smooth_kernel when h*h is a power of two and inv_hh = 1 / (h*h): the
-// quotient r2 / (h*h) is then exactly r2 * inv_hh, so the result is the same.
-__device__ __forceinline__ float smooth_kernel_scaled(float r2, float inv_hh)
-{
-    float q = r2 * inv_hh;
-    float v = 1.0f - q;
-    return v > 0.0f ? v : 0.0f;
-}
-
 // kernel.cpp:8-10 sharp_kernel: (float)fmax((double)(h*h)/fmax((double)r2,1e-5) - 1.0, 0.0).
 // r2 >= h*h gives exactly 0 (quotient <= 1); NaN r2 takes the full path like fmax does.
 __device__ __forceinline__ float sharp_kernel_hh(float r2, float hh)
@@ -50,6 +41,47 @@ __device__ __forceinline__ float sharp_kernel_hh(float r2, float hh)
     if (r2 >= hh) return 0.0f;
     double v = (double)hh / fmax((double)r2, 1.0e-5) - 1.0;
     return (float)fmax(v, 0.0);
+}
+
+// a / b correctly rounded for positive normal a and b far from the ends of
+// the double range (here a is a positive normal float h*h and b >= 1e-5, so
+// a, b, 1/b and a/b all lie within 2^-260..2^150).  This is the IEEE division sequence
+// the compiler emits for '/' (v_rcp_f64, two Newton steps on the
+// reciprocal, one remainder correction) without v_div_scale and
+// v_div_fixup: those rescale operands only near overflow, underflow or
+// denormals and patch zero, infinite and NaN operands, and return their
+// input unchanged on this range, while v_div_fmas is then a plain fma.
+// Being plain VGPR arithmetic (no VCC), several of these can be in flight
+// in one wave at once.
+__device__ __forceinline__ double div_pos(double a, double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+}
+
+// sharp_kernel_hh for r2 < hh; callers discard the value for other r2
+// (NaN and infinite r2 included), so no early return is needed
+__device__ __forceinline__ float sharp_kernel_in(float r2, float hh)
+{
+    const double v = div_pos((double)hh, fmax((double)r2, 1.0e-5)) - 1.0;
+    return (float)fmax(v, 0.0);
+}
+
+// smooth_kernel when h*h is a power of two and inv_hh = 1 / (h*h): the
+// quotient r2 / (h*h) is then exactly r2 * inv_hh, so fma(-r2, inv_hh, 1)
+// rounds 1 - r2/(h*h) once, as the float subtraction does.  The value is at
+// most 1, so clamping it to [0, 1] (the fma's clamp bit) is the max with 0,
+// NaN included.
+__device__ __forceinline__ float smooth_kernel_pow2(float r2, float inv_hh)
+{
+    const float v = __builtin_fmaf(-r2, inv_hh, 1.0f);
+    return fminf(fmaxf(v, 0.0f), 1.0f);
 }
 
 // utility.cpp:20-26 length2: float (a*a + b*b) + c*c of a - b

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
             for_range4(pos, b, e, [&](const float4 &q, int) {
                 if (is_wall_w(q.w)) return;
                 const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
-                float w = q.w * (POW2 ? smooth_kernel_scaled(r2, inv_hh) : smooth_kernel(r2, h));
+                float w = q.w * (POW2 ? smooth_kernel_pow2(r2, inv_hh) : smooth_kernel(r2, h));
                 wsum += w;
             });
         }
@@ -172,16 +172,41 @@ __device__ __forceinline__ void p2g_face_coord(int i, int j, int k, float &fx, f
     fz = COMP == 2 ? (float)k : (float)(k + 0.5);
 }
 
+// Branch-free: four window particles' weights are formed side by side (four
+// independent division chains per wave), then added in window order.  A
+// particle outside the support adds +0, which leaves both sums unchanged:
+// they start at +0 and a float sum of values that is -0 needs every term
+// -0, so a sum never becomes -0 and x + 0 == x.
+__device__ __forceinline__ void p2g_term(const float4 &q, float fx, float fy, float fz, float hh, float &tw,
+                                         float &tx)
+{
+    const float r2 = length2(q.x, q.y, q.z, fx, fy, fz);
+    const bool in = r2 < hh;
+    const float w = sharp_kernel_in(r2, hh);
+    tw = in ? w : 0.0f;
+    tx = in ? w * q.w : 0.0f;
+}
+
 __device__ __forceinline__ void p2g_accum(const float4 *__restrict__ src, int b, int e, float fx, float fy, float fz,
                                           float hh, float &sumw, float &sumx)
 {
-    for_range4(src, b, e, [&](const float4 &q, int) {
-        float r2 = length2(q.x, q.y, q.z, fx, fy, fz);
-        if (!(r2 < hh)) return;
-        float w = sharp_kernel_hh(r2, hh);
-        sumx += w * q.w;
-        sumw += w;
-    });
+    int a = b;
+    for (; a + 3 < e; a += 4) {
+        float tw[4], tx[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) p2g_term(src[a + u], fx, fy, fz, hh, tw[u], tx[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sumx += tx[u];
+            sumw += tw[u];
+        }
+    }
+    for (; a < e; ++a) {
+        float tw, tx;
+        p2g_term(src[a], fx, fy, fz, hh, tw, tx);
+        sumx += tx;
+        sumw += tw;
+    }
 }
 
 // u_save = raw value (save_grid, flip3D.cpp:971) and u = value with
@@ -975,32 +1000,14 @@ __global__ void k_spring_fix(int m, const int32_t *__restrict__ list, const int3
 // corrector.cpp:15-47 resample at the pseudo-moved position with the old
 // positions and velocities (re = DENSITY/N); keeps u when no weight.
 constexpr int RESAMPLE_TB = 256;
+constexpr int RESAMPLE_NEAR = 16;
 
-__global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, const float4 *__restrict__ pos,
-                                                          const float4 *__restrict__ vel,
-                                                          const int32_t *__restrict__ cs,
-                                                          const float4 *__restrict__ tmp0, float4 *tmp1, float hh)
+// the weighted window sum of one particle in window order (the reference's loop)
+__device__ __forceinline__ void resample_window(int n, const float4 &p, const WinCull &wc, int ci, int cj, int ck,
+                                                const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                                const int32_t *__restrict__ cs, float hh, float &ax, float &ay,
+                                                float &az, float &wsum)
 {
-    __shared__ int32_t s_off[9 * RESAMPLE_TB / 64], s_idx[RESAMPLE_TB];
-    const int64_t base = (int64_t)blockIdx.x * RESAMPLE_TB;
-    const float g = (float)n;
-    {
-        const int64_t i = base + threadIdx.x;
-        int grp = 8;
-        if (i < np && !is_wall_w(pos[i].w)) {
-            const float4 p = tmp0[i];
-            grp = octant_of(g * p.x, g * p.y, g * p.z, cell_of(n, p.x), cell_of(n, p.y), cell_of(n, p.z));
-        }
-        const int t = octant_regroup<RESAMPLE_TB>(grp, s_off, s_idx);
-        if (base + t >= np) return;
-    }
-    const int64_t i = base + s_idx[threadIdx.x];
-    if (is_wall_w(pos[i].w)) return;
-    float4 p = tmp0[i];
-    float4 v0 = vel[i];
-    int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
-    const WinCull wc(n, g * p.x, g * p.y, g * p.z, ci, cj, ck, hh);
-    float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
@@ -1024,8 +1031,78 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
             });
         }
     }
-    if (wsum != 0.0f) tmp1[i] = make_float4(ax / wsum, ay / wsum, az / wsum, 0.0f);
-    else tmp1[i] = make_float4(v0.x, v0.y, v0.z, 0.0f);
+}
+
+// Two phases per particle, so that a wave's lanes run the division-heavy
+// weight arithmetic together: phase 1 scans the window with the squared
+// distance test alone and lists the non-wall particles inside the support
+// (window order) in LDS; phase 2 forms the weights and sums over the list in
+// that order, exactly as resample_window would.  More than RESAMPLE_NEAR
+// listed particles falls back to resample_window itself.
+__global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, const float4 *__restrict__ pos,
+                                                          const float4 *__restrict__ vel,
+                                                          const int32_t *__restrict__ cs,
+                                                          const float4 *__restrict__ tmp0, float4 *tmp1, float hh)
+{
+    __shared__ int32_t near[RESAMPLE_NEAR][RESAMPLE_TB];
+    __shared__ int32_t s_off[9 * RESAMPLE_TB / 64], s_idx[RESAMPLE_TB];
+    const int64_t base = (int64_t)blockIdx.x * RESAMPLE_TB;
+    const float g = (float)n;
+    {
+        const int64_t i = base + threadIdx.x;
+        int grp = 8;
+        if (i < np && !is_wall_w(pos[i].w)) {
+            const float4 p = tmp0[i];
+            grp = octant_of(g * p.x, g * p.y, g * p.z, cell_of(n, p.x), cell_of(n, p.y), cell_of(n, p.z));
+        }
+        const int t = octant_regroup<RESAMPLE_TB>(grp, s_off, s_idx);
+        if (base + t >= np) return;
+    }
+    const int tid = threadIdx.x;
+    const int64_t i = base + s_idx[tid];
+    if (is_wall_w(pos[i].w)) return;
+    const float4 p = tmp0[i];
+    const int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+    const WinCull wc(n, g * p.x, g * p.y, g * p.z, ci, cj, ck, hh);
+    int cnt = 0;
+    for (int si = ci - 1; si <= ci + 1; ++si) {
+        if (si < 0 || si > n - 1) continue;
+        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
+            if (sj < 0 || sj > n - 1) continue;
+            float dij;
+            int klo, khi;
+            if (!wc.row(ci, cj, si, sj, dij)) continue;
+            wc.cells(n, ck, dij, klo, khi);
+            if (klo > khi) continue;
+            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
+            for_range4(pos, b, e, [&](const float4 &q, int a) {
+                if (is_wall_w(q.w) || !(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
+                if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
+                ++cnt;
+            });
+        }
+    }
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
+    if (cnt > RESAMPLE_NEAR) {
+        resample_window(n, p, wc, ci, cj, ck, pos, vel, cs, hh, ax, ay, az, wsum);
+    } else {
+        for (int k = 0; k < cnt; ++k) {
+            const int a = near[k][tid];
+            const float4 q = pos[a];
+            const float w = q.w * sharp_kernel_in(length2(p.x, p.y, p.z, q.x, q.y, q.z), hh);
+            const float4 u = vel[a];
+            ax += w * u.x;
+            ay += w * u.y;
+            az += w * u.z;
+            wsum += w;
+        }
+    }
+    if (wsum != 0.0f) {
+        tmp1[i] = make_float4(ax / wsum, ay / wsum, az / wsum, 0.0f);
+    } else {
+        const float4 v0 = vel[i];
+        tmp1[i] = make_float4(v0.x, v0.y, v0.z, 0.0f);
+    }
 }
 
 // corrector.cpp:103-114 update
