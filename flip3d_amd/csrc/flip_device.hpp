@@ -141,6 +141,29 @@ __device__ __forceinline__ void for_range4(const T *__restrict__ src, int b, int
     for (; a < e; ++a) body(src[a], a);
 }
 
+// for_range4 with eight loads in flight (then four, then one at a time)
+template <class T, class F>
+__device__ __forceinline__ void for_range8(const T *__restrict__ src, int b, int e, F &&body)
+{
+    int a = b;
+    for (; a + 7 < e; a += 8) {
+        T q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = src[a + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) body(q[u], a + u);
+    }
+    if (a + 3 < e) {
+        T q0 = src[a], q1 = src[a + 1], q2 = src[a + 2], q3 = src[a + 3];
+        body(q0, a);
+        body(q1, a + 1);
+        body(q2, a + 2);
+        body(q3, a + 3);
+        a += 4;
+    }
+    for (; a < e; ++a) body(src[a], a);
+}
+
 // ------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_sum(double v)
 {

@@ -52,20 +52,28 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
     }
     int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
     int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
-    float wsum = 0.0f;
-    for (int si = ci - 1; si <= ci + 1; ++si) {
-        if (si < 0 || si > n - 1) continue;
-        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
-            if (sj < 0 || sj > n - 1) continue;
-            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for_range4(pos, b, e, [&](const float4 &q, int) {
-                if (is_wall_w(q.w)) return;
-                const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
-                float w = q.w * (POW2 ? smooth_kernel_pow2(r2, inv_hh) : smooth_kernel(r2, h));
-                wsum += w;
-            });
-        }
+    // The window's nine rows (si, sj) in the reference's order; their bounds
+    // are loaded together (rows outside the grid are empty) and each row is
+    // read eight particles at a time, so a wave waits on memory fewer times.
+    int rlo[9], rhi[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int si = ci - 1 + r / 3, sj = cj - 1 + r % 3;
+        const bool ok = si >= 0 && si <= n - 1 && sj >= 0 && sj <= n - 1;
+        const long c0 = ok ? cidx(n, si, sj, klo) : cidx(n, ci, cj, klo);
+        const int b = cs[c0], e = cs[c0 + (khi - klo) + 1];
+        rlo[r] = ok ? b : 0;
+        rhi[r] = ok ? e : 0;
     }
+    float wsum = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int) {
+            if (is_wall_w(q.w)) return;
+            const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
+            float w = q.w * (POW2 ? smooth_kernel_pow2(r2, inv_hh) : smooth_kernel(r2, h));
+            wsum += w;
+        });
     if (FORCE) {
         float4 v = vel[i];
         v.y = (float)((double)v.y + gdt);
@@ -715,6 +723,28 @@ struct WinCull {
     }
 };
 
+// Bounds of the nine window rows (si, sj) = (ci - 1 + r / 3, cj - 1 + r % 3),
+// in the reference's visiting order, loaded together: rows outside the grid
+// or out of reach (wc.row), and rows whose cells in reach are none, are
+// empty.
+__device__ __forceinline__ void window_rows(int n, const WinCull &wc, int ci, int cj, int ck,
+                                            const int32_t *__restrict__ cs, int rlo[9], int rhi[9])
+{
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int si = ci - 1 + r / 3, sj = cj - 1 + r % 3;
+        float dij;
+        int klo, khi;
+        const bool reach = wc.row(ci, cj, si, sj, dij);
+        wc.cells(n, ck, dij, klo, khi);
+        const bool ok = si >= 0 && si <= n - 1 && sj >= 0 && sj <= n - 1 && reach && klo <= khi;
+        const long c0 = ok ? cidx(n, si, sj, klo) : 0;
+        const int b = cs[c0], e = cs[c0 + (ok ? khi - klo : 0) + 1];
+        rlo[r] = ok ? b : 0;
+        rhi[r] = ok ? e : 0;
+    }
+}
+
 // Block-local regrouping of a block's B particles by the octant of the cell
 // that centres their window.  With the corrector's half-cell radius the
 // octant (which half of the cell along each axis) fixes the neighbour rows
@@ -897,24 +927,16 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
     const float far2 = re * re * 1.0001f;  // as in spring_one
     const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
     int cnt = 0;
-    for (int si = ci - 1; si <= ci + 1; ++si) {
-        if (si < 0 || si > n - 1) continue;
-        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
-            if (sj < 0 || sj > n - 1) continue;
-            float dij;
-            int klo, khi;
-            if (!wc.row(ci, cj, si, sj, dij)) continue;
-            wc.cells(n, ck, dij, klo, khi);
-            if (klo > khi) continue;
-            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for_range4(pos, b, e, [&](const float4 &q, int a) {
-                if (a == i) return;
-                if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
-                if (cnt < SPRING_NEAR) near[cnt][tid] = a;
-                ++cnt;
-            });
-        }
-    }
+    int rlo[9], rhi[9];
+    window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
+            if (a == i) return;
+            if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
+            if (cnt < SPRING_NEAR) near[cnt][tid] = a;
+            ++cnt;
+        });
     int r = 0;
     if (cnt > SPRING_NEAR) {
         r = spring_one<false>(i, n, pos, meta, nrm_id, cs, dt, re, nullptr, tmp0);
@@ -1065,23 +1087,15 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
     const int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
     const WinCull wc(n, g * p.x, g * p.y, g * p.z, ci, cj, ck, hh);
     int cnt = 0;
-    for (int si = ci - 1; si <= ci + 1; ++si) {
-        if (si < 0 || si > n - 1) continue;
-        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
-            if (sj < 0 || sj > n - 1) continue;
-            float dij;
-            int klo, khi;
-            if (!wc.row(ci, cj, si, sj, dij)) continue;
-            wc.cells(n, ck, dij, klo, khi);
-            if (klo > khi) continue;
-            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for_range4(pos, b, e, [&](const float4 &q, int a) {
-                if (is_wall_w(q.w) || !(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
-                if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
-                ++cnt;
-            });
-        }
-    }
+    int rlo[9], rhi[9];
+    window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
+            if (is_wall_w(q.w) || !(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
+            if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
+            ++cnt;
+        });
     float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
     if (cnt > RESAMPLE_NEAR) {
         resample_window(n, p, wc, ci, cj, ck, pos, vel, cs, hh, ax, ay, az, wsum);

@@ -92,7 +92,7 @@ def smooth_noise(A, rs, passes=4):
 
 
 # ---------------------------------------------------------------- V-cycle replay
-MG_W = 0.8   # damped Jacobi weight of the GPU V-cycle
+MG_W = 0.9   # damped Jacobi weight of the GPU V-cycle (the shipped weights were trained with 0.8)
 MG_S = 10.0  # its coarse-correction scale
 
 
