@@ -35,7 +35,7 @@ EXPORTS = [
     "flip_gpu_step_begin",
     "flip_gpu_set_learned_weights", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
     "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
-    "flip_unet_predict", "flip_unet_save", "flip_unet_load",
+    "flip_unet_predict", "flip_unet_predict_fused", "flip_unet_save", "flip_unet_load",
 ]
 
 
@@ -126,6 +126,7 @@ def lib():
         L.flip_unet_backward.argtypes = [P, C.c_int, P, P]
         L.flip_unet_train_step.argtypes = [P, C.c_int, P, P, P, P, C.c_float, C.POINTER(C.c_double)]
         L.flip_unet_predict.argtypes = [P, C.c_int, P, P, P, P, P]
+        L.flip_unet_predict_fused.argtypes = [P, P, P, P, P]
         L.flip_unet_save.argtypes = [P, C.c_char_p]
         L.flip_unet_load.argtypes = [P, C.c_char_p]
         _lib = L
@@ -449,6 +450,16 @@ class UNet:
         S = A.size // self.R ** 3
         x = np.empty(b.shape, np.float32)
         _check(lib().flip_unet_predict(self.h, S, _ptr(A), _ptr(L), _ptr(b), _ptr(x), None))
+        return x
+
+    def predict_fused(self, A, L, b):
+        """predict() of one system through the solver's fused inference
+        kernels (bit-identical to predict())."""
+        A, L, b, _ = self._systems(A, L, b)
+        if A.size != self.R ** 3:
+            raise ValueError("predict_fused takes one system")
+        x = np.empty(b.shape, np.float32)
+        _check(lib().flip_unet_predict_fused(self.h, _ptr(A), _ptr(L), _ptr(b), _ptr(x)))
         return x
 
     def save(self, path):

@@ -530,7 +530,9 @@ __global__ void k_rhs_partials(const int8_t *__restrict__ A, const float *__rest
 
 // Inference forward of one sample: the decoder's upsample-adds are formed
 // inside the convolutions, and the last convolution applies the head and
-// writes the solution directly.
+// writes the solution directly.  (Forming the encoder's poolings inside the
+// next convolution's operand loads was slower: each pooled voxel is then
+// averaged once per tap, 27 times.)
 static void unet_infer(UNet *u, const float *sigma, float *x)
 {
     hipStream_t s = u->stream;
@@ -781,6 +783,22 @@ int flip_unet_predict(flip_unet *h, int S, const int8_t *A, const float *L, cons
         FLIP_CHECK(hipMemcpyAsync(x, t.err, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         FLIP_CHECK(hipStreamSynchronize(s));
         (void)loss_vs;
+        return FLIP_OK;
+    });
+}
+
+int flip_unet_predict_fused(flip_unet *h, const int8_t *A, const float *L, const float *b, float *x)
+{
+    return guarded_u([&]() -> int {
+        if (!h || !A || !L || !b || !x) return fail(FLIP_E_ARG, "null argument");
+        Trainer &t = h->impl.tr;
+        UNet *u = t.net;
+        hipStream_t s = u->stream;
+        const long n = u->vox(0, 1);
+        stage_systems(t, 1, A, L, b, nullptr);
+        unet_infer(u, t.sigma, t.err);
+        FLIP_CHECK(hipMemcpyAsync(x, t.err, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        FLIP_CHECK(hipStreamSynchronize(s));
         return FLIP_OK;
     });
 }

@@ -48,6 +48,11 @@ int flip_unet_train_step(flip_unet *net, int S, const int8_t *A, const float *L,
 /* x ~ A^-1 b for S systems. */
 int flip_unet_predict(flip_unet *net, int S, const int8_t *A, const float *L, const float *b, float *x,
                       double *reserved);
+/* predict() for one system (S = 1) through the solver's inference kernels:
+ * the same network with its upsample-adds and head fused into the
+ * convolutions.  The result equals predict()'s bit for bit; no counterpart
+ * in the reference (the learned solver is this library's own). */
+int flip_unet_predict_fused(flip_unet *net, const int8_t *A, const float *L, const float *b, float *x);
 /* Weights file: "FLIPUNET", int32 version 1, int32 count, count floats. */
 int flip_unet_save(flip_unet *net, const char *path);
 int flip_unet_load(flip_unet *net, const char *path);

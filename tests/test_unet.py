@@ -1,6 +1,8 @@
 """The learned coarse solver's U-Net: HIP BF16-MFMA forward/backward against
 the numpy reference, the reference pinned by finite differences, and
 training on solver-generated systems."""
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -189,3 +191,23 @@ def test_training_fits_solver_systems():
             t = x[s][fl].astype(np.float64)
             rel = (e @ (M @ e)) / (t @ (M @ t))
             assert rel < 0.6, (s, rel)
+
+
+@pytest.mark.gpu
+def test_fused_inference_equals_predict():
+    """The solver's inference path (upsample-adds and head fused into the
+    convolutions) returns predict()'s result bit for bit, with the shipped
+    weights and with random ones, at the coarse level's side 32."""
+    import flip3d_amd as fa
+    R = 32
+    for seed in (7, 8):
+        A, L, b, _ = box_system(seed, R)
+        with fa.UNet(R, capacity=1, seed=seed) as net:
+            if seed == 7:
+                net.load(os.path.join(os.path.dirname(fa.__file__), "weights", "unet_coarse.bin"))
+            else:
+                net.set_params(random_params(seed).astype(np.float32))
+            ref = net.predict(A[None], L[None], b[None])[0]
+            got = net.predict_fused(A, L, b)
+            assert np.array_equal(ref.view(np.uint32), got.view(np.uint32)), (seed, np.abs(ref - got).max())
+            assert np.abs(ref).max() > 0
