@@ -232,6 +232,9 @@ def main():
     ap.add_argument("--no-profile-step", action="store_true",
                     help="skip the per-launch timed step (for runs under an external profiler)")
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesher / preview leg")
+    ap.add_argument("--direct-launch", action="store_true",
+                    help="launch solver iterations kernel by kernel instead of replaying HIP graphs "
+                         "(for kernel tracers that cannot follow graph launches; slower)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -240,6 +243,8 @@ def main():
               "(the slab decomposition is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.direct_launch:
+        os.environ["FLIP_DIRECT_LAUNCH"] = "1"
     import flip3d_amd as fa
 
     n = args.n
@@ -267,7 +272,8 @@ def main():
         "config": {"workload": f"benchmark dam-break scene N={n}, {n_particles} particles "
                                f"({main_run['n_fluid']} fluid cells), CFL dt, pressure solve to rel 1e-5, "
                                f"{args.precond} preconditioner; plain reference-preconditioned PCG timed beside it",
-                   "n": n, "particles": n_particles, "precond": args.precond, "tol": TOL},
+                   "n": n, "particles": n_particles, "precond": args.precond, "tol": TOL,
+                   "solver_launch": "direct" if args.direct_launch else "graph"},
         "particles_per_s": round(value * n_particles, 1),
         "speedup_vs_plain_step": round(value / plain["steps_per_s"], 3),
         "speedup_vs_plain_solve": round(plain["solve_ms_mean"] / main_run["solve_ms_mean"], 3),

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -516,6 +517,8 @@ int flip_gpu_create(const flip_params *p, flip_ctx **out)
             c->F = (long)(p->n + 1) * p->n * p->n;
             c->dt = p->dt;
             c->rng.seed(p->rand_seed);
+            const char *dl = std::getenv("FLIP_DIRECT_LAUNCH");
+            c->direct_launch = dl && dl[0] == '1';
             FLIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             alloc_grids(c);
             alloc_particles(c, 1);

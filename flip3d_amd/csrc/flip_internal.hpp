@@ -47,6 +47,10 @@ struct FlipCtx {
     long C = 0, F = 0;  // cells n^3, face-grid size (n+1) n^2
     hipStream_t stream = nullptr;
     bool profiling = false;
+    // FLIP_DIRECT_LAUNCH=1 at creation: solver iterations are launched kernel
+    // by kernel instead of replayed from captured HIP graphs (for tracing
+    // tools that cannot follow graph launches); results are unchanged
+    bool direct_launch = false;
 
     // particles (double-buffered for the permutation gather)
     int64_t np = 0, cap = 0;

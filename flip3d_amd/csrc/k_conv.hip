@@ -208,14 +208,20 @@ __global__ __launch_bounds__(64 * KS) void k_conv3_fwd_sk(const uint16_t *__rest
         if (st >= STEPS || q >= NCH) return zero8();
         return load8(Wt + ((long)(q / CPT) * COUT + n0 + row) * CIN + (q % CPT) * 8);
     };
-    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    bf16x8 a = load_a(wave), b = load_b(wave);
-    for (int st = wave; st < STEPS; st += KS) {
-        const bf16x8 an = load_a(st + KS), bn = load_b(st + KS);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-        a = an;
-        b = bn;
+    // A wave has only a few K steps (STEPS / KS, about four): all of their
+    // operands are loaded before the first MFMA, so the wave waits on memory
+    // once, and the steps are then summed in the same order as a loop would.
+    constexpr int NS = (STEPS + KS - 1) / KS;
+    bf16x8 a[NS], b[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        a[i] = load_a(wave + i * KS);
+        b[i] = load_b(wave + i * KS);
     }
+    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+        if (wave + i * KS < STEPS) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[i], acc, 0, 0, 0);
     if (wave > 0) red[wave - 1][lane] = acc;
     __syncthreads();
     if (wave == 0) {

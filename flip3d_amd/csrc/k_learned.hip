@@ -712,7 +712,8 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
     t_end(c);
     vcycle(c, ls, G, h2, subcell, nullptr, pz, pb, false);  // z_0 into B_1.x
 
-    if (!ls->graph || ls->graph_subcell != subcell || c->profiling) {
+    const bool direct = c->profiling || c->direct_launch;
+    if (!ls->graph || ls->graph_subcell != subcell || direct) {
         if (ls->graph) {
             (void)hipGraphExecDestroy(ls->graph);
             ls->graph = nullptr;
@@ -726,7 +727,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
         // its first kernel applies the stop test, its last writes the flag
         vcycle(c, ls, G, h2, subcell, c->q, pz, pb, true);
     };
-    if (!c->profiling && !ls->graph) {
+    if (!direct && !ls->graph) {
         hipGraph_t g;
         FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         iteration();
@@ -740,7 +741,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
     volatile int32_t *flag = ls->h_flag;
     flag[0] = 0;
     for (int k = 0; k < FCG_MAX; ++k) {
-        if (c->profiling) {
+        if (direct) {
             iteration();
         } else {
             FLIP_CHECK(hipGraphLaunch(ls->graph, c->stream));

@@ -611,9 +611,10 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
     }
     if (st) st->learned_iterations = learned_iters;
     // Jacobi iterations run as replays of a captured graph of BATCH
-    // iterations plus the stop check; profiling or the wavefront path launch
-    // kernel by kernel (one iteration per host check when profiling).
-    const bool use_graph = jacobi && !c->profiling;
+    // iterations plus the stop check; profiling, direct launch or the
+    // wavefront path launch kernel by kernel (one iteration per host check
+    // when profiling).
+    const bool use_graph = jacobi && !c->profiling && !c->direct_launch;
     if (use_graph && (!c->pcg_graph || c->pcg_graph_subcell != subcell)) {
         if (c->pcg_graph) {
             (void)hipGraphExecDestroy(c->pcg_graph);
