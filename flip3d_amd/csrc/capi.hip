@@ -69,6 +69,7 @@ void free_particles(FlipCtx *c)
     dfree(c->nrm_id);
     dfree(c->inv);
     dfree(c->spos);
+    dfree(c->xyz3);
     dfree(c->tmp0);
     dfree(c->tmp1);
     dfree(c->pkey);
@@ -95,6 +96,7 @@ void alloc_particles(FlipCtx *c, int64_t cap)
     dalloc(&c->nrm_id, n);
     dalloc(&c->inv, n);
     dalloc(&c->spos, n);
+    dalloc(&c->xyz3, 3 * n);
     dalloc(&c->tmp0, n);
     dalloc(&c->tmp1, n);
     dalloc(&c->pkey, n);

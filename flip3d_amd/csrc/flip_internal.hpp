@@ -61,6 +61,10 @@ struct FlipCtx {
     float4 *nrm_id = nullptr;   // wall normals by creation index
     int32_t *inv = nullptr;     // creation index -> storage slot
     float4 *spos = nullptr;     // P2G positions: clamp(n p, 0, n), m (far away for non-fluid)
+    // (x, y, z) of each stored particle, +inf for WALL particles, written by
+    // every sort: the 12-byte window records of the density and resample
+    // scans, valid while `sorted` holds (positions unchanged since the sort)
+    float *xyz3 = nullptr;
     float4 *tmp0 = nullptr, *tmp1 = nullptr;  // corrector pseudo-position / velocity
     int32_t *pkey = nullptr, *pslot = nullptr, *perm = nullptr;
 

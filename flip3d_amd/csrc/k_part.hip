@@ -31,11 +31,22 @@ __device__ __forceinline__ float mass_of(float w) { return __builtin_fabsf(w); }
 // (scaling by a power of two rounds nothing while the result stays finite).
 // FORCE: add_ExtForce fused in (flip3D.cpp:870-875, u_y += gdt for every
 // particle); it reads and writes only velocities, which density does not use.
-template <bool POW2, bool FORCE>
+// PACKED (POW2 and every mass 1): the window is read from the sort's 12-byte
+// records xyz3, where a WALL particle sits at +inf.  Its weight is then
+// exactly +0, which leaves the sum unchanged (it starts at +0 and never
+// becomes -0), as skipping it does; a unit mass times the kernel is the
+// kernel.
+struct P3 {
+    float x, y, z;
+};
+
+template <bool POW2, bool FORCE, bool PACKED>
 __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, const int32_t *__restrict__ cs, float h,
-                                                float inv_hh, float max_dens, double gdt)
+                                                float inv_hh, float max_dens, double gdt,
+                                                const P3 *__restrict__ xyz3)
 {
+    static_assert(!PACKED || POW2, "the packed pass uses the power-of-two kernel");
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     float4 p = pos[i];
@@ -67,13 +78,20 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
     }
     float wsum = 0.0f;
 #pragma unroll
-    for (int r = 0; r < 9; ++r)
-        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int) {
-            if (is_wall_w(q.w)) return;
-            const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
-            float w = q.w * (POW2 ? smooth_kernel_pow2(r2, inv_hh) : smooth_kernel(r2, h));
-            wsum += w;
-        });
+    for (int r = 0; r < 9; ++r) {
+        if (PACKED) {
+            for_range8(xyz3, rlo[r], rhi[r], [&](const P3 &q, int) {
+                wsum += smooth_kernel_pow2(length2(q.x, q.y, q.z, p.x, p.y, p.z), inv_hh);
+            });
+        } else {
+            for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int) {
+                if (is_wall_w(q.w)) return;
+                const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
+                float w = q.w * (POW2 ? smooth_kernel_pow2(r2, inv_hh) : smooth_kernel(r2, h));
+                wsum += w;
+            });
+        }
+    }
     if (FORCE) {
         float4 v = vel[i];
         v.y = (float)((double)v.y + gdt);
@@ -1060,11 +1078,14 @@ __device__ __forceinline__ void resample_window(int n, const float4 &p, const Wi
 // distance test alone and lists the non-wall particles inside the support
 // (window order) in LDS; phase 2 forms the weights and sums over the list in
 // that order, exactly as resample_window would.  More than RESAMPLE_NEAR
-// listed particles falls back to resample_window itself.
+// listed particles falls back to resample_window itself.  PACKED: phase 1
+// reads the sort's 12-byte records (WALL particles at +inf fail the test).
+template <bool PACKED>
 __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, const float4 *__restrict__ pos,
                                                           const float4 *__restrict__ vel,
                                                           const int32_t *__restrict__ cs,
-                                                          const float4 *__restrict__ tmp0, float4 *tmp1, float hh)
+                                                          const float4 *__restrict__ tmp0, float4 *tmp1, float hh,
+                                                          const P3 *__restrict__ xyz3)
 {
     __shared__ int32_t near[RESAMPLE_NEAR][RESAMPLE_TB];
     __shared__ int32_t s_off[9 * RESAMPLE_TB / 64], s_idx[RESAMPLE_TB];
@@ -1090,12 +1111,21 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
     int rlo[9], rhi[9];
     window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
 #pragma unroll
-    for (int r = 0; r < 9; ++r)
-        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
-            if (is_wall_w(q.w) || !(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
-            if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
-            ++cnt;
-        });
+    for (int r = 0; r < 9; ++r) {
+        if (PACKED) {
+            for_range8(xyz3, rlo[r], rhi[r], [&](const P3 &q, int a) {
+                if (!(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
+                if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
+                ++cnt;
+            });
+        } else {
+            for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
+                if (is_wall_w(q.w) || !(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
+                if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
+                ++cnt;
+            });
+        }
+    }
     float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
     if (cnt > RESAMPLE_NEAR) {
         resample_window(n, p, wc, ci, cj, ck, pos, vel, cs, hh, ax, ay, az, wsum);
@@ -1203,10 +1233,13 @@ void k_launch_density(FlipCtx *c, bool with_force)
     const bool pow2 = std::isnormal(hh) && std::frexp(hh, &e) == 0.5f;
     const float inv_hh = pow2 ? std::ldexp(1.0f, 1 - e) : 0.0f;  // hh = 2^(e-1)
     const double gdt = -c->dt * c->prm.gravity;
-    auto kern = pow2 ? (with_force ? k_density<true, true> : k_density<true, false>)
-                     : (with_force ? k_density<false, true> : k_density<false, false>);
+    // the sort's packed records are current while the storage is sorted
+    const bool packed = pow2 && c->unit_mass && c->sorted;
+    auto kern = packed ? (with_force ? k_density<true, true, true> : k_density<true, false, true>)
+                       : pow2 ? (with_force ? k_density<true, true, false> : k_density<true, false, false>)
+                              : (with_force ? k_density<false, true, false> : k_density<false, false, false>);
     launch_timed(c, with_force ? "density_force" : "density", kern, nblk(c->np, TB), TB, c->np, c->n, c->P4(),
-                 c->V4(), c->cell_start, h, inv_hh, c->max_dens, gdt);
+                 c->V4(), c->cell_start, h, inv_hh, c->max_dens, gdt, reinterpret_cast<const P3 *>(c->xyz3));
 }
 
 void k_launch_ext_force(FlipCtx *c)
@@ -1459,8 +1492,10 @@ int k_correct(FlipCtx *c)
                            c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
         FLIP_CHECK(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
     }
-    launch_timed(c, "resample", k_resample, nblk(c->np, RESAMPLE_TB), RESAMPLE_TB, c->np, n, c->P4(), c->V4(), c->cell_start, c->tmp0,
-                 c->tmp1, re * re);
+    // the sort's packed records are current while the storage is sorted
+    launch_timed(c, "resample", c->sorted ? k_resample<true> : k_resample<false>, nblk(c->np, RESAMPLE_TB),
+                 RESAMPLE_TB, c->np, n, c->P4(), c->V4(), c->cell_start, c->tmp0, c->tmp1, re * re,
+                 reinterpret_cast<const P3 *>(c->xyz3));
     launch_timed(c, "correct_update", k_correct_update, nblk(c->np, TB), TB, c->np, c->P4(), c->V4(), c->tmp0,
                  c->tmp1);
     return events;

@@ -203,12 +203,20 @@ __global__ void k_sort_order(long C, const int32_t *__restrict__ start, int32_t 
 // 5. permute particle state
 __global__ void k_sort_gather(int64_t np, const int32_t *__restrict__ perm, const float4 *__restrict__ pin,
                               const float4 *__restrict__ vin, const int32_t *__restrict__ min, float4 *pout,
-                              float4 *vout, int32_t *mout, int32_t *inv)
+                              float4 *vout, int32_t *mout, int32_t *inv, float *xyz3)
 {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= np) return;
     int32_t s = perm[j];
-    pout[j] = pin[s];
+    const float4 p = pin[s];
+    pout[j] = p;
+    // the packed window record: a WALL particle (sign bit of m) sits at
+    // +inf, so every squared distance to it is +inf
+    const bool wall = __builtin_signbit(p.w);
+    const float inf = __builtin_inff();
+    xyz3[3 * j + 0] = wall ? inf : p.x;
+    xyz3[3 * j + 1] = wall ? inf : p.y;
+    xyz3[3 * j + 2] = wall ? inf : p.z;
     vout[j] = vin[s];
     int32_t m = min[s];
     mout[j] = m;
@@ -255,7 +263,7 @@ void k_launch_sort(FlipCtx *c)
     t_begin(c, "sort_gather");
     int nx = c->cur ^ 1;
     hipLaunchKernelGGL(k_sort_gather, nblk(np, 256), 256, 0, c->stream, np, c->perm, c->pos[c->cur],
-                       c->vel[c->cur], c->meta[c->cur], c->pos[nx], c->vel[nx], c->meta[nx], c->inv);
+                       c->vel[c->cur], c->meta[c->cur], c->pos[nx], c->vel[nx], c->meta[nx], c->inv, c->xyz3);
     c->cur = nx;
     t_end(c);
 }
