@@ -77,10 +77,11 @@ def kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid):
         # flexible-CG kernels of the learned solver, per FLUID cell
         "fcg_a": 28 * n_fluid,    # fl 4, meta 4, L 4 (AIR-adjacent), (z, s) 8 -> (z, s) 8, q 4
         "fcg_b": 28 * n_fluid,    # fl 4, x 4, s 4, r 4, q 4 -> x 4, r 4
-        # particle gathers, per particle (float4 pos/vel = 16 B)
-        "density": 16 * n_particles + 4 * n_particles + 4 * n_cells,
+        # particle gathers, per particle (float4 pos/vel = 16 B); density reads
+        # its windows from the sort's 12-byte (x, y, z) records
+        "density": 16 * n_particles + 12 * n_particles + 4 * n_particles + 4 * n_cells,
         # density with add_ExtForce fused in: the velocity is read and written whole
-        "density_force": 16 * n_particles + 32 * n_particles + 4 * n_cells,
+        "density_force": 16 * n_particles + 12 * n_particles + 32 * n_particles + 4 * n_cells,
         "p2g": 32 * n_particles + 4 * n_cells + 2 * 4 * n_faces + n_cells,
         "spring": 16 * n_particles + 4 * n_particles + 16 * n_particles + 4 * n_cells,
         "resample": 16 * n_particles * 3 + 16 * n_particles + 4 * n_cells,
