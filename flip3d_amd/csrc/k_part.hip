@@ -1078,9 +1078,8 @@ __device__ __forceinline__ void resample_window(int n, const float4 &p, const Wi
 // distance test alone and lists the non-wall particles inside the support
 // (window order) in LDS; phase 2 forms the weights and sums over the list in
 // that order, exactly as resample_window would.  More than RESAMPLE_NEAR
-// listed particles falls back to resample_window itself.  PACKED: phase 1
-// reads the sort's 12-byte records (WALL particles at +inf fail the test).
-template <bool PACKED>
+// listed particles falls back to resample_window itself.  Phase 1 reads the
+// sort's 12-byte records (WALL particles sit at +inf and fail the test).
 __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, const float4 *__restrict__ pos,
                                                           const float4 *__restrict__ vel,
                                                           const int32_t *__restrict__ cs,
@@ -1111,21 +1110,12 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
     int rlo[9], rhi[9];
     window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        if (PACKED) {
-            for_range8(xyz3, rlo[r], rhi[r], [&](const P3 &q, int a) {
-                if (!(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
-                if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
-                ++cnt;
-            });
-        } else {
-            for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
-                if (is_wall_w(q.w) || !(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
-                if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
-                ++cnt;
-            });
-        }
-    }
+    for (int r = 0; r < 9; ++r)
+        for_range8(xyz3, rlo[r], rhi[r], [&](const P3 &q, int a) {
+            if (!(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
+            if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
+            ++cnt;
+        });
     float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
     if (cnt > RESAMPLE_NEAR) {
         resample_window(n, p, wc, ci, cj, ck, pos, vel, cs, hh, ax, ay, az, wsum);
@@ -1453,6 +1443,8 @@ int k_stuck_and_reposition(FlipCtx *c)
 int k_correct(FlipCtx *c)
 {
     if (c->np == 0) return 0;
+    // the sort's packed records (resample) are current while the storage is sorted
+    if (!c->sorted) throw flip_arg_error{FLIP_E_STATE, "the corrector needs sorted particles"};
     int n = c->n;
     float dt = (float)c->dt;
     float re = (float)(c->prm.density / n);
@@ -1492,10 +1484,8 @@ int k_correct(FlipCtx *c)
                            c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
         FLIP_CHECK(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
     }
-    // the sort's packed records are current while the storage is sorted
-    launch_timed(c, "resample", c->sorted ? k_resample<true> : k_resample<false>, nblk(c->np, RESAMPLE_TB),
-                 RESAMPLE_TB, c->np, n, c->P4(), c->V4(), c->cell_start, c->tmp0, c->tmp1, re * re,
-                 reinterpret_cast<const P3 *>(c->xyz3));
+    launch_timed(c, "resample", k_resample, nblk(c->np, RESAMPLE_TB), RESAMPLE_TB, c->np, n, c->P4(), c->V4(),
+                 c->cell_start, c->tmp0, c->tmp1, re * re, reinterpret_cast<const P3 *>(c->xyz3));
     launch_timed(c, "correct_update", k_correct_update, nblk(c->np, TB), TB, c->np, c->P4(), c->V4(), c->tmp0,
                  c->tmp1);
     return events;

@@ -244,6 +244,29 @@ def test_solver_modes_match_oracle(precond):
     assert rel_err(x, xo) <= REL_TOL
 
 
+def test_density_general_mass():
+    """Non-unit masses take the density kernel that reads whole particle
+    records (m * kernel, walls skipped): densities equal the oracle's bit
+    for bit, for kernel supports h*h that are (gn 8) and are not (gn 12)
+    powers of two."""
+    d = gold("p2g")
+    for gn in (8, 12):
+        p, u, t = d[f"p{gn}"], d[f"u{gn}"], d[f"t{gn}"]
+        m = (0.5 + np.random.RandomState(gn).uniform(size=len(t))).astype(np.float32)
+        s = po.Sim(gn)
+        s.set_particles(p, u, np.zeros_like(p), t, dens=np.ones(len(t), np.float32), m=m)
+        s.max_dens = 2.0
+        s.call("sort")
+        s.call("compute_density")
+        with fa.FlipContext(gn) as g:
+            g.upload(p, u, np.zeros_like(p), t, dens=np.ones(len(t), np.float32), mass=m)
+            g.max_dens = 2.0
+            g.stage(fa.STAGE_DENSITY)
+            got, want = g.download()["dens"], s.particles()["dens"]
+        assert np.array_equal(got, want), gn
+        assert (want[t == 1] > 0).any()
+
+
 def test_p2g_general_mass():
     """Non-unit masses take the general P2G kernel (m * sharp weights)."""
     d = gold("p2g")
