@@ -84,7 +84,8 @@ def kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid):
         "density_force": 16 * n_particles + 12 * n_particles + 32 * n_particles + 4 * n_cells,
         "p2g": 32 * n_particles + 4 * n_cells + 2 * 4 * n_faces + n_cells,
         "spring": 16 * n_particles + 4 * n_particles + 16 * n_particles + 4 * n_cells,
-        "resample": 16 * n_particles * 3 + 16 * n_particles + 4 * n_cells,
+        # own pos, pseudo-position, window records (12 B), window velocities -> tmp1
+        "resample": 16 * n_particles * 2 + 12 * n_particles + 16 * n_particles + 16 * n_particles + 4 * n_cells,
         "g2p_flip": 32 * n_particles + 16 * n_particles + 24 * n_faces,
         "advect": 32 * n_particles + 32 * n_particles + 12 * n_faces,
     }
