@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
     }
 }
 
-// corrector.cpp:103-114 update
+// corrector.cpp:103-114 update (tmp1 null: every velocity is kept)
 __global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const float4 *__restrict__ tmp0,
                                  const float4 *__restrict__ tmp1)
 {
@@ -1147,8 +1147,10 @@ __global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const flo
     if (i >= np) return;
     float4 p = pos[i];
     if (is_wall_w(p.w)) return;
-    float4 a = tmp0[i], b = tmp1[i];
+    float4 a = tmp0[i];
     pos[i] = make_float4(a.x, a.y, a.z, p.w);
+    if (!tmp1) return;
+    float4 b = tmp1[i];
     float4 v = vel[i];
     vel[i] = make_float4(b.x, b.y, b.z, v.w);
 }
@@ -1484,9 +1486,17 @@ int k_correct(FlipCtx *c)
                            c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
         FLIP_CHECK(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
     }
-    launch_timed(c, "resample", k_resample, nblk(c->np, RESAMPLE_TB), RESAMPLE_TB, c->np, n, c->P4(), c->V4(),
-                 c->cell_start, c->tmp0, c->tmp1, re * re, reinterpret_cast<const P3 *>(c->xyz3));
+    // With re^2 <= 1e-5 (N >= 159) every resample weight m * (re^2 /
+    // fmax(r2, 1e-5) - 1) clamps to exactly 0 (r2 < re^2 puts the divisor at
+    // 1e-5), so the weight sum is 0 and the reference keeps every velocity
+    // (corrector.cpp:39-44): only the positions change.  Unit masses keep the
+    // products exact zeros.
+    const float hh = re * re;
+    const bool keep = c->unit_mass && (double)hh <= 1.0e-5;
+    if (!keep)
+        launch_timed(c, "resample", k_resample, nblk(c->np, RESAMPLE_TB), RESAMPLE_TB, c->np, n, c->P4(), c->V4(),
+                     c->cell_start, c->tmp0, c->tmp1, hh, reinterpret_cast<const P3 *>(c->xyz3));
     launch_timed(c, "correct_update", k_correct_update, nblk(c->np, TB), TB, c->np, c->P4(), c->V4(), c->tmp0,
-                 c->tmp1);
+                 keep ? nullptr : c->tmp1);
     return events;
 }

@@ -244,6 +244,34 @@ def test_solver_modes_match_oracle(precond):
     assert rel_err(x, xo) <= REL_TOL
 
 
+def test_corrector_fine_grid():
+    """At N=160 the corrector's resample radius satisfies re^2 <= 1e-5, so
+    every resample weight is 0 and velocities are kept (the kernel is
+    skipped): positions and velocities after the corrector stage equal the
+    oracle's, for a dense cluster where springs and resample neighbours
+    are plentiful."""
+    gn = 160
+    rs = np.random.RandomState(5)
+    m = 6000
+    p = (0.4 + 0.03 * rs.uniform(size=(m, 3))).astype(np.float32)
+    u = rs.normal(0, 0.1, size=(m, 3)).astype(np.float32)
+    t = np.full(m, po.FLUID, np.int8)
+    s = po.Sim(gn)
+    s.set_particles(p, u, np.zeros_like(p), t, dens=np.ones(m, np.float32))
+    s.srand(1)
+    with fa.FlipContext(gn) as g:
+        g.upload(p, u, np.zeros_like(p), t, dens=np.ones(m, np.float32))
+        g.srand(1)
+        gst = g.stage(fa.STAGE_CORRECT)
+        a = g.download()
+    sst = s.call_stats("correct")
+    b = s.particles()
+    assert gst["n_jitter"] == sst["n_jitter"]
+    assert np.array_equal(a["p"], b["p"])
+    assert np.array_equal(a["u"], b["u"])
+    assert not np.array_equal(b["p"], p)  # the springs moved particles
+
+
 def test_density_general_mass():
     """Non-unit masses take the density kernel that reads whole particle
     records (m * kernel, walls skipped): densities equal the oracle's bit
