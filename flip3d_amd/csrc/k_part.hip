@@ -114,20 +114,50 @@ __global__ void k_ext_force(int64_t np, float4 *vel, double gdt)
 // sorter.cpp:70-93 markWater + levelset (flip3D.cpp:824-826 recomputes the
 // same L): WALL if the bucket holds a wall particle (L = 1.0), else
 // L = (float)(0.2*n0 - accm) with accm the float sum of dens in bucket order.
-__global__ void k_flags(long C, const int32_t *__restrict__ cs, const float4 *__restrict__ pos,
-                        const float4 *__restrict__ vel, int8_t *A, float *L, double c0)
+// The block's cells hold one contiguous range of particles: it is first
+// copied to LDS with coalesced loads (the wall bit and dens of each), and
+// each thread then walks its bucket there; a range over FLAGS_CAP reads
+// global memory instead.
+constexpr int FLAGS_TB = 256, FLAGS_CAP = 4096;
+
+__global__ __launch_bounds__(FLAGS_TB) void k_flags(long C, const int32_t *__restrict__ cs,
+                                                    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                                    int8_t *A, float *L, double c0)
 {
-    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float s_dens[FLAGS_CAP];
+    __shared__ uint8_t s_wall[FLAGS_CAP];
+    const long cb = (long)blockIdx.x * FLAGS_TB;
+    const long ce = cb + FLAGS_TB < C ? cb + FLAGS_TB : C;
+    const int pb = cs[cb], pe = cs[ce];
+    const bool staged = pe - pb <= FLAGS_CAP;
+    if (staged) {
+        for (int k = threadIdx.x; k < pe - pb; k += FLAGS_TB) {
+            s_dens[k] = vel[pb + k].w;
+            s_wall[k] = is_wall_w(pos[pb + k].w) ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    long c = cb + threadIdx.x;
     if (c >= C) return;
     int b = cs[c], e = cs[c + 1];
     float accm = 0.0f;
     bool wall = false;
-    for (int a = b; a < e; ++a) {
-        if (is_wall_w(pos[a].w)) {
-            wall = true;
-            break;
+    if (staged) {
+        for (int a = b - pb; a < e - pb; ++a) {
+            if (s_wall[a]) {
+                wall = true;
+                break;
+            }
+            accm += s_dens[a];
         }
-        accm += vel[a].w;
+    } else {
+        for (int a = b; a < e; ++a) {
+            if (is_wall_w(pos[a].w)) {
+                wall = true;
+                break;
+            }
+            accm += vel[a].w;
+        }
     }
     if (wall) {
         A[c] = WALL;
@@ -1246,8 +1276,8 @@ void k_launch_flags(FlipCtx *c)
 {
     float d = (float)c->prm.density;
     float n0 = (float)(1.0 / (d * d * d));
-    launch_timed(c, "flags", k_flags, nblk(c->C, TB), TB, c->C, c->cell_start, c->P4(), c->V4(), c->A, c->L,
-                 0.2 * (double)n0);
+    launch_timed(c, "flags", k_flags, nblk(c->C, FLAGS_TB), FLAGS_TB, c->C, c->cell_start, c->P4(), c->V4(), c->A,
+                 c->L, 0.2 * (double)n0);
 }
 
 void k_launch_p2g(FlipCtx *c)
