@@ -178,6 +178,7 @@ void alloc_grids(FlipCtx *c)
     for (int k = 0; k < 3; ++k) {
         dalloc(&c->ug[k], c->F);
         dalloc(&c->us[k], c->F);
+        dalloc(&c->ud[k], c->F);
         FLIP_CHECK(hipMemset(c->ug[k], 0, c->F * sizeof(float)));
         FLIP_CHECK(hipMemset(c->us[k], 0, c->F * sizeof(float)));
     }
@@ -223,6 +224,7 @@ void free_all(FlipCtx *c)
     for (int k = 0; k < 3; ++k) {
         dfree(c->ug[k]);
         dfree(c->us[k]);
+        dfree(c->ud[k]);
     }
     dfree(c->A);
     dfree(c->L);
@@ -909,7 +911,7 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         k_launch_p2g(c);
         run_stage(c, FLIP_STAGE_PROJECT, st);
         k_launch_extrapolate(c);
-        k_launch_g2p(c);
+        k_launch_g2p(c, true);  // reads the extrapolation's (u, u - u_save) pairs
         run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
         if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
         FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));

@@ -76,6 +76,9 @@ struct FlipCtx {
 
     // grids
     float *ug[3] = {nullptr, nullptr, nullptr}, *us[3] = {nullptr, nullptr, nullptr};
+    // (u, u - u_save) per face as written by the extrapolation, for the
+    // step's G2P: one 8-byte load serves both interpolations
+    float2 *ud[3] = {nullptr, nullptr, nullptr};
     int8_t *A = nullptr;
     float *L = nullptr, *press = nullptr, *div = nullptr;
 
@@ -151,7 +154,7 @@ void k_launch_p2g(FlipCtx *c);                  // P2G + save_grid + enforce_bou
 void k_launch_divergence(FlipCtx *c);           // div on FLUID cells
 void k_launch_gradient(FlipCtx *c);             // ghost-fluid gradient + enforce_boundary
 void k_launch_extrapolate(FlipCtx *c);          // extrapolate_velocity + subtract_grid
-void k_launch_g2p(FlipCtx *c);                  // FLIP/PIC blend
+void k_launch_g2p(FlipCtx *c, bool fresh = false);  // FLIP/PIC blend; fresh: right after k_launch_extrapolate
 void k_launch_advect(FlipCtx *c);               // advect + clamp + wall push-out
 int k_stuck_and_reposition(FlipCtx *c);         // returns repositioned count (host syncs)
 int k_correct(FlipCtx *c);                      // corrector; returns jitter events

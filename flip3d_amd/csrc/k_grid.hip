@@ -86,7 +86,7 @@ __device__ __forceinline__ void face_marks(int n, const int8_t *__restrict__ A, 
 // u_save = u - u_save.  Extrapolated faces read only marked faces, which are
 // never written here.
 template <int COMP>
-__global__ void k_extrapolate(int n, const int8_t *__restrict__ A, float *u, float *us)
+__global__ void k_extrapolate(int n, const int8_t *__restrict__ A, float *u, float *us, float2 *ud)
 {
     const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
     long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,7 +118,9 @@ __global__ void k_extrapolate(int n, const int8_t *__restrict__ A, float *u, flo
             u[f] = v;
         }
     }
-    us[f] = v - us[f];
+    const float d = v - us[f];
+    us[f] = d;
+    ud[f] = make_float2(v, d);
 }
 
 }  // namespace
@@ -146,8 +148,8 @@ void k_launch_extrapolate(FlipCtx *c)
 {
     t_begin(c, "extrapolate");
     unsigned nb = nblk(c->F, TB);
-    hipLaunchKernelGGL(k_extrapolate<0>, nb, TB, 0, c->stream, c->n, c->A, c->ug[0], c->us[0]);
-    hipLaunchKernelGGL(k_extrapolate<1>, nb, TB, 0, c->stream, c->n, c->A, c->ug[1], c->us[1]);
-    hipLaunchKernelGGL(k_extrapolate<2>, nb, TB, 0, c->stream, c->n, c->A, c->ug[2], c->us[2]);
+    hipLaunchKernelGGL(k_extrapolate<0>, nb, TB, 0, c->stream, c->n, c->A, c->ug[0], c->us[0], c->ud[0]);
+    hipLaunchKernelGGL(k_extrapolate<1>, nb, TB, 0, c->stream, c->n, c->A, c->ug[1], c->us[1], c->ud[1]);
+    hipLaunchKernelGGL(k_extrapolate<2>, nb, TB, 0, c->stream, c->n, c->A, c->ug[2], c->us[2], c->ud[2]);
     t_end(c);
 }

@@ -500,6 +500,41 @@ __device__ __forceinline__ float linear(const float *__restrict__ q, int n, floa
     return kz * lo + zk * hi;
 }
 
+// linear() of the two grids interleaved in q (.x and .y), same arithmetic
+template <int COMP>
+__device__ __forceinline__ float2 linear2(const float2 *__restrict__ q, int n, float x, float y, float z)
+{
+    const int w = n + (COMP == 0), h = n + (COMP == 1), d = n + (COMP == 2);
+    x = fmaxf(0.0f, fminf((float)w, x));
+    y = fmaxf(0.0f, fminf((float)h, y));
+    z = fmaxf(0.0f, fminf((float)d, z));
+    int i = (int)(x > (float)(w - 2) ? (float)(w - 2) : x);
+    int j = (int)(y > (float)(h - 2) ? (float)(h - 2) : y);
+    int k = (int)(z > (float)(h - 2) ? (float)(h - 2) : z);
+    float ix = (float)(i + 1) - x, xi = x - (float)i;
+    float jy = (float)(j + 1) - y, yj = y - (float)j;
+    float kz = (float)(k + 1) - z, zk = z - (float)k;
+    auto at = [&](int a, int b, int c) -> float2 {
+        if (c >= d) return make_float2(0.0f, 0.0f);
+        return q[COMP == 0 ? fidx0(n, a, b, c) : (COMP == 1 ? fidx1(n, a, b, c) : fidx2(n, a, b, c))];
+    };
+    const float2 q000 = at(i, j, k), q100 = at(i + 1, j, k), q010 = at(i, j + 1, k), q110 = at(i + 1, j + 1, k);
+    const float2 q001 = at(i, j, k + 1), q101 = at(i + 1, j, k + 1), q011 = at(i, j + 1, k + 1),
+                 q111 = at(i + 1, j + 1, k + 1);
+    float2 r;
+    {
+        float lo = (ix * q000.x + xi * q100.x) * jy + (ix * q010.x + xi * q110.x) * yj;
+        float hi = (ix * q001.x + xi * q101.x) * jy + (ix * q011.x + xi * q111.x) * yj;
+        r.x = kz * lo + zk * hi;
+    }
+    {
+        float lo = (ix * q000.y + xi * q100.y) * jy + (ix * q010.y + xi * q110.y) * yj;
+        float hi = (ix * q001.y + xi * q101.y) * jy + (ix * q011.y + xi * q111.y) * yj;
+        r.y = kz * lo + zk * hi;
+    }
+    return r;
+}
+
 // mapper.cpp:108-112 fetchVelocity; gn*p - 0.5 in double rounds like the
 // float subtraction (one rounding of an exact value).
 __device__ __forceinline__ float3 fetch_velocity(const float *g0, const float *g1, const float *g2, int n, float px,
@@ -533,6 +568,28 @@ __global__ __launch_bounds__(TB) void k_g2p_flip(int64_t np, int n, const float4
     v.x = (float)(c_pic * (double)pic.x + c_flip * (double)tx);
     v.y = (float)(c_pic * (double)pic.y + c_flip * (double)ty);
     v.z = (float)(c_pic * (double)pic.z + c_flip * (double)tz);
+    vel[i] = v;
+}
+
+// k_g2p_flip reading the extrapolation's interleaved (u, u - u_save) faces
+__global__ __launch_bounds__(TB) void k_g2p_flip2(int64_t np, int n, const float4 *__restrict__ pos,
+                                                  float4 *__restrict__ vel, const float2 *__restrict__ q0,
+                                                  const float2 *__restrict__ q1, const float2 *__restrict__ q2,
+                                                  double c_pic, double c_flip)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float4 p = pos[i];
+    if (is_wall_w(p.w)) return;
+    float4 v = vel[i];
+    const float gx = (float)n * p.x, gy = (float)n * p.y, gz = (float)n * p.z;
+    const float2 a = linear2<0>(q0, n, gx, gy - 0.5f, gz - 0.5f);
+    const float2 b = linear2<1>(q1, n, gx - 0.5f, gy, gz - 0.5f);
+    const float2 c = linear2<2>(q2, n, gx - 0.5f, gy - 0.5f, gz);
+    float tx = a.y + v.x, ty = b.y + v.y, tz = c.y + v.z;
+    v.x = (float)(c_pic * (double)a.x + c_flip * (double)tx);
+    v.y = (float)(c_pic * (double)b.x + c_flip * (double)ty);
+    v.z = (float)(c_pic * (double)c.x + c_flip * (double)tz);
     vel[i] = v;
 }
 
@@ -1319,10 +1376,14 @@ void k_launch_p2g(FlipCtx *c)
     t_end(c);
 }
 
-void k_launch_g2p(FlipCtx *c)
+void k_launch_g2p(FlipCtx *c, bool fresh)
 {
     double a = c->prm.alpha;
-    if (c->np > 0)
+    if (c->np == 0) return;
+    if (fresh)  // the extrapolation has just written ud
+        launch_timed(c, "g2p_flip", k_g2p_flip2, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ud[0],
+                     c->ud[1], c->ud[2], 1.0 - a, a);
+    else
         launch_timed(c, "g2p_flip", k_g2p_flip, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0],
                      c->ug[1], c->ug[2], c->us[0], c->us[1], c->us[2], 1.0 - a, a);
 }
