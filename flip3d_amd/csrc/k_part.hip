@@ -735,50 +735,88 @@ __global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const fl
 // new positions).  Particles of one level never read each other, and every
 // particle they read from the list is on a lower level (earlier in the list,
 // new value) or a higher one (later in the list, old value), exactly as in
-// the serial loop.  One block walks the levels; within a level each thread
-// resamples one particle with the reference's serial window sum.
-__global__ __launch_bounds__(1024) void k_resample_levels(int nlev, const int32_t *__restrict__ loff,
-                                                          const int32_t *__restrict__ order,
-                                                          const int32_t *__restrict__ ids,
-                                                          const int32_t *__restrict__ inv, int n,
-                                                          const float4 *__restrict__ pos, float4 *vel,
-                                                          const int32_t *__restrict__ cs, float hh)
+// the serial loop.  One block walks the levels; within a level each wave
+// resamples one particle: its lanes form the window's terms (w u, w) in
+// parallel into LDS, in window order, and lane 0 adds them up in that order,
+// so the sums are the reference's serial ones (a skipped particle stores +0
+// terms, which leave the sums unchanged: they start at +0 and never become
+// -0).
+constexpr int RL_WAVES = 16, RL_CAP = 384;
+
+__global__ __launch_bounds__(64 * RL_WAVES) void k_resample_levels(int nlev, const int32_t *__restrict__ loff,
+                                                                   const int32_t *__restrict__ order,
+                                                                   const int32_t *__restrict__ ids,
+                                                                   const int32_t *__restrict__ inv, int n,
+                                                                   const float4 *__restrict__ pos, float4 *vel,
+                                                                   const int32_t *__restrict__ cs, float hh)
 {
+    __shared__ float4 terms[RL_WAVES][RL_CAP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int l = 0; l < nlev; ++l) {
-        for (int x = loff[l] + (int)threadIdx.x; x < loff[l + 1]; x += blockDim.x) {
+        for (int x = loff[l] + wave; x < loff[l + 1]; x += RL_WAVES) {
             const int32_t slot = inv[ids[order[x]]];
             const float4 p = pos[slot];
             const int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
             const int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+            int rb[9], rl[9], total = 0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const int si = ci - 1 + r / 3, sj = cj - 1 + r % 3;
+                const bool ok = si >= 0 && si <= n - 1 && sj >= 0 && sj <= n - 1;
+                const long c0 = ok ? cidx(n, si, sj, klo) : 0;
+                const int b = cs[c0], e = cs[c0 + (ok ? khi - klo : 0) + 1];
+                rb[r] = b;
+                rl[r] = ok ? e - b : 0;
+                total += rl[r];
+            }
             float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
-            for (int si = ci - 1; si <= ci + 1; ++si) {
-                if (si < 0 || si > n - 1) continue;
-                for (int sj = cj - 1; sj <= cj + 1; ++sj) {
-                    if (sj < 0 || sj > n - 1) continue;
-                    const int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-                    for (int a = b; a < e; ++a) {
-                        const float4 q = pos[a];
-                        if (is_wall_w(q.w)) continue;
-                        const float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
-                        if (!(d2 < hh)) continue;
+            for (int base = 0; base < total; base += RL_CAP) {
+                const int cnt = min(RL_CAP, total - base);
+                for (int k = lane; k < cnt; k += 64) {
+                    // window position base + k -> storage slot (rows in window order)
+                    int rem = base + k, a = -1;
+#pragma unroll
+                    for (int r = 0; r < 9; ++r) {
+                        if (a < 0) {
+                            if (rem < rl[r]) a = rb[r] + rem;
+                            else rem -= rl[r];
+                        }
+                    }
+                    float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    const float4 q = pos[a];
+                    const float d2 = length2(p.x, p.y, p.z, q.x, q.y, q.z);
+                    if (!is_wall_w(q.w) && d2 < hh) {
                         const float w = q.w * sharp_kernel_hh(d2, hh);
                         const float4 u = vel[a];
-                        ax += w * u.x;
-                        ay += w * u.y;
-                        az += w * u.z;
-                        wsum += w;
+                        t = make_float4(w * u.x, w * u.y, w * u.z, w);
+                    }
+                    terms[wave][k] = t;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) {
+                    for (int k = 0; k < cnt; ++k) {
+                        const float4 t = terms[wave][k];
+                        ax += t.x;
+                        ay += t.y;
+                        az += t.z;
+                        wsum += t.w;
                     }
                 }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
-            float4 v = vel[slot];
-            if (wsum != 0.0f) {
-                v.x = ax / wsum;
-                v.y = ay / wsum;
-                v.z = az / wsum;
-            } else {
-                v.x = v.y = v.z = 0.0f;
+            if (lane == 0) {
+                float4 v = vel[slot];
+                if (wsum != 0.0f) {
+                    v.x = ax / wsum;
+                    v.y = ay / wsum;
+                    v.z = az / wsum;
+                } else {
+                    v.x = v.y = v.z = 0.0f;
+                }
+                vel[slot] = v;
             }
-            vel[slot] = v;
         }
         __threadfence_block();
         __syncthreads();
@@ -1520,7 +1558,7 @@ int k_stuck_and_reposition(FlipCtx *c)
             FLIP_CHECK(hipMemcpyAsync(c->ev_off, loff.data(), sizeof(int32_t) * (nlev + 1), hipMemcpyHostToDevice,
                                       c->stream));
             FLIP_CHECK(hipMemcpyAsync(c->deep, order.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL(k_resample_levels, 1, 1024, 0, c->stream, nlev, c->ev_off, c->deep, c->ev_list, c->inv,
+            hipLaunchKernelGGL(k_resample_levels, 1, 64 * RL_WAVES, 0, c->stream, nlev, c->ev_off, c->deep, c->ev_list, c->inv,
                                n, c->P4(), c->V4(), c->cell_start, hh);
             t_end(c);
             FLIP_CHECK(hipStreamSynchronize(c->stream));
