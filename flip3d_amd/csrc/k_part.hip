@@ -1130,25 +1130,16 @@ __global__ void k_spring_count(int m, const int32_t *__restrict__ list, int n, c
     const float far2 = re * re * 1.0001f;  // as in spring_one
     const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
     int ev = 0;
-    for (int si = ci - 1; si <= ci + 1; ++si) {
-        if (si < 0 || si > n - 1) continue;
-        for (int sj = cj - 1; sj <= cj + 1; ++sj) {
-            if (sj < 0 || sj > n - 1) continue;
-            float dij;
-            int klo, khi;
-            if (!wc.row(ci, cj, si, sj, dij)) continue;
-            wc.cells(n, ck, dij, klo, khi);
-            if (klo > khi) continue;
-            int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            for (int a = b; a < e; ++a) {
-                if (a == i) continue;
-                float4 q = pos[a];
-                if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) continue;
-                float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
-                if (!((double)dist > rre) && !is_wall_w(q.w)) ev++;
-            }
-        }
-    }
+    int rlo[9], rhi[9];
+    window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
+            if (a == i) return;
+            if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
+            float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+            if (!((double)dist > rre) && !is_wall_w(q.w)) ev++;
+        });
     events[t] = ev;
 }
 
