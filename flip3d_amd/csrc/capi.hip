@@ -254,6 +254,8 @@ void free_all(FlipCtx *c)
     c->h_part64 = nullptr;
     if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
     c->pcg_graph = nullptr;
+    if (c->ic_graph) (void)hipGraphExecDestroy(c->ic_graph);
+    c->ic_graph = nullptr;
     learned_free(c->learned);
     c->learned = nullptr;
     mesh_free(c->mesh);

@@ -93,6 +93,8 @@ struct FlipCtx {
     uint32_t *flmeta = nullptr;  // per FLUID cell: neighbour kinds (2 bits x 6) + has-AIR bit
     hipGraphExec_t pcg_graph = nullptr;  // captured batch of PCG iterations
     int pcg_graph_subcell = -1;
+    hipGraphExec_t ic_graph = nullptr;  // captured batch of wavefront-preconditioned iterations
+    int ic_graph_key = -1;              // subcell * 2 + preconditioner mode it was captured for
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
     // FP64 solve / iterative refinement vectors (allocated on first use)
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr, *s64 = nullptr, *q64 = nullptr;

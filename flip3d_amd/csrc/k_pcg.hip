@@ -32,9 +32,9 @@
 // every coupling term is a signed zero unless some P == 1.0 exactly, so M is
 // applied per cell, keeping the signed zeros so even the sign of a zero
 // result matches.  If a P == 1.0 occurs (or for the flag-keyed IC(0) mode)
-// the triangular solves run as i+j+k hyperplane wavefronts; each cell's
-// formula and operand order are the reference's, so any topological order
-// gives identical results.
+// the factorization and the triangular solves run as i+j+k hyperplane
+// wavefronts over 8^3 tiles; each cell's formula and operand order are the
+// reference's, so any topological order gives identical results.
 #include "pcg_device.hpp"
 
 #include <algorithm>
@@ -45,7 +45,8 @@ using namespace pcgdev;
 
 namespace {
 
-constexpr int BATCH = 8;  // iterations per captured graph
+constexpr int BATCH = 8;   // iterations per captured graph (per-cell preconditioner)
+constexpr int WBATCH = 2;  // iterations per captured graph (wavefront sweeps)
 
 
 // ---------------------------------------------------------------- setup
@@ -341,6 +342,7 @@ __global__ __launch_bounds__(TB) void k_pcg_zr(int nfl, const int32_t *__restric
                                                const int32_t *__restrict__ iflags)
 {
     if (iflags[I_DONE]) return;
+    nfl = iflags[I_NFL];  // the FLUID count lives in device memory (graph replays)
     double zr = 0.0;
     for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
@@ -370,108 +372,176 @@ __global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb,
 }
 
 // z from the wavefront preconditioner into the .x slots of a (z, s) buffer
+// (with iflags, the FLUID count is read from device memory: graph replays)
 __global__ void k_copy_fluid_x(int nfl, const int32_t *__restrict__ fl, const float *__restrict__ z, float2 *B,
                                const int32_t *__restrict__ iflags)
 {
     if (iflags && iflags[I_DONE]) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nfl) B[fl[t]].x = z[fl[t]];
-}
-
-// ------------------------------------------------ wavefront preconditioner
-// P_ref (solver.cpp:119-123): mode 0 value test (== 1.0), mode 1 flag test
-__device__ __forceinline__ float pref_d(int mode, int n, const int8_t *A, const double *P, int i, int j, int k)
-{
-    if (i < 0 || i > n - 1 || j < 0 || j > n - 1 || k < 0 || k > n - 1) return 0.0f;
-    long c = cidx(n, i, j, k);
-    if (mode == 1) return A[c] == FLUID ? (float)P[c] : 0.0f;
-    if (P[c] != 1.0) return 0.0f;
-    return (float)P[c];
-}
-__device__ __forceinline__ float pref_f(int mode, int n, const int8_t *A, const float *z, int i, int j, int k)
-{
-    if (i < 0 || i > n - 1 || j < 0 || j > n - 1 || k < 0 || k > n - 1) return 0.0f;
-    long c = cidx(n, i, j, k);
-    if (mode == 1) return A[c] == FLUID ? z[c] : 0.0f;
-    if (z[c] != 1.0f) return 0.0f;
-    return z[c];
-}
-// solver.cpp:97-101 A_ref
-__device__ __forceinline__ float aref(int n, const int8_t *A, int i, int j, int k, int qi, int qj, int qk)
-{
-    if (i < 0 || i > n - 1 || j < 0 || j > n - 1 || k < 0 || k > n - 1 || A[cidx(n, i, j, k)] != FLUID) return 0.0f;
-    if (qi < 0 || qi > n - 1 || qj < 0 || qj > n - 1 || qk < 0 || qk > n - 1 || A[cidx(n, qi, qj, qk)] != FLUID)
-        return 0.0f;
-    return -1.0f;
-}
-
-__device__ __forceinline__ bool plane_cell(int n, int d, int &i, int &j, int &k)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    i = t / n;
-    j = t % n;
-    k = d - i - j;
-    return i < n && k >= 0 && k < n;
-}
-
-// buildPreconditioner on hyperplane d (solver.cpp:125-139)
-__global__ void k_ic_build_plane(int n, int d, int mode, const int8_t *A, const float *L, int subcell, double *P)
-{
-    int i, j, k;
-    if (!plane_cell(n, d, i, j, k)) return;
-    long c = cidx(n, i, j, k);
-    if (A[c] != FLUID) {
-        P[c] = 0.0;
-        return;
-    }
-    float l = aref(n, A, i - 1, j, k, i, j, k) * pref_d(mode, n, A, P, i - 1, j, k);
-    float bo = aref(n, A, i, j - 1, k, i, j, k) * pref_d(mode, n, A, P, i, j - 1, k);
-    float ba = aref(n, A, i, j, k - 1, i, j, k) * pref_d(mode, n, A, P, i, j, k - 1);
-    double diag = (double)a_diag(n, c, A, L, subcell);
-    double e = diag - (double)(l * l) - (double)(bo * bo) - (double)(ba * ba);
-    if (e < 0.25 * diag) e = diag;
-    P[c] = 1.0 / (double)sqrtf((float)e);
-}
-
-// forward sweep on plane d (solver.cpp:147-160)
-__global__ void k_ic_fwd_plane(int n, int d, int mode, const int8_t *A, const double *P, const float *r, double *qv)
-{
-    int i, j, k;
-    if (!plane_cell(n, d, i, j, k)) return;
-    long c = cidx(n, i, j, k);
-    if (A[c] != FLUID) {
-        qv[c] = 0.0;
-        return;
-    }
-    double left = (double)(aref(n, A, i - 1, j, k, i, j, k) * pref_d(mode, n, A, P, i - 1, j, k) *
-                           pref_d(mode, n, A, qv, i - 1, j, k));
-    double bottom = (double)(aref(n, A, i, j - 1, k, i, j, k) * pref_d(mode, n, A, P, i, j - 1, k) *
-                             pref_d(mode, n, A, qv, i, j - 1, k));
-    double back = (double)(aref(n, A, i, j, k - 1, i, j, k) * pref_d(mode, n, A, P, i, j, k - 1) *
-                           pref_d(mode, n, A, qv, i, j, k - 1));
-    double t = (double)r[c] - left - bottom - back;
-    qv[c] = t * P[c];
-}
-
-// backward sweep on plane d (solver.cpp:163-176)
-__global__ void k_ic_bwd_plane(int n, int d, int mode, const int8_t *A, const double *P, const double *qv, float *z)
-{
-    int i, j, k;
-    if (!plane_cell(n, d, i, j, k)) return;
-    long c = cidx(n, i, j, k);
-    if (A[c] != FLUID) return;
-    float ps = pref_d(mode, n, A, P, i, j, k);
-    double right = (double)(aref(n, A, i, j, k, i + 1, j, k) * ps * pref_f(mode, n, A, z, i + 1, j, k));
-    double top = (double)(aref(n, A, i, j, k, i, j + 1, k) * ps * pref_f(mode, n, A, z, i, j + 1, k));
-    double front = (double)(aref(n, A, i, j, k, i, j, k + 1) * ps * pref_f(mode, n, A, z, i, j, k + 1));
-    double t = qv[c] - right - top - front;
-    z[c] = (float)(t * P[c]);
+    if (iflags) nfl = iflags[I_NFL];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) B[fl[t]].x = z[fl[t]];
 }
 
 __global__ void k_fill_f64(long n, double *a, double v)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
+}
+
+// ----------------------------------------- tiled hyperplane wavefront
+// The factorization and both sweeps (solver.cpp:125-180) over 8^3 tiles.
+// Tile (I, J, K) depends only on tiles (I-1, J, K), (I, J-1, K) and
+// (I, J, K-1) in the forward direction, so the tiles of one tile-plane
+// I + J + K = D are independent: one launch per tile-plane (3 ceil(n/8) - 2
+// per sweep, 46 at n = 128, against 3n - 2 cell planes).  Inside a tile,
+// one workgroup walks the 22 local planes a + b + c = p with a barrier
+// between planes; in-tile neighbours come from LDS, neighbours in earlier
+// tiles from global memory (written by earlier launches).  Every cell's
+// formula and operand order are the reference's (solver.cpp:127-135,
+// 149-158, 165-174), so the result is the reference's bit for bit (any
+// topological order of the cells gives the same values).
+constexpr int WT = 8;                   // tile edge
+constexpr int WT3 = WT * WT * WT;       // threads per tile
+constexpr int WPL = 3 * (WT - 1) + 1;   // local planes per tile
+
+__device__ __forceinline__ float pref_v(int mode, bool fluid, double v)
+{
+    if (mode == 1) return fluid ? (float)v : 0.0f;
+    if (v != 1.0) return 0.0f;
+    return (float)v;
+}
+__device__ __forceinline__ float pref_vf(int mode, bool fluid, float v)
+{
+    if (mode == 1) return fluid ? v : 0.0f;
+    if (v != 1.0f) return 0.0f;
+    return v;
+}
+
+// OP 0: factorization (P), 1: forward sweep (qv from r), 2: backward sweep (z from qv)
+// (the sweeps return at once after the solve has stopped: iflags[I_DONE])
+template <int OP>
+__global__ __launch_bounds__(WT3) void k_ic_tile(int n, int nt, int D, int mode, const int8_t *__restrict__ A,
+                                                 const float *__restrict__ L, int subcell, double *P,
+                                                 const float *__restrict__ r, double *qv, float *z,
+                                                 const int32_t *__restrict__ iflags)
+{
+    __shared__ double sv[WT3];  // P (OP 0) or q (OP 1) of the tile's cells
+    __shared__ float sz[WT3];   // z (OP 2)
+    __shared__ int8_t sf[WT3];  // FLUID flags
+    const int I = (int)blockIdx.x / nt, J = (int)blockIdx.x % nt, K = D - I - J;
+    if (K < 0 || K >= nt) return;  // uniform over the block
+    if (iflags && iflags[I_DONE]) return;
+    const int t = threadIdx.x;
+    const int a = t / (WT * WT), b = (t / WT) % WT, cz = t % WT;
+    const int i = I * WT + a, j = J * WT + b, k = K * WT + cz;
+    const bool in = i < n && j < n && k < n;
+    const long nn = (long)n * n;
+    const long c = in ? cidx(n, i, j, k) : 0;
+    const bool fluid = in && A[c] == FLUID;
+    sf[t] = fluid ? 1 : 0;
+    if (OP == 2) sz[t] = in ? z[c] : 0.0f;  // non-FLUID z is never written by the sweep
+    __syncthreads();
+    const int p = OP == 2 ? (WPL - 1) - (a + b + cz) : a + b + cz;
+    for (int s = 0; s < WPL; ++s) {
+        if (s == p && in) {
+            if (OP == 0) {
+                if (!fluid) {
+                    P[c] = 0.0;
+                    sv[t] = 0.0;
+                } else {
+                    // solver.cpp:129-131: a neighbour term is live only for a FLUID neighbour
+                    float pl = 0.0f, pb = 0.0f, pk = 0.0f, al = 0.0f, ab = 0.0f, ak = 0.0f;
+                    if (i > 0) {
+                        bool f = a > 0 ? sf[t - WT * WT] != 0 : A[c - nn] == FLUID;
+                        al = f ? -1.0f : 0.0f;
+                        pl = pref_v(mode, f, a > 0 ? sv[t - WT * WT] : P[c - nn]);
+                    }
+                    if (j > 0) {
+                        bool f = b > 0 ? sf[t - WT] != 0 : A[c - n] == FLUID;
+                        ab = f ? -1.0f : 0.0f;
+                        pb = pref_v(mode, f, b > 0 ? sv[t - WT] : P[c - n]);
+                    }
+                    if (k > 0) {
+                        bool f = cz > 0 ? sf[t - 1] != 0 : A[c - 1] == FLUID;
+                        ak = f ? -1.0f : 0.0f;
+                        pk = pref_v(mode, f, cz > 0 ? sv[t - 1] : P[c - 1]);
+                    }
+                    float l = al * pl, bo = ab * pb, ba = ak * pk;
+                    double diag = (double)a_diag(n, c, A, L, subcell);
+                    double e = diag - (double)(l * l) - (double)(bo * bo) - (double)(ba * ba);
+                    if (e < 0.25 * diag) e = diag;
+                    double pv = 1.0 / (double)sqrtf((float)e);
+                    P[c] = pv;
+                    sv[t] = pv;
+                }
+            } else if (OP == 1) {
+                if (!fluid) {
+                    qv[c] = 0.0;
+                    sv[t] = 0.0;
+                } else {
+                    float al = 0.0f, ab = 0.0f, ak = 0.0f, pl = 0.0f, pb = 0.0f, pk = 0.0f, ql = 0.0f, qb = 0.0f,
+                          qk = 0.0f;
+                    if (i > 0) {
+                        bool f = a > 0 ? sf[t - WT * WT] != 0 : A[c - nn] == FLUID;
+                        al = f ? -1.0f : 0.0f;
+                        pl = pref_v(mode, f, P[c - nn]);
+                        ql = pref_v(mode, f, a > 0 ? sv[t - WT * WT] : qv[c - nn]);
+                    }
+                    if (j > 0) {
+                        bool f = b > 0 ? sf[t - WT] != 0 : A[c - n] == FLUID;
+                        ab = f ? -1.0f : 0.0f;
+                        pb = pref_v(mode, f, P[c - n]);
+                        qb = pref_v(mode, f, b > 0 ? sv[t - WT] : qv[c - n]);
+                    }
+                    if (k > 0) {
+                        bool f = cz > 0 ? sf[t - 1] != 0 : A[c - 1] == FLUID;
+                        ak = f ? -1.0f : 0.0f;
+                        pk = pref_v(mode, f, P[c - 1]);
+                        qk = pref_v(mode, f, cz > 0 ? sv[t - 1] : qv[c - 1]);
+                    }
+                    double left = (double)(al * pl * ql);
+                    double bottom = (double)(ab * pb * qb);
+                    double back = (double)(ak * pk * qk);
+                    double tt = (double)r[c] - left - bottom - back;
+                    double q = tt * P[c];
+                    qv[c] = q;
+                    sv[t] = q;
+                }
+            } else if (fluid) {
+                const float ps = pref_v(mode, true, P[c]);
+                float ar = 0.0f, at = 0.0f, af = 0.0f, zr = 0.0f, zt = 0.0f, zf = 0.0f;
+                if (i < n - 1) {
+                    bool f = a < WT - 1 ? sf[t + WT * WT] != 0 : A[c + nn] == FLUID;
+                    ar = f ? -1.0f : 0.0f;
+                    zr = pref_vf(mode, f, a < WT - 1 ? sz[t + WT * WT] : z[c + nn]);
+                }
+                if (j < n - 1) {
+                    bool f = b < WT - 1 ? sf[t + WT] != 0 : A[c + n] == FLUID;
+                    at = f ? -1.0f : 0.0f;
+                    zt = pref_vf(mode, f, b < WT - 1 ? sz[t + WT] : z[c + n]);
+                }
+                if (k < n - 1) {
+                    bool f = cz < WT - 1 ? sf[t + 1] != 0 : A[c + 1] == FLUID;
+                    af = f ? -1.0f : 0.0f;
+                    zf = pref_vf(mode, f, cz < WT - 1 ? sz[t + 1] : z[c + 1]);
+                }
+                double right = (double)(ar * ps * zr);
+                double top = (double)(at * ps * zt);
+                double front = (double)(af * ps * zf);
+                double tt = qv[c] - right - top - front;
+                float zv = (float)(tt * P[c]);
+                z[c] = zv;
+                sz[t] = zv;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// cells with P == 1.0 exactly (the value test's live factors)
+__global__ void k_count_unit(int nfl, const int32_t *__restrict__ fl, const double *__restrict__ P, int32_t *iflags)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nfl && P[fl[t]] == 1.0) atomicAdd(&iflags[I_UNIT], 1);
 }
 
 
@@ -504,14 +574,28 @@ int k_build_fluid_list(FlipCtx *c)
     return c->nfl;
 }
 
-static void precond_wavefront(FlipCtx *c, int mode, const float *r, float *z)
+// the triangular solves (solver.cpp:141-180) as tile-plane launches;
+// iflags (may be null) ends them once the solve has stopped
+static void precond_wavefront(FlipCtx *c, int mode, const float *r, float *z, const int32_t *iflags)
 {
-    int n = c->n;
-    unsigned nb = nblk((int64_t)n * n, TB);
-    for (int d = 0; d <= 3 * n - 3; ++d)
-        hipLaunchKernelGGL(k_ic_fwd_plane, nb, TB, 0, c->stream, n, d, mode, c->A, c->P, r, c->qv);
-    for (int d = 3 * n - 3; d >= 0; --d)
-        hipLaunchKernelGGL(k_ic_bwd_plane, nb, TB, 0, c->stream, n, d, mode, c->A, c->P, c->qv, z);
+    const int n = c->n, nt = (n + WT - 1) / WT;
+    const unsigned nb = (unsigned)(nt * nt);
+    for (int d = 0; d <= 3 * nt - 3; ++d)
+        hipLaunchKernelGGL(k_ic_tile<1>, nb, WT3, 0, c->stream, n, nt, d, mode, c->A, c->L, 0, c->P, r, c->qv,
+                           z, iflags);
+    for (int d = 3 * nt - 3; d >= 0; --d)
+        hipLaunchKernelGGL(k_ic_tile<2>, nb, WT3, 0, c->stream, n, nt, d, mode, c->A, c->L, 0, c->P, r, c->qv,
+                           z, iflags);
+}
+
+// buildPreconditioner (solver.cpp:125-139) as tile-plane launches
+static void build_factor_wavefront(FlipCtx *c, int mode, int subcell)
+{
+    const int n = c->n, nt = (n + WT - 1) / WT;
+    hipLaunchKernelGGL(k_fill_f64, nblk(c->C, TB), TB, 0, c->stream, c->C, c->P, 0.0);
+    for (int d = 0; d <= 3 * nt - 3; ++d)
+        hipLaunchKernelGGL(k_ic_tile<0>, (unsigned)(nt * nt), WT3, 0, c->stream, n, nt, d, mode, c->A, c->L,
+                           subcell, c->P, nullptr, c->qv, nullptr, nullptr);
 }
 
 // PCG on the context's A, L, press (x) and div (b, negated in place).  With
@@ -560,10 +644,14 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
         jacobi = unit == 0;
     }
     if (!jacobi) {
-        hipLaunchKernelGGL(k_fill_f64, nblk(C, TB), TB, 0, c->stream, C, c->P, 0.0);
-        unsigned pb2 = nblk((int64_t)n * n, TB);
-        for (int d = 0; d <= 3 * n - 3; ++d)
-            hipLaunchKernelGGL(k_ic_build_plane, pb2, TB, 0, c->stream, n, d, mode, c->A, c->L, subcell, c->P);
+        build_factor_wavefront(c, mode, subcell);
+        // the count of live factors (P == 1.0) of the whole factorization
+        int32_t zero = 0, unit = 0;
+        FLIP_CHECK(hipMemcpyAsync(c->iflags + I_UNIT, &zero, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_count_unit, nb, TB, 0, c->stream, nfl, c->fl, c->P, c->iflags);
+        FLIP_CHECK(hipMemcpyAsync(&unit, c->iflags + I_UNIT, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        FLIP_CHECK(hipStreamSynchronize(c->stream));
+        if (st) st->n_precond_unit = unit;
     }
     // the initial z goes to B_1.x, where pcg_a(0) reads it
     if (jacobi) {
@@ -572,7 +660,7 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
     } else {
         hipLaunchKernelGGL(k_pcg_init<false>, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
                            c->press, c->div, c->P, c->r, c->zs[1], pi);
-        precond_wavefront(c, mode, c->r, c->z);
+        precond_wavefront(c, mode, c->r, c->z, nullptr);
         hipLaunchKernelGGL(k_copy_fluid_x, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[1], nullptr);
         hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pi, 3, 1, c->iflags);
     }
@@ -610,12 +698,32 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
         }
     }
     if (st) st->learned_iterations = learned_iters;
-    // Jacobi iterations run as replays of a captured graph of BATCH
-    // iterations plus the stop check; profiling, direct launch or the
-    // wavefront path launch kernel by kernel (one iteration per host check
-    // when profiling).
-    const bool use_graph = jacobi && !c->profiling && !c->direct_launch;
-    if (use_graph && (!c->pcg_graph || c->pcg_graph_subcell != subcell)) {
+    // Iterations run as replays of a captured graph of BATCH iterations
+    // (Jacobi) or WBATCH iterations (wavefront sweeps) plus the stop check;
+    // profiling or direct launch launch kernel by kernel (one iteration per
+    // host check when profiling).
+    const bool use_graph = !c->profiling && !c->direct_launch;
+    const int wkey = subcell * 2 + mode;
+    if (use_graph && !jacobi && (!c->ic_graph || c->ic_graph_key != wkey)) {
+        if (c->ic_graph) {
+            (void)hipGraphExecDestroy(c->ic_graph);
+            c->ic_graph = nullptr;
+        }
+        hipGraph_t g;
+        FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int t = 0; t < WBATCH; ++t) {
+            launch_pair(c, false, G, h2, subcell, pa, pb);
+            precond_wavefront(c, mode, c->r, c->z, c->iflags);
+            hipLaunchKernelGGL(k_copy_fluid_x, G, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[t & 1], c->iflags);
+            hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
+        }
+        hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G);
+        FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
+        FLIP_CHECK(hipGraphInstantiate(&c->ic_graph, g, nullptr, nullptr, 0));
+        FLIP_CHECK(hipGraphDestroy(g));
+        c->ic_graph_key = wkey;
+    }
+    if (use_graph && jacobi && (!c->pcg_graph || c->pcg_graph_subcell != subcell)) {
         if (c->pcg_graph) {
             (void)hipGraphExecDestroy(c->pcg_graph);
             c->pcg_graph = nullptr;
@@ -633,14 +741,14 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
     int hk = 0;  // host mirror of the iteration index (parity of the z buffer)
     while (!done) {
         if (use_graph) {
-            FLIP_CHECK(hipGraphLaunch(c->pcg_graph, c->stream));
+            FLIP_CHECK(hipGraphLaunch(jacobi ? c->pcg_graph : c->ic_graph, c->stream));
         } else {
             int batch = c->profiling ? 1 : 4;
             for (int t = 0; t < batch; ++t, ++hk) {
                 launch_pair(c, jacobi, G, h2, subcell, pa, pb);
                 if (!jacobi) {
                     t_begin(c, "precond_wavefront");
-                    precond_wavefront(c, mode, c->r, c->z);
+                    precond_wavefront(c, mode, c->r, c->z, c->iflags);
                     hipLaunchKernelGGL(k_copy_fluid_x, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[hk & 1],
                                        c->iflags);
                     hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pb, 2, 1, c->iflags);

@@ -29,6 +29,12 @@ if ROOT not in sys.path:
 import flip3d_amd as fa  # noqa: E402
 
 WEIGHTS = os.path.join(ROOT, "flip3d_amd", "weights", "unet_coarse.bin")
+SEED = 20261203
+
+
+def ensemble_sizes(scenes, seed=SEED):
+    """The ensemble's grid sizes: N uniform in {24..96}, seeded."""
+    return [int(n) for n in np.random.RandomState(seed).randint(24, 97, size=scenes)]
 
 
 def main():
@@ -37,12 +43,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--threads", type=int, default=8)
-    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--seed", type=int, default=SEED)
     ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
 
-    rs = np.random.RandomState(args.seed)
-    sizes = rs.randint(24, 97, size=args.scenes)
+    sizes = np.array(ensemble_sizes(args.scenes, args.seed))
     ctxs, particles, learned = [], 0, 0
     for i, n in enumerate(sizes):
         use_learned = (not args.no_learned) and n % 16 == 0

@@ -117,6 +117,25 @@ int ref_init_test(int which, unsigned seed)
     case 1: spherePourTest(); break;
     case 2: cliffPourTest(); break;
     case 3: waterDropTest(); break;
+    case 4: {
+        // the benchmark scene's liquid (box x<0.4, y<0.6, all z, plus a
+        // sphere r=0.12 at (0.6,0.7,0.5)) as reference Objects, so that the
+        // reference's own object test (pushParticle, flip3D.cpp:580-633)
+        // decides which sub-lattice points hold liquid
+        Object obj;
+        obj.type = FLUID;
+        obj.shape = BOX;
+        obj.visible = true;
+        obj.p[0][0] = 0.0; obj.p[1][0] = 0.4;
+        obj.p[0][1] = 0.0; obj.p[1][1] = 0.6;
+        obj.p[0][2] = 0.0; obj.p[1][2] = 1.0;
+        objects.push_back(obj);
+        obj.shape = SPHERE;
+        obj.c[0] = 0.6; obj.c[1] = 0.7; obj.c[2] = 0.5;
+        obj.r = 0.12;
+        objects.push_back(obj);
+        break;
+    }
     default: return -1;
     }
     srand(seed);

@@ -26,7 +26,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 SECTIONS = ["kernels", "sort", "p2g", "g2p", "correct", "solve", "scene_ref", "traj_ref", "traj_bench",
-            "checkpoint", "scenes_test", "mesh", "exports"]
+            "checkpoint", "scenes_test", "mesh", "exports", "scene_bench_ref",
+            "solve_unit"]
 
 
 def digest(a):
@@ -287,6 +288,40 @@ def sec_traj_bench(out):
     p, u, nr, t, d = _ref_particles(R)
     np.savez_compressed(out + ".init.npz", p=p, n=nr, t=t, max_dens=np.float32(R.ref_get_max_dens()))
     _trajectory(R, 100, (1, 10), out)
+
+
+def sec_solve_unit(out):
+    """solver::solve on a system with P == 1.0 cells (tests/pocket_system.py),
+    where the reference's value-test couplings are live: its own result
+    under its stop rule, and its factor P."""
+    from tests.pocket_system import pocket_system
+    R = ref()
+    n = 12
+    A, L, x, b = pocket_system(n)
+    xo, bo = x.copy(), b.copy()
+    with Quiet():
+        R.ref_kat_solve(n, P(A), P(L), P(xo), P(bo), 1)
+    np.savez_compressed(out, n=n, A=A, L=L, x=x, b=b, x_out=xo, b_out=bo)
+
+
+def sec_scene_bench_ref(out):
+    """The benchmark scene's liquid objects placed by the reference's own
+    placement loop and object test (flip3D.cpp:580-633, 711-761) at N=32:
+    the sub-lattice index of every FLUID particle in creation order (the
+    reference's +-0.002/N rand() jitter is far below the half spacing, so
+    rounding recovers the index) and the WALL particle positions (no
+    jitter).  The oracle's benchmark placement is checked against these."""
+    R = ref()
+    with Quiet():
+        R.ref_init_test(4, 1)
+    p, u, nr, t, d = _ref_particles(R)
+    n = R.ref_grid_n()
+    w = 0.5 / n
+    fl = t == 1
+    idx = np.rint(p[fl].astype(np.float64) / w - 0.5).astype(np.int16)
+    assert np.abs(p[fl] - (idx + 0.5) * w).max() < 0.01 / n
+    np.savez_compressed(out, n=np.int32(n), fluid_lattice=idx, wall_p=p[~fl], wall_n=nr[~fl],
+                        n_fluid=np.int32(fl.sum()), n_wall=np.int32((~fl).sum()))
 
 
 def sec_scenes_test(out):

@@ -183,12 +183,21 @@ def test_stage_parity_first_step(n):
         s.close()
 
 
-@pytest.mark.parametrize("scene,n", [("benchmark", 32), ("reference", 32), ("benchmark", 24)])
-def test_trajectory_parity(scene, n):
-    """Free-running steps: flags bit-exact, fields within 1e-5 relative."""
-    steps = int(os.environ.get("FLIP_PARITY_STEPS", "30"))
-    g, s = _pair(n=n, scene=scene)
-    exact = 0
+# step policies: the reference's (fixed dt 6e-3, absolute stop sum r^2 <= 1e-2 n^3)
+# and the benchmark configuration's (CFL dt, relative residual 1e-5)
+POLICIES = {"reference": {}, "config": dict(dt_policy=1, tol_policy=1, tol=1e-5)}
+
+
+@pytest.mark.parametrize("scene,n,policy", [("benchmark", 32, "config"), ("benchmark", 32, "reference"),
+                                            ("reference", 32, "reference"), ("reference", 32, "config"),
+                                            ("benchmark", 24, "reference")])
+def test_trajectory_parity(scene, n, policy):
+    """Free-running steps (100 by default): flags bit-exact, the same time
+    step, PCG iteration count and rand() events every step, fields within
+    1e-5 relative.  Prints the error growth and the bit-identical steps."""
+    steps = int(os.environ.get("FLIP_PARITY_STEPS", "100"))
+    g, s = _pair(n=n, scene=scene, **POLICIES[policy])
+    exact, worst, growth = 0, 0.0, {}
     try:
         for k in range(steps):
             gs = g.step()
@@ -200,12 +209,74 @@ def test_trajectory_parity(scene, n):
             eu = rel_err(a["u"][fl], b["u"][fl])
             epr = rel_err(g.grid(fa.GRID_PRESS), s.grid(po.G_PRESS))
             assert max(ep, eu, epr) <= REL_TOL, (k + 1, ep, eu, epr, gs, ss)
+            assert gs["dt"] == ss["dt"], (k + 1, gs["dt"], ss["dt"])
+            assert gs["iterations"] == ss["iterations"], (k + 1, gs["iterations"], ss["iterations"])
             exact += int(np.array_equal(a["p"], b["p"]) and np.array_equal(a["u"][fl], b["u"][fl]))
             assert gs["n_reposition"] == ss["n_reposition"] and gs["n_jitter"] == ss["n_jitter"]
-        print(f"{scene}: {exact}/{steps} steps bit-identical")
+            worst = max(worst, ep, eu, epr)
+            if k + 1 in (1, 10, 25, 50, 100, steps):
+                growth[k + 1] = f"{max(ep, eu, epr):.1e}"
+        print(f"{scene} N={n} {policy} policy: {exact}/{steps} steps bit-identical, "
+              f"max rel error by step {growth}, worst {worst:.2e}")
     finally:
         g.close()
         s.close()
+
+
+@pytest.mark.parametrize("speed,wall_speed", [(0.5, 0.0), (20.0, 0.0), (55.5, 0.0), (0.5, 100.0)])
+def test_cfl_time_step_on_device(speed, wall_speed):
+    """The device's CFL rule (max-speed reduction over FLUID particles):
+    dt = min(6e-3, 0.6 / (N max|u|)), equal to the closed form and the
+    oracle's."""
+    n = 8
+    rs = np.random.RandomState(int(speed * 10 + wall_speed))
+    m = 5000
+    p = rs.uniform(0.3, 0.7, size=(m, 3)).astype(np.float32)
+    dirs = rs.normal(size=(m, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    scale = rs.uniform(0.0, 1.0, size=(m, 1))
+    scale[4321] = 1.0
+    u = (dirs * scale * speed).astype(np.float32)
+    t = np.full(m, po.FLUID, np.int8)
+    t[5] = po.WALL
+    u[5] = np.float32(wall_speed) * dirs[5].astype(np.float32)
+    fl = t == po.FLUID
+    umax = np.sqrt((u[fl].astype(np.float64) ** 2).sum(1).max())
+    want = min(6e-3, 0.6 / (n * umax))
+    with fa.FlipContext(n, dt_policy=1) as g:
+        g.upload(p, u, np.zeros_like(p), t, dens=np.ones(m, np.float32))
+        gs = g.step()
+    s = po.Sim(n, dt_policy=1)
+    try:
+        s.set_particles(p, u, np.zeros_like(p), t, dens=np.ones(m, np.float32))
+        ss = s.step()
+    finally:
+        s.close()
+    assert gs["dt"] == want == ss["dt"]
+
+
+@pytest.mark.parametrize("tol_policy,tol", [(0, 1e-5), (1, 1e-6)])
+def test_unit_factor_solve_matches_reference(tol_policy, tol):
+    """A system with P == 1.0 cells (a FLUID cell with five WALL neighbours):
+    the reference-form value test makes the factor and sweep couplings live,
+    so the device runs the hyperplane-wavefront sweeps.  Under the
+    reference's stop rule the result equals the reference's own solve
+    (golden solve_unit.npz); under a relative tolerance it equals the
+    oracle's, iteration for iteration."""
+    d = gold("solve_unit")
+    n = int(d["n"])
+    kw = dict(tol_policy=tol_policy, tol=tol)
+    with fa.FlipContext(n, **kw) as g:
+        x, b, st = g.solve(d["A"], d["L"], d["x"], d["b"])
+    assert st["n_precond_unit"] == 10, st
+    xo, bo, so = po.solve(n, d["A"], d["L"], d["x"].copy(), d["b"].copy(), **kw)
+    assert so["n_precond_unit"] == 10
+    assert st["iterations"] == so["iterations"] > 1
+    assert np.array_equal(b.reshape(-1), bo.reshape(-1))
+    assert rel_err(x, xo) <= REL_TOL
+    if tol_policy == 0:
+        assert np.array_equal(b.reshape(-1), d["b_out"].reshape(-1))
+        assert rel_err(x, d["x_out"]) <= REL_TOL
 
 
 def test_pcg_graph_replays_match_eager_launches():

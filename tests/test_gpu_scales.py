@@ -1,0 +1,291 @@
+"""The benchmark configurations at their own scales, on the GPU.
+
+  * N=128 (the metric's grid): one step of the benchmark scene under the
+    configuration's policy (CFL dt, relative residual 1e-5) against the
+    oracle: flags bit-exact, fields within 1e-5 relative.
+  * The learned solver against the oracle's plain PCG at N=32 and 64, and
+    against the plain GPU solve at N=128: both solve the same system to the
+    same relative residual, so their difference is bounded by the residuals:
+        ||p_l - p_o||_2 <= (||r_l||_2 + ||r_o||_2) / lambda_min(A),
+    with r = b - A p the true FP64 residual of each FP32 result and
+    lambda_min from Lanczos on A^-1 (FP64 solves).  Both stop on a residual
+    evaluated in FP32 arithmetic (plain PCG on its recursively updated
+    residual, as the reference does), so their true residuals exceed 1e-5;
+    the learned result's is the smaller.  The trained network needs at most
+    0.7x the iterations of the same V-cycle without it.
+  * N=256: one step with FP64-residual iterative refinement reaches 1e-7;
+    its FP32 pressure is within the bound of an FP64 solve's:
+        ||p_r - p_64||_2 <= (||r_r||_2 + ||r_64||_2) / lambda_min
+                            + 2^-24 (||p_r||_2 + ||p_64||_2)
+    (FP64 residuals of the solvers' FP64 iterates, plus the rounding of
+    those iterates to FP32 on output).
+  * 64 independent scenes with N drawn from {24..96}: stepping them together
+    equals stepping them one by one, bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.sparse.linalg import LinearOperator, eigsh
+
+import flip3d_amd as fa
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "flip3d_amd", "weights", "unet_coarse.bin")
+TOL = 1e-5
+CONFIG = dict(dt_policy=1, tol_policy=1, tol=TOL)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)) if a.size else 0.0
+
+
+def true_residual(n, A, L, b, x):
+    """b - A x over FLUID cells, in FP64 (the oracle's FP64 stencil)."""
+    Ax = po.compute_ax(n, A, L, np.asarray(x, np.float64), f64=True)
+    fl = A == po.FLUID
+    return (np.asarray(b, np.float64) - Ax)[fl]
+
+
+def inv_lambda_min(n, A, L, solve64):
+    """1 / lambda_min of the pressure operator restricted to FLUID cells:
+    Lanczos (ARPACK) on A^-1, applied by FP64 solves to 1e-12."""
+    idx = np.flatnonzero(A.reshape(-1) == po.FLUID)
+    m = len(idx)
+
+    def ainv(v):
+        b = np.zeros(n ** 3)
+        b[idx] = -np.ravel(v)  # solve() negates b: it solves A x = -b
+        return np.asarray(solve64(b.reshape(n, n, n)), np.float64).reshape(-1)[idx]
+
+    w = eigsh(LinearOperator((m, m), matvec=ainv, dtype=np.float64), k=1, which="LA", tol=1e-8,
+              return_eigenvectors=False)
+    return float(w[0])
+
+
+def oracle_solver64(n, A, L):
+    return lambda b: po.solve(n, A, L, np.zeros((n, n, n)), b, f64=True, tol_policy=1, tol=1e-12)[0]
+
+
+def gpu_solver64(ctx, A, L):
+    n = ctx.n
+    return lambda b: ctx.solve(A, L, np.zeros((n, n, n), np.float32), np.asarray(b, np.float32))[0]
+
+
+def _system(g):
+    return g.grid(fa.GRID_A), g.grid(fa.GRID_L), g.grid(fa.GRID_DIV)
+
+
+# ---------------------------------------------------------------- N=128
+def test_benchmark_step_n128_matches_oracle():
+    """Step 1 of the 4,048,638-particle benchmark scene at N=128 under the
+    configuration policy: flags, time step and iteration count equal the
+    oracle's; pressure, face velocities and particles within 1e-5."""
+    n = 128
+    print("N=128 benchmark step vs oracle", flush=True)
+    with fa.FlipContext(n, **CONFIG) as g:
+        assert g.build_scene(fa.SCENE_BENCHMARK) == 3951862 + 96776
+        g.srand(1)
+        gs = g.step()
+        a = g.download()
+        ga = {w: g.grid(w) for w in (fa.GRID_A, fa.GRID_L, fa.GRID_PRESS, fa.GRID_U0, fa.GRID_U1, fa.GRID_U2)}
+    s = po.Sim(n, **CONFIG)
+    try:
+        s.scene_benchmark()
+        s.srand(1)
+        ss = s.step()
+        b = s.particles()
+        sa = {w: s.grid(w) for w in (po.G_A, po.G_L, po.G_PRESS, po.G_U0, po.G_U1, po.G_U2)}
+    finally:
+        s.close()
+    assert gs["dt"] == ss["dt"] and gs["iterations"] == ss["iterations"], (gs, ss)
+    assert gs["n_reposition"] == ss["n_reposition"] and gs["n_jitter"] == ss["n_jitter"]
+    assert np.array_equal(ga[fa.GRID_A], sa[po.G_A])
+    assert np.array_equal(ga[fa.GRID_L], sa[po.G_L])
+    assert rel_err(ga[fa.GRID_PRESS], sa[po.G_PRESS]) <= TOL
+    for gw, sw in ((fa.GRID_U0, po.G_U0), (fa.GRID_U1, po.G_U1), (fa.GRID_U2, po.G_U2)):
+        assert rel_err(ga[gw], sa[sw]) <= TOL, gw
+    fl = b["type"] == po.FLUID
+    assert rel_err(a["p"], b["p"]) <= TOL
+    assert rel_err(a["u"][fl], b["u"][fl]) <= TOL
+    print(f"N=128: {gs['iterations']} iterations, dt {gs['dt']:.6g}, press err "
+          f"{rel_err(ga[fa.GRID_PRESS], sa[po.G_PRESS]):.2e}, u err {rel_err(a['u'][fl], b['u'][fl]):.2e}")
+
+
+def _zero_weights(path):
+    with fa.UNet(32, capacity=1, seed=1) as net:
+        net.set_params(np.zeros(net.num_params, np.float32))
+        net.save(path)
+
+
+def _run(n, precond, weights=None, steps=1):
+    with fa.FlipContext(n, precond=precond, **CONFIG) as g:
+        if weights:
+            g.set_learned_weights(weights)
+        g.build_scene(fa.SCENE_BENCHMARK)
+        g.srand(1)
+        sts = [g.step() for _ in range(steps)]
+        return sts, _system(g), g.grid(fa.GRID_PRESS)
+
+
+def test_learned_n128_within_residual_bound_of_plain(tmp_path):
+    """The metric's configuration: the learned step's solve meets the
+    tolerance, its pressure is within the residual bound of the plain
+    solve's on the identical system, and the trained network needs at most
+    0.7x the iterations of the zero network over three steps."""
+    n = 128
+    print("N=128 learned vs plain", flush=True)
+    sp, (A, L, b), pp = _run(n, fa.PRECOND_REFERENCE)
+    sl, (A2, L2, b2), pl = _run(n, fa.PRECOND_LEARNED, WEIGHTS)
+    assert np.array_equal(A, A2) and np.array_equal(L, L2) and np.array_equal(b, b2)
+    assert sl[0]["learned_iterations"] > 0
+    assert sl[0]["error2"] <= TOL * TOL * sl[0]["bnorm2"] * 1.0001, sl[0]
+    fl = A == po.FLUID
+    bn = np.linalg.norm(np.asarray(b, np.float64)[fl])
+    rl, rp = np.linalg.norm(true_residual(n, A, L, b, pl)), np.linalg.norm(true_residual(n, A, L, b, pp))
+    with fa.FlipContext(n, tol_policy=1, tol=1e-12, solve_fp64=1) as g64:
+        mu = inv_lambda_min(n, A, L, gpu_solver64(g64, A, L))
+    diff = np.linalg.norm((np.asarray(pl, np.float64) - pp)[fl])
+    bound = mu * (rl + rp) * 1.001
+    print(f"N=128: true rel residual learned {rl / bn:.2e} plain {rp / bn:.2e}; 1/lambda_min {mu:.5g}; "
+          f"||p_l - p_p|| {diff:.3e} <= bound {bound:.3e} ({diff / np.linalg.norm(pp[fl]):.2e} relative)")
+    # both stop on a residual evaluated in FP32 arithmetic (plain PCG on its
+    # recursively updated residual, which drifts from the true one over
+    # ~270 iterations); the learned result's true residual is the smaller
+    assert rl <= rp
+    assert diff <= bound
+    zero = str(tmp_path / "zero.bin")
+    _zero_weights(zero)
+    st, _, _ = _run(n, fa.PRECOND_LEARNED, WEIGHTS, steps=3)
+    sz, _, _ = _run(n, fa.PRECOND_LEARNED, zero, steps=3)
+    it_t, it_z = sum(s["iterations"] for s in st), sum(s["iterations"] for s in sz)
+    for s in st + sz:
+        assert s["error2"] <= TOL * TOL * s["bnorm2"] * 1.0001, s
+    print(f"N=128: iterations over 3 steps, trained {it_t} vs zero network {it_z}")
+    assert it_t <= 0.7 * it_z, (it_t, it_z)
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_learned_step_matches_oracle(n):
+    """Teacher-forced step 1: the stages up to the solve are bit-exact with
+    the oracle's; the learned pressure is within the residual bound of the
+    oracle's plain-PCG pressure; and the face velocities after the learned
+    projection equal the oracle's projection of that same pressure (the
+    oracle's PCG warm-started from it)."""
+    print(f"N={n} learned vs oracle", flush=True)
+    g = fa.FlipContext(n, precond=fa.PRECOND_LEARNED, **CONFIG)
+    g.set_learned_weights(WEIGHTS)
+    s = po.Sim(n, **CONFIG)
+    s2 = po.Sim(n, **CONFIG)
+    try:
+        g.build_scene(fa.SCENE_BENCHMARK)
+        for o in (s, s2):
+            o.scene_benchmark()
+        for stg in (fa.STAGE_DENSITY, fa.STAGE_EXT_FORCE, fa.STAGE_P2G):
+            g.stage(stg)
+        for o in (s, s2):
+            for nm in ("sort", "compute_density", "add_ext_force", "sort", "map_p2g", "mark_water", "save_grid",
+                       "enforce_boundary"):
+                o.call(nm)
+        gst = g.stage(fa.STAGE_PROJECT)
+        sst = s.call_stats("project")
+        s.call("enforce_boundary")
+        A, L, b = _system(g)
+        assert np.array_equal(A, s.grid(po.G_A)) and np.array_equal(L, s.grid(po.G_L))
+        assert np.array_equal(b, s.grid(po.G_DIV))
+        pl, po_ = g.grid(fa.GRID_PRESS), s.grid(po.G_PRESS)
+        assert gst["learned_iterations"] > 0
+        assert gst["error2"] <= TOL * TOL * gst["bnorm2"] * 1.0001
+        fl = A == po.FLUID
+        bn = np.linalg.norm(np.asarray(b, np.float64)[fl])
+        rl, ro = np.linalg.norm(true_residual(n, A, L, b, pl)), np.linalg.norm(true_residual(n, A, L, b, po_))
+        mu = inv_lambda_min(n, A, L, oracle_solver64(n, A, L))
+        diff = np.linalg.norm((np.asarray(pl, np.float64) - po_)[fl])
+        bound = mu * (rl + ro) * 1.001
+        print(f"N={n}: learned {gst['learned_iterations']}+{gst['iterations'] - gst['learned_iterations']} vs "
+              f"oracle {sst['iterations']} iterations; true rel residual learned {rl / bn:.2e} oracle {ro / bn:.2e}; "
+              f"||dp|| {diff:.3e} <= {bound:.3e} ({diff / np.linalg.norm(po_[fl]):.2e} relative)")
+        assert rl <= ro
+        assert diff <= bound
+        # the projection of the learned pressure: the oracle's PCG warm-started from it
+        s2.set_grid(po.G_PRESS, pl)
+        s2st = s2.call_stats("project")
+        s2.call("enforce_boundary")
+        for gw, sw in ((fa.GRID_U0, po.G_U0), (fa.GRID_U1, po.G_U1), (fa.GRID_U2, po.G_U2)):
+            assert rel_err(g.grid(gw), s2.grid(sw)) <= TOL, (gw, s2st["iterations"])
+    finally:
+        g.close()
+        s.close()
+        s2.close()
+
+
+# ---------------------------------------------------------------- N=256
+def test_refinement_n256_tracks_fp64_solve():
+    """The N=256 configuration: one benchmark-scene step whose FP32 PCG
+    (relative 1e-5 per sweep) runs inside FP64-residual refinement to 1e-7,
+    against the same step with the whole solve in FP64 (to 1e-12)."""
+    n = 256
+    print("N=256 refinement vs FP64 solve", flush=True)
+    res = {}
+    for name, kw in (("refine", dict(refine=10, refine_tol=1e-7, tol=1e-5)),
+                     ("fp64", dict(solve_fp64=1, tol=1e-12))):
+        with fa.FlipContext(n, tol_policy=1, dt_policy=1, **kw) as g:
+            g.build_scene(fa.SCENE_BENCHMARK)
+            g.srand(1)
+            st = g.step()
+            res[name] = (st, _system(g), g.grid(fa.GRID_PRESS))
+    (sr, (A, L, b), pr), (s64, (A2, L2, b2), p64) = res["refine"], res["fp64"]
+    assert np.array_equal(A, A2) and np.array_equal(L, L2) and np.array_equal(b, b2)
+    assert sr["refine_sweeps"] >= 2
+    assert sr["error2"] <= 1e-14 * sr["bnorm2"], sr
+    assert s64["error2"] <= 1e-24 * s64["bnorm2"], s64
+    fl = A == po.FLUID
+    with fa.FlipContext(n, tol_policy=1, tol=1e-12, solve_fp64=1) as g64:
+        mu = inv_lambda_min(n, A, L, gpu_solver64(g64, A, L))
+    pr64, pp64 = np.asarray(pr, np.float64)[fl], np.asarray(p64, np.float64)[fl]
+    diff = np.linalg.norm(pr64 - pp64)
+    bound = mu * (np.sqrt(sr["error2"]) + np.sqrt(s64["error2"])) * 1.001 + \
+        2.0 ** -24 * (np.linalg.norm(pr64) + np.linalg.norm(pp64))
+    print(f"N=256: {sr['refine_sweeps']} sweeps, rel residual {np.sqrt(sr['error2'] / sr['bnorm2']):.2e}; "
+          f"1/lambda_min {mu:.5g}; ||p_r - p_64|| {diff:.3e} <= {bound:.3e} "
+          f"({diff / np.linalg.norm(pp64):.2e} relative)")
+    assert diff <= bound
+
+
+# ---------------------------------------------------------------- ensemble
+def test_ensemble_of_64_scenes_equals_single_steps():
+    """Configuration 5's ensemble: 64 scenes with N uniform in {24..96} and
+    1-3 random liquid boxes each (seeded), stepped together and one by one:
+    particles and pressure identical bit for bit after each of two steps."""
+    from flip3d_amd.ensemble_bench import ensemble_sizes
+    sizes = ensemble_sizes(64)
+    assert len(sizes) == 64 and min(sizes) >= 24 and max(sizes) <= 96
+
+    def make():
+        out = []
+        for i, n in enumerate(sizes):
+            c = fa.FlipContext(n, **CONFIG)
+            c.build_scene(fa.SCENE_RANDOM_BOXES, i)
+            c.srand(1)
+            out.append(c)
+        return out
+
+    a, b = make(), make()
+    try:
+        for _ in range(2):
+            fa.step_many(a, threads=8)
+            for c in b:
+                c.step()
+            for ca, cb in zip(a, b):
+                pa, pb = ca.download(), cb.download()
+                for k in ("p", "u", "dens", "type"):
+                    assert np.array_equal(pa[k], pb[k]), (ca.n, k)
+                assert np.array_equal(ca.grid(fa.GRID_PRESS), cb.grid(fa.GRID_PRESS))
+    finally:
+        for c in a + b:
+            c.close()

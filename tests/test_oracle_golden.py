@@ -106,6 +106,35 @@ def test_solver_known_answer():
     assert np.array_equal(b.reshape(-1).view(np.uint32), g["b_out"].reshape(-1).view(np.uint32))
 
 
+def test_solver_unit_factor_known_answer():
+    """A system with P == 1.0 cells (tests/pocket_system.py): the value-test
+    couplings of the factor (solver.cpp:127-135) and of both sweeps
+    (:151-156, :167-172) are live.  The oracle's solve equals the
+    reference's bit for bit, its factor is exactly 1.0 at the ten tunnel
+    cells (1/sqrt(2) at the upper ones if the coupling were dropped), and
+    the result differs from the same solve with couplings keyed off."""
+    from tests.pocket_system import pocket_system, unit_cells
+    g = gold("solve_unit")
+    n = int(g["n"])
+    A, L, x0, b0 = pocket_system(n)
+    assert np.array_equal(A, g["A"]) and np.array_equal(b0, g["b"])
+    x, b, st = po.solve(n, g["A"], g["L"], g["x"], g["b"])
+    assert st["n_precond_unit"] == 10
+    assert np.array_equal(x.reshape(-1).view(np.uint32), g["x_out"].reshape(-1).view(np.uint32))
+    assert np.array_equal(b.reshape(-1).view(np.uint32), g["b_out"].reshape(-1).view(np.uint32))
+    P = po.build_precond(n, g["A"], g["L"])
+    assert sorted(zip(*np.nonzero(P == 1.0))) == sorted(unit_cells(n))
+    # the forward sweep's value test on q is live too: with r = 1.0 at a
+    # tunnel bottom, the cell above gets q = (r + 1) * 1
+    r = np.zeros((n, n, n), np.float32)
+    for (i, j, k) in unit_cells(n):
+        if k == 3:
+            r[i, j, k] = 1.0
+    z = po.apply_precond(n, g["A"], P, r)
+    i, j = unit_cells(n)[0][:2]
+    assert z[i, j, 4] != 0.0  # zero under a per-cell (coupling-free) preconditioner
+
+
 def test_reference_scene():
     g = gold("scene_ref")
     s = po.Sim(32)

@@ -291,3 +291,157 @@ def test_f64_oracle_tracks_f32_oracle():
         b.step()
     pa, pb = a.particles()["p"], b.particles()["p"]
     assert np.abs(pa - pb).max() < 1e-3
+
+
+# ------------------------------------------------------- benchmark scene
+_M64 = (1 << 64) - 1
+
+
+def splitmix64_stream(seed, count):
+    """SplitMix64 (Steele, Lea and Flood, OOPSLA 2014; Vigna's splitmix64.c):
+    state += 0x9E3779B97F4A7C15, then the two xor-shift-multiply rounds."""
+    out, x = [], seed
+    for _ in range(count):
+        x = (x + 0x9E3779B97F4A7C15) & _M64
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def splitmix64_at(seed, k):
+    """Value k (0-based) of the stream seeded with `seed`, vectorised over k."""
+    k = np.asarray(k, np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (k + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def test_splitmix64_published_values():
+    """The generator's published outputs: seed 1234567 starts 6457827717110365317,
+    3203168211198807973, ... (splitmix64.c test vector) and seed 0 starts
+    0xE220A8397B1DCDAF; the vectorised form equals the stream."""
+    assert splitmix64_stream(1234567, 5) == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                             4593380528125082431, 16408922859458223821]
+    assert splitmix64_stream(0, 1) == [0xE220A8397B1DCDAF]
+    assert [int(v) for v in splitmix64_at(20261203, np.arange(64))] == splitmix64_stream(20261203, 64)
+
+
+def _hypotf(a, b):
+    # glibc hypotf == (float)sqrt((double)a^2 + (double)b^2) (pinned above)
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.sqrt(a * a + b * b).astype(np.float32)
+
+
+def benchmark_placement(n, seed=20261203, density=0.5):
+    """The benchmark scene written independently of the oracle: the
+    reference's sub-lattice loop (flip3D.cpp:711-727) with its object test
+    (pushParticle, :580-633) for the liquid box {x<0.4, y<0.6} (FLOAT
+    corners compared in double) and sphere (float hypotf distance to the
+    float centre, < 0.12f), jitter U(-0.25, 0.25)*dx per axis from SplitMix64
+    value 3*fluid_index + axis (53-bit uniform), then one WALL particle at
+    the centre of every boundary cell (:730-740)."""
+    w = density * (1.0 / n)
+    lim = int(n / density)
+    c = np.arange(lim) * w + w / 2.0
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    X, Y, Z = X.reshape(-1), Y.reshape(-1), Z.reshape(-1)
+    wt = 1.0 / n
+    inside = (X > wt) & (X < 1 - wt) & (Y > wt) & (Y < 1 - wt) & (Z > wt) & (Z < 1 - wt)
+    box = (X > 0) & (X < np.float64(np.float32(0.4))) & (Y > 0) & (Y < np.float64(np.float32(0.6))) & (Z > 0) & (Z < 1)
+    f32 = np.float32
+    d0 = X.astype(f32) - f32(0.6)
+    d1 = Y.astype(f32) - f32(0.7)
+    d2 = Z.astype(f32) - f32(0.5)
+    sphere = _hypotf(_hypotf(d0, d1), d2) < f32(0.12)
+    keep = inside & (box | sphere)
+    lat = np.stack([X[keep], Y[keep], Z[keep]], 1)
+    m = len(lat)
+    v = splitmix64_at(seed, np.arange(3 * m, dtype=np.uint64)).reshape(m, 3)
+    uu = (v >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    jit = (uu * 0.5 - 0.25) * (1.0 / n)
+    fluid = (lat + jit).astype(np.float32)
+    cc = np.arange(n) * wt + wt / 2.0
+    A, B, Cc = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    edge = (np.minimum(np.minimum(A, B), Cc) == 0) | (np.maximum(np.maximum(A, B), Cc) == n - 1)
+    wall = np.stack([cc[A[edge]], cc[B[edge]], cc[Cc[edge]]], 1).astype(np.float32)
+    return fluid, lat, wall
+
+
+def test_benchmark_placement_matches_reference_lattice():
+    """At N=32 the reference's own placement of the benchmark objects
+    (tests/golden/scene_bench_ref.npz, from the compiled reference) keeps
+    exactly the sub-lattice points and wall cells of the independent
+    placement above, in the same order: 53,730 liquid + 5,768 wall."""
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "scene_bench_ref.npz"), allow_pickle=False)
+    fluid, lat, wall = benchmark_placement(32)
+    assert (len(fluid), len(wall)) == (53730, 5768) == (int(d["n_fluid"]), int(d["n_wall"]))
+    w = 0.5 / 32
+    assert np.array_equal(np.rint(lat / w - 0.5).astype(np.int16), d["fluid_lattice"])
+    assert np.array_equal(wall, d["wall_p"])
+
+
+@pytest.mark.parametrize("n,counts", [(32, (53730, 5768)), (128, (3951862, 96776))])
+def test_oracle_benchmark_scene_is_pinned(n, counts):
+    """The oracle's benchmark scene equals the independent placement bit for
+    bit (liquid then walls, creation order), with the particle counts of the
+    reference's placement (53,730 + 5,768 at N=32; 3,951,862 + 96,776 at
+    N=128); every jitter stays within +-0.25 dx of its sub-point.  In exact
+    arithmetic that keeps each particle in its sub-point's grid cell; the
+    float rounding of position + jitter can land a particle exactly on the
+    cell face above (23 particles at N=128, none at N=32), and only there."""
+    fluid, lat, wall = benchmark_placement(n)
+    assert (len(fluid), len(wall)) == counts
+    s = po.Sim(n)
+    try:
+        m = s.scene_benchmark(20261203)
+        q = s.particles()
+    finally:
+        s.close()
+    assert m == sum(counts)
+    t = q["type"]
+    assert np.all(t[:counts[0]] == po.FLUID) and np.all(t[counts[0]:] == po.WALL)
+    assert np.array_equal(q["p"][:counts[0]], fluid)
+    assert np.array_equal(q["p"][counts[0]:], wall)
+    off = np.abs(fluid.astype(np.float64) - lat) * n
+    assert off.max() <= 0.25 + 1e-6 and off.max() > 0.24
+    cell = lambda a: np.floor(np.asarray(a, np.float64) * n)  # noqa: E731
+    moved = cell(fluid) != cell(lat)
+    on_face = fluid.astype(np.float64) * n == cell(fluid)
+    assert np.all(on_face[moved])
+    assert moved.sum() == {32: 0, 128: 23}[n]
+
+
+# ------------------------------------------------------------- CFL dt
+@pytest.mark.parametrize("speed,wall_speed", [(0.0, 0.0), (0.5, 0.0), (20.0, 0.0), (55.5, 0.0), (0.5, 100.0)])
+def test_cfl_time_step_closed_form(speed, wall_speed):
+    """dt_policy 1: dt = min(dt_cap, cfl / (N max|u|)) over FLUID particles
+    only (walls do not move), with |u| the double norm of the float
+    components; dt_cap = 6e-3 (flip3D.cpp:31) and cfl = 0.6."""
+    n = 8
+    rs = np.random.RandomState(int(speed * 10 + wall_speed))
+    m = 200
+    p = rs.uniform(0.3, 0.7, size=(m, 3)).astype(np.float32)
+    dirs = rs.normal(size=(m, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    scale = rs.uniform(0.0, 1.0, size=(m, 1))
+    scale[17] = 1.0
+    u = (dirs * scale * speed).astype(np.float32)
+    t = np.full(m, po.FLUID, np.int8)
+    t[5] = po.WALL
+    u[5] = np.float32(wall_speed) * dirs[5].astype(np.float32)
+    fl = t == po.FLUID
+    umax = np.sqrt((u[fl].astype(np.float64) ** 2).sum(1).max())
+    want = 6e-3 if umax == 0 else min(6e-3, 0.6 / (n * umax))
+    s = po.Sim(n, dt_policy=1)
+    try:
+        s.set_particles(p, u, np.zeros_like(p), t, dens=np.ones(m, np.float32))
+        st = s.step()
+    finally:
+        s.close()
+    assert st["dt"] == want
+    if speed >= 20.0:
+        assert want < 6e-3  # the CFL branch is the one taken (0.6 / (8 * 12.5) = 6e-3)
