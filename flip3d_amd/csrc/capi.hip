@@ -74,6 +74,7 @@ void free_particles(FlipCtx *c)
     dfree(c->tmp1);
     dfree(c->pkey);
     dfree(c->pslot);
+    dfree(c->sort_big);
     dfree(c->perm);
     dfree(c->deep);
     dfree(c->ev_list);
@@ -101,6 +102,7 @@ void alloc_particles(FlipCtx *c, int64_t cap)
     dalloc(&c->tmp1, n);
     dalloc(&c->pkey, n);
     dalloc(&c->pslot, n);
+    dalloc(&c->sort_big, n / 17 + 2);  // at most np / 17 buckets hold more than 16
     dalloc(&c->perm, n);
     size_t big = std::max<size_t>(n, (size_t)c->C) + 1;
     dalloc(&c->deep, big);

@@ -67,6 +67,7 @@ struct FlipCtx {
     float *xyz3 = nullptr;
     float4 *tmp0 = nullptr, *tmp1 = nullptr;  // corrector pseudo-position / velocity
     int32_t *pkey = nullptr, *pslot = nullptr, *perm = nullptr;
+    int32_t *sort_big = nullptr;  // buckets over 16 particles: count, then cell ids
 
     // cells
     int32_t *cell_count = nullptr, *cell_start = nullptr;

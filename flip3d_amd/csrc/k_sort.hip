@@ -7,10 +7,12 @@
 //      counter (arbitrary order inside the cell),
 //   2. scans the per-cell counts into cell_start,
 //   3. scatters particle indices to cell_start[key] + slot,
-//   4. restores creation order inside each cell (insertion sort by index;
-//      cells hold ~8 particles), and
+//   4. restores creation order inside each cell (cells hold ~8 particles,
+//      ranked in registers; larger buckets are ranked by whole blocks), and
 //   5. permutes all particle arrays into the new order (coalesced gather).
 // The result is bit-identical to the reference's buckets.
+#include <mutex>
+
 #include "flip_device.hpp"
 #include "flip_internal.hpp"
 
@@ -113,15 +115,29 @@ __global__ __launch_bounds__(SB) void k_scan_apply(const int32_t *__restrict__ i
 // 1. cell key and slot claim.  Storage is sorted by the previous step's cells,
 // so neighbouring lanes mostly share a key: each run of equal keys within a
 // wave claims its slots with one atomic (slots in lane order within the run).
-__global__ void k_sort_count(const float4 *__restrict__ pos, int64_t np, int n, int32_t *key, int32_t *slot,
-                             int32_t *count)
+// Positions are float4 records (x, y, z, m) or packed xyz triples.  Thread 0
+// also clears the count of large buckets for step 4.
+template <bool XYZ>
+__global__ void k_sort_count(const float *__restrict__ pos, int64_t np, int n, int32_t *key, int32_t *slot,
+                             int32_t *count, int32_t *big)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) big[0] = 0;
     const int lane = threadIdx.x & 63;
     int32_t c = -1;
     if (i < np) {
-        float4 p = pos[i];
-        c = (int32_t)cidx(n, cell_of(n, p.x), cell_of(n, p.y), cell_of(n, p.z));
+        float x, y, z;
+        if (XYZ) {
+            x = pos[3 * i];
+            y = pos[3 * i + 1];
+            z = pos[3 * i + 2];
+        } else {
+            float4 p = reinterpret_cast<const float4 *>(pos)[i];
+            x = p.x;
+            y = p.y;
+            z = p.z;
+        }
+        c = (int32_t)cidx(n, cell_of(n, x), cell_of(n, y), cell_of(n, z));
     }
     int32_t prev = __shfl_up(c, 1, 64);
     bool head = lane == 0 || prev != c;
@@ -148,16 +164,20 @@ __global__ void k_sort_scatter(int64_t np, const int32_t *__restrict__ key, cons
     if (i >= np) return;
     int32_t d = start[key[i]] + slot[i];
     perm[d] = (int32_t)i;
-    pid[d] = meta_id(meta[i]);
+    pid[d] = meta ? meta_id(meta[i]) : (int32_t)i;  // without records the input index is the creation index
 }
 
 // 4. creation order inside every bucket.  A bucket of at most SO_REG
 // particles is ranked in registers (ids are unique, so every rank is
 // distinct) and skipped when already in order, which the atomic slot
-// claim of step 1 mostly leaves it; larger buckets use an insertion sort.
+// claim of step 1 mostly leaves it.  A larger bucket is copied to (tpid,
+// tperm) (the key/slot arrays, free after the scatter) and listed in big
+// (big[0] = count, entries from big[1]); k_sort_order_big ranks it with
+// whole blocks.
 constexpr int SO_REG = 16;
 
-__global__ void k_sort_order(long C, const int32_t *__restrict__ start, int32_t *perm, int32_t *pid)
+__global__ void k_sort_order(long C, const int32_t *__restrict__ start, int32_t *perm, int32_t *pid, int32_t *tperm,
+                             int32_t *tpid, int32_t *big)
 {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
@@ -187,16 +207,51 @@ __global__ void k_sort_order(long C, const int32_t *__restrict__ start, int32_t 
         }
         return;
     }
-    for (int a = b + 1; a < e; ++a) {
-        int32_t kid = pid[a], kp = perm[a];
-        int t = a - 1;
-        while (t >= b && pid[t] > kid) {
-            pid[t + 1] = pid[t];
-            perm[t + 1] = perm[t];
-            --t;
+    for (int a = b; a < e; ++a) {
+        tpid[a] = pid[a];
+        tperm[a] = perm[a];
+    }
+    int slot = atomicAdd(&big[0], 1);
+    big[1 + slot] = (int32_t)c;
+}
+
+// Large buckets: chunk ch of 256 particles of listed bucket q goes to the
+// block with (ch + q) % gridDim.x == blockIdx.x, so one huge bucket spreads
+// over the whole grid.  A particle's rank is the number of smaller ids in
+// its bucket, counted against the bucket staged through LDS 256 ids at a
+// time (ids are unique, so the ranks are a permutation).
+constexpr int OB = 256;
+
+__global__ __launch_bounds__(OB) void k_sort_order_big(const int32_t *__restrict__ start,
+                                                       const int32_t *__restrict__ big,
+                                                       const int32_t *__restrict__ tperm,
+                                                       const int32_t *__restrict__ tpid, int32_t *perm, int32_t *pid)
+{
+    __shared__ int32_t ids[OB];
+    const int nbig = big[0];
+    for (int q = 0; q < nbig; ++q) {
+        const int c = big[1 + q];
+        const int b = start[c], e = start[c + 1];
+        const int chunks = (e - b + OB - 1) / OB;
+        int ch = (int)blockIdx.x - q % (int)gridDim.x;
+        if (ch < 0) ch += gridDim.x;
+        for (; ch < chunks; ch += gridDim.x) {
+            const int a = b + ch * OB + (int)threadIdx.x;
+            const bool mine = a < e;
+            const int32_t key = mine ? tpid[a] : 0;
+            int rank = 0;
+            for (int t0 = b; t0 < e; t0 += OB) {
+                __syncthreads();
+                if (t0 + (int)threadIdx.x < e) ids[threadIdx.x] = tpid[t0 + threadIdx.x];
+                __syncthreads();
+                const int lim = min(OB, e - t0);
+                for (int u = 0; u < lim; ++u) rank += ids[u] < key;
+            }
+            if (mine) {
+                pid[b + rank] = key;
+                perm[b + rank] = tperm[a];
+            }
         }
-        pid[t + 1] = kid;
-        perm[t + 1] = kp;
     }
 }
 
@@ -227,17 +282,23 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
-void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks)
+namespace {
+void scan_on(hipStream_t st, const int32_t *in, int32_t *out, long n, int32_t *blocks)
 {
     if (n <= 0) {
-        hipLaunchKernelGGL(k_fill_i32, 1, 1, 0, c->stream, out, 1L, 0);
+        hipLaunchKernelGGL(k_fill_i32, 1, 1, 0, st, out, 1L, 0);
         return;
     }
-    if (!blocks) blocks = c->scan_blocks;
     int nb = (int)((n + TILE - 1) / TILE);
-    hipLaunchKernelGGL(k_scan_reduce, nb, SB, 0, c->stream, in, n, blocks);
-    hipLaunchKernelGGL(k_scan_top, 1, SB, 0, c->stream, blocks, nb, out + n);
-    hipLaunchKernelGGL(k_scan_apply, nb, SB, 0, c->stream, in, n, blocks, out);
+    hipLaunchKernelGGL(k_scan_reduce, nb, SB, 0, st, in, n, blocks);
+    hipLaunchKernelGGL(k_scan_top, 1, SB, 0, st, blocks, nb, out + n);
+    hipLaunchKernelGGL(k_scan_apply, nb, SB, 0, st, in, n, blocks, out);
+}
+}  // namespace
+
+void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks)
+{
+    scan_on(c->stream, in, out, n, blocks ? blocks : c->scan_blocks);
 }
 
 void k_launch_sort(FlipCtx *c)
@@ -246,8 +307,8 @@ void k_launch_sort(FlipCtx *c)
     t_begin(c, "sort_count");
     hipLaunchKernelGGL(k_fill_i32, nblk(c->C, 256), 256, 0, c->stream, c->cell_count, c->C, 0);
     if (np > 0)
-        hipLaunchKernelGGL(k_sort_count, nblk(np, 256), 256, 0, c->stream, c->P4(), np, c->n, c->pkey, c->pslot,
-                           c->cell_count);
+        hipLaunchKernelGGL(k_sort_count<false>, nblk(np, 256), 256, 0, c->stream, (const float *)c->P4(), np, c->n,
+                           c->pkey, c->pslot, c->cell_count, c->sort_big);
     t_end(c);
     t_begin(c, "sort_scan");
     k_launch_scan(c, c->cell_count, c->cell_start, c->C);
@@ -258,7 +319,10 @@ void k_launch_sort(FlipCtx *c)
     int32_t *pid = c->deep;  // scratch sized max(C, np) + 1: creation index per bucket slot
     hipLaunchKernelGGL(k_sort_scatter, nblk(np, 256), 256, 0, c->stream, np, c->pkey, c->pslot, c->cell_start,
                        c->M(), c->perm, pid);
-    hipLaunchKernelGGL(k_sort_order, nblk(c->C, 256), 256, 0, c->stream, c->C, c->cell_start, c->perm, pid);
+    hipLaunchKernelGGL(k_sort_order, nblk(c->C, 256), 256, 0, c->stream, c->C, c->cell_start, c->perm, pid, c->pslot,
+                       c->pkey, c->sort_big);
+    hipLaunchKernelGGL(k_sort_order_big, 256, OB, 0, c->stream, c->cell_start, c->sort_big, c->pslot, c->pkey,
+                       c->perm, pid);
     t_end(c);
     t_begin(c, "sort_gather");
     int nx = c->cur ^ 1;
