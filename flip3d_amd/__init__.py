@@ -28,6 +28,7 @@ EXPORTS = [
     "flip_gpu_num_particles", "flip_gpu_set_max_dens", "flip_gpu_get_max_dens", "flip_gpu_srand",
     "flip_gpu_set_dt", "flip_gpu_get_grid", "flip_gpu_set_grid", "flip_gpu_step", "flip_gpu_stage",
     "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
+    "flip_gpu_sort_cells_noalloc",
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
     "flip_gpu_save_state", "flip_gpu_load_state", "flip_gpu_step_many",
     "flip_gpu_set_emitter", "flip_gpu_set_step", "flip_gpu_get_step", "flip_gpu_mesh", "flip_gpu_mesh_get",
@@ -109,6 +110,7 @@ def lib():
         L.flip_gpu_map_g2p.argtypes = [P, P, P, P, C.c_int64, P, P]
         L.flip_gpu_solve.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(Stats)]
         L.flip_gpu_sorted_cells.argtypes = [P, P, P]
+        L.flip_gpu_sort_cells_noalloc.argtypes = [P, C.c_int64, C.c_int, P, P, P]
         L.flip_gpu_kernel_times.argtypes = [P, C.c_char_p, C.c_int, P, C.c_int]
         L.flip_gpu_set_profiling.argtypes = [P, C.c_int]
         L.flip_gpu_build_scene.argtypes = [P, C.c_int, C.c_uint64, C.POINTER(C.c_int64)]

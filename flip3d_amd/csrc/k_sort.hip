@@ -301,6 +301,29 @@ void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t 
     scan_on(c->stream, in, out, n, blocks ? blocks : c->scan_blocks);
 }
 
+namespace {
+
+// the launch sequence of steps 1-4 on caller-provided buffers
+void launch_bucket_sort(hipStream_t st, bool xyz, const float *pos, int64_t np, int n, long C, int32_t *count,
+                        int32_t *start, int32_t *scan_blocks, int32_t *key, int32_t *slot, int32_t *perm,
+                        int32_t *pid, const int32_t *meta, int32_t *big)
+{
+    hipLaunchKernelGGL(k_fill_i32, nblk(C, 256), 256, 0, st, count, C, 0);
+    if (np > 0) {
+        if (xyz)
+            hipLaunchKernelGGL(k_sort_count<true>, nblk(np, 256), 256, 0, st, pos, np, n, key, slot, count, big);
+        else
+            hipLaunchKernelGGL(k_sort_count<false>, nblk(np, 256), 256, 0, st, pos, np, n, key, slot, count, big);
+    }
+    scan_on(st, count, start, C, scan_blocks);
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_sort_scatter, nblk(np, 256), 256, 0, st, np, key, slot, start, meta, perm, pid);
+    hipLaunchKernelGGL(k_sort_order, nblk(C, 256), 256, 0, st, C, start, perm, pid, slot, key, big);
+    hipLaunchKernelGGL(k_sort_order_big, 256, OB, 0, st, start, big, slot, key, perm, pid);
+}
+
+}  // namespace
+
 void k_launch_sort(FlipCtx *c)
 {
     int64_t np = c->np;
@@ -330,4 +353,61 @@ void k_launch_sort(FlipCtx *c)
                        c->vel[c->cur], c->meta[c->cur], c->pos[nx], c->vel[nx], c->meta[nx], c->inv, c->xyz3);
     c->cur = nx;
     t_end(c);
+}
+
+// ------------------------------------------------ sorting with no allocation
+// The scratch of flip_gpu_sort_cells_noalloc: one fixed array per GPU (each
+// device holds its own copy for the life of the process), laid out as
+//   key[NA_NP] | slot[NA_NP] | start[NA_C + 1] | scan blocks | big list
+// Host code never passes the array's name to a kernel: it is a host-side
+// stand-in, not a device address.  Each device's address comes from
+// hipGetSymbolAddress on the first call on that device.
+namespace {
+constexpr int64_t NA_NP = int64_t(1) << 20;                // particles
+constexpr long NA_C = 1L << 20;                            // cells (n^3)
+constexpr long NA_BLOCKS = NA_C / TILE + 8;                // scan block sums
+constexpr long NA_BIG = NA_NP / (SO_REG + 1) + 2;          // big-bucket list (count + entries)
+constexpr size_t NA_WORDS = 2 * (size_t)NA_NP + (NA_C + 1) + NA_BLOCKS + NA_BIG;
+
+__device__ __align__(256) int32_t scratch[NA_WORDS];
+
+constexpr int NA_MAX_DEV = 64;
+std::mutex g_na_mu;
+int32_t *g_na_ptr[NA_MAX_DEV] = {};
+}  // namespace
+
+extern "C" int flip_gpu_sort_cells_noalloc(const float *pos, int64_t np, int n, int32_t *counts, int32_t *ids,
+                                           void *stream)
+{
+    if (n < 1 || np < 0 || (np > 0 && (!pos || !ids)) || !counts)
+        return flip_internal_fail(FLIP_E_ARG, "sort_cells_noalloc: bad argument");
+    if (np > NA_NP)
+        return flip_internal_fail(FLIP_E_ARG, "sort_cells_noalloc: np exceeds the scratch limit of 1048576 particles");
+    const long C = (long)n * n * n;
+    if (n > 1024 || C > NA_C)
+        return flip_internal_fail(FLIP_E_ARG, "sort_cells_noalloc: n^3 exceeds the scratch limit of 1048576 cells");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEV)
+        return flip_internal_fail(FLIP_E_HIP, "sort_cells_noalloc: no current HIP device");
+    int32_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_na_mu);
+        if (!g_na_ptr[dev]) {
+            void *p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return flip_internal_fail(FLIP_E_HIP, "sort_cells_noalloc: hipGetSymbolAddress of the scratch failed");
+            g_na_ptr[dev] = static_cast<int32_t *>(p);
+        }
+        base = g_na_ptr[dev];
+    }
+    int32_t *key = base, *slot = key + NA_NP, *start = slot + NA_NP, *blocks = start + (NA_C + 1),
+            *big = blocks + NA_BLOCKS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // ids doubles as the scatter's source-index and creation-index arrays:
+    // the input index is the creation index, so both hold the same values
+    launch_bucket_sort(st, true, pos, np, n, C, counts, start, blocks, key, slot, ids, ids, nullptr, big);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return flip_internal_fail(FLIP_E_HIP, hipGetErrorString(e));
+    return FLIP_OK;
 }

@@ -215,6 +215,20 @@ int flip_gpu_solve(flip_ctx *ctx, const int8_t *A, const float *L, float *x, flo
  * cell (n^3) and particle indices (reference order) cell by cell (np). */
 int flip_gpu_sorted_cells(flip_ctx *ctx, int32_t *counts, int32_t *ids);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the buckets of flip_gpu_sorted_cells, computed by the same
+ * kernels with no context and no allocation.  All pointers are device memory
+ * owned by the caller: pos holds 3*np floats (xyz per particle, the particle
+ * order is the creation order), counts receives n^3 per-cell counts and ids
+ * the np particle indices cell by cell in creation order.  Scratch is a fixed
+ * array of 12.3 MiB per GPU inside the library (held until the process ends),
+ * so np <= 1048576 and n^3 <= 1048576; larger sizes fail with FLIP_E_ARG.  The
+ * kernels are queued on `stream` (a hipStream_t; NULL = the null stream) and
+ * the call returns without waiting.  Every call on a GPU shares that GPU's
+ * scratch: calls that overlap on one GPU race. */
+int flip_gpu_sort_cells_noalloc(const float *pos, int64_t np, int n, int32_t *counts, int32_t *ids, void *stream);
+
 /* Per-kernel device timings of the last step (ms), in launch-name order;
  * names is filled with a ';'-separated list.  Returns the count. */
 int flip_gpu_kernel_times(flip_ctx *ctx, char *names, int names_cap, double *ms, int ms_cap);
