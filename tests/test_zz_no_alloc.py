@@ -26,25 +26,58 @@ def _reference_path(n, p):
         return g.sorted_cells()
 
 
+class _Hip:
+    """Device buffers through the HIP runtime the library itself links
+    (libamdhip64.so.7 of /opt/rocm; a second runtime in the process, such as
+    one bundled with another package, would not share its devices)."""
+
+    def __init__(self):
+        fa.lib()
+        self.h = ctypes.CDLL("libamdhip64.so.7")
+        self.bufs = []
+
+    def _ok(self, rc, what):
+        assert rc == 0, f"{what}: HIP error {rc}"
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        self._ok(self.h.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(a.nbytes, 4))), "hipMalloc")
+        self.bufs.append(p)
+        self._ok(self.h.hipMemcpy(p, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.nbytes), 1), "hipMemcpy")
+        return p
+
+    def get(self, p, like):
+        out = np.empty_like(like)
+        self._ok(self.h.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        self._ok(self.h.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), p, ctypes.c_size_t(out.nbytes), 2), "hipMemcpy")
+        return out
+
+    def close(self):
+        for p in self.bufs:
+            self.h.hipFree(p)
+        self.bufs = []
+
+
 def test_noalloc_sort_matches_context_sort_twice():
-    import torch
     lib = fa.lib()
     rs = np.random.RandomState(11)
-    for n, m, crowd in ((96, 900000, 5000), (40, 150000, 700)):
-        p = _positions(rs, m, crowd)
-        want_counts, want_ids = _reference_path(n, p)
-        dp = torch.from_numpy(p.reshape(-1)).cuda()
-        dc = torch.full((n ** 3,), -7, dtype=torch.int32, device="cuda")
-        di = torch.full((m,), -7, dtype=torch.int32, device="cuda")
-        torch.cuda.synchronize()
-        rc = lib.flip_gpu_sort_cells_noalloc(ctypes.c_void_p(dp.data_ptr()), m, n, ctypes.c_void_p(dc.data_ptr()),
-                                             ctypes.c_void_p(di.data_ptr()), None)
-        assert rc == 0, lib.flip_gpu_last_error()
-        torch.cuda.synchronize()
-        counts, ids = dc.cpu().numpy(), di.cpu().numpy()
-        assert counts.max() >= crowd
-        assert np.array_equal(counts, want_counts), n
-        assert np.array_equal(ids, want_ids), n
+    hip = _Hip()
+    try:
+        for n, m, crowd in ((96, 900000, 5000), (40, 150000, 700)):
+            p = _positions(rs, m, crowd)
+            want_counts, want_ids = _reference_path(n, p)
+            counts0 = np.full(n ** 3, -7, np.int32)
+            ids0 = np.full(m, -7, np.int32)
+            dp, dc, di = hip.put(p.reshape(-1)), hip.put(counts0), hip.put(ids0)
+            rc = lib.flip_gpu_sort_cells_noalloc(dp, m, n, dc, di, None)
+            assert rc == 0, lib.flip_gpu_last_error()
+            counts, ids = hip.get(dc, counts0), hip.get(di, ids0)
+            assert counts.max() >= crowd
+            assert np.array_equal(counts, want_counts), n
+            assert np.array_equal(ids, want_ids), n
+    finally:
+        hip.close()
 
 
 def test_noalloc_sort_rejects_sizes_beyond_the_scratch():
