@@ -95,6 +95,49 @@ void wall_normal_grid(int N, const std::vector<RefParticle> &ps, std::vector<flo
     wn.swap(out);
 }
 
+// render/save/objects.dat sits beside the <step>_scene.dat files
+std::string objects_path(const char *path)
+{
+    std::string p(path);
+    size_t slash = p.find_last_of('/');
+    return (slash == std::string::npos ? std::string() : p.substr(0, slash + 1)) + "objects.dat";
+}
+
+// saveState's object list (flip3D.cpp:498-508), in its text format
+bool write_objects(const std::string &path, const std::vector<FlipCtx::SceneObject> &obj)
+{
+    File f;
+    f.f = fopen(path.c_str(), "w");
+    if (!f.f) return false;
+    bool ok = fprintf(f.f, "%d\n", (int)obj.size()) > 0;
+    for (const auto &o : obj) {
+        ok = ok && fprintf(f.f, "%d %d %d %d\n", o.type, o.shape, o.material, o.visible) > 0;
+        ok = ok && fprintf(f.f, "%f %f %f %f %f %f %f %f %f %f\n", o.f[0], o.f[1], o.f[2], o.f[3], o.f[4], o.f[5],
+                           o.f[6], o.f[7], o.f[8], o.f[9]) > 0;
+    }
+    return ok;
+}
+
+// loadState's read of it (flip3D.cpp:561-574); false when the file is absent
+// or malformed
+bool read_objects(const std::string &path, std::vector<FlipCtx::SceneObject> &obj)
+{
+    File f;
+    f.f = fopen(path.c_str(), "r");
+    if (!f.f) return false;
+    int count = 0;
+    if (fscanf(f.f, "%d\n", &count) != 1 || count < 0 || count > 1000000) return false;
+    std::vector<FlipCtx::SceneObject> out(count);
+    for (auto &o : out) {
+        if (fscanf(f.f, "%d %d %d %d\n", &o.type, &o.shape, &o.material, &o.visible) != 4) return false;
+        if (fscanf(f.f, "%f %f %f %f %f %f %f %f %f %f\n", &o.f[0], &o.f[1], &o.f[2], &o.f[3], &o.f[4], &o.f[5],
+                   &o.f[6], &o.f[7], &o.f[8], &o.f[9]) != 10)
+            return false;
+    }
+    obj.swap(out);
+    return true;
+}
+
 }  // namespace
 
 extern "C" int flip_gpu_save_state(flip_ctx *ctx, const char *path)
@@ -150,6 +193,11 @@ extern "C" int flip_gpu_save_state(flip_ctx *ctx, const char *path)
         ok = ok && fwrite(press.data(), 4, c->C, f.f) == (size_t)c->C;
         ok = ok && fwrite(wn.data(), 4, wn.size(), f.f) == wn.size();
         if (!ok) return flip_internal_fail(FLIP_E_ARG, "short write to checkpoint");
+        if (!c->objects_saved) {  // the first save of the context, as saveState's firstTime
+            if (!write_objects(objects_path(path), c->objects))
+                return flip_internal_fail(FLIP_E_ARG, "cannot write objects.dat beside the checkpoint");
+            c->objects_saved = true;
+        }
         return FLIP_OK;
     } catch (const flip_hip_error &e) {
         return flip_internal_fail(FLIP_E_HIP, e.what.c_str());
@@ -204,6 +252,14 @@ extern "C" int flip_gpu_load_state(flip_ctx *ctx, const char *path)
         // the emitter travels with the state (flip3D.cpp:536-538); the caller
         // sets the step number, which the reference takes from the file name
         set_emitter(c, pour_pos[0], pour_pos[1], pour_rad, pour_time);
+        // the object list on the context's first load, as loadState's firstTime
+        // (the reference reads it unchecked; a missing or malformed file keeps
+        // the context's list)
+        if (!c->objects_loaded) {
+            std::vector<FlipCtx::SceneObject> obj;
+            if (read_objects(objects_path(path), obj)) c->objects = obj;
+            c->objects_loaded = true;
+        }
         return FLIP_OK;
     } catch (const flip_hip_error &e) {
         return flip_internal_fail(FLIP_E_HIP, e.what.c_str());

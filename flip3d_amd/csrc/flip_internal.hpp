@@ -41,6 +41,8 @@ void learned_free(LearnedSolver *ls);
 
 // Device buffers of one context.  Particles live in cell-sorted order and are
 // re-permuted after every sort; meta carries creation index << 2 | type.
+constexpr int EV_NONFINITE = 2;  // slot of ev_count set when a step writes a non-finite particle
+
 struct FlipCtx {
     flip_params prm;
     int n = 0;
@@ -112,13 +114,22 @@ struct FlipCtx {
 
     // event buffers (host-drawn rand() consumers)
     int32_t *ev_list = nullptr;  // (slot) of particles with events
-    int32_t *ev_count = nullptr;
+    int32_t *ev_count = nullptr;  // [0] event count, [EV_NONFINITE] a non-finite particle was written
     float *ev_rand = nullptr;    // rand()%101 values, 3 per event
     int32_t *ev_off = nullptr;   // per listed particle: offset into ev_rand
     int64_t ev_cap = 0;
     int32_t *h_pinned = nullptr;  // pinned host scalars
 
     std::string learned_weights;  // U-Net weights file for FLIP_PRECOND_LEARNED
+    // the scene's object list as saveState writes it to objects.dat
+    // (flip3D.cpp:498-508): type, shape, material, visible, then r, c[3],
+    // p[0][3], p[1][3]; written on the context's first save
+    struct SceneObject {
+        int type, shape, material, visible;
+        float f[10];
+    };
+    std::vector<SceneObject> objects;
+    bool objects_saved = false, objects_loaded = false;
     LearnedSolver *learned = nullptr;
 
     float max_dens = 1.0f;

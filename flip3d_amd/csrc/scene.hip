@@ -287,6 +287,91 @@ static float calibrate_max_dens(flip_ctx *ctx, FlipCtx *c)
     return md;
 }
 
+// objects.dat records (flip3D.cpp:498-508) of a scene.  Each reference
+// scene function fills one reused stack Object, so a field it set for an
+// earlier object carries over to the next; fields it never sets are
+// indeterminate in the reference and written as 0 here.
+namespace {
+using SceneObject = FlipCtx::SceneObject;
+
+SceneObject rec_of(const Obj &b, int material, int visible)
+{
+    SceneObject r{};
+    r.type = b.type;
+    r.shape = b.shape;
+    r.material = material;
+    r.visible = visible;
+    r.f[0] = b.r;
+    for (int d = 0; d < 3; ++d) {
+        r.f[1 + d] = b.c[d];
+        r.f[4 + d] = b.lo[d];
+        r.f[7 + d] = b.hi[d];
+    }
+    return r;
+}
+
+// the second object of a scene function inherits the first's fields that it does not set
+SceneObject carry(SceneObject second, const SceneObject &first, bool sets_r_c, bool sets_p, bool sets_material,
+                  bool sets_visible)
+{
+    if (!sets_r_c)
+        for (int t = 0; t < 4; ++t) second.f[t] = first.f[t];
+    if (!sets_p)
+        for (int t = 4; t < 10; ++t) second.f[t] = first.f[t];
+    if (!sets_material) second.material = first.material;
+    if (!sets_visible) second.visible = first.visible;
+    return second;
+}
+
+// placeWalls (flip3D.cpp:300-362): GLASS walls, a GRAY floor, none visible
+void wall_records(const std::vector<Obj> &o, std::vector<SceneObject> &out)
+{
+    for (int w = 0; w < 6; ++w) out.push_back(rec_of(o[w], w == 2 ? 2 : 1, 0));
+}
+
+void scene_records(int scene, const std::vector<Obj> &o, std::vector<SceneObject> &out)
+{
+    wall_records(o, out);
+    if (o.size() < 8) {
+        for (size_t t = 6; t < o.size(); ++t) out.push_back(rec_of(o[t], 0, 1));
+        return;
+    }
+    const Obj &a = o[6], &b = o[7];
+    switch (scene) {
+    case FLIP_SCENE_REFERENCE:  // damBreakTest: both boxes set type, shape, visible, p
+        out.push_back(rec_of(a, 0, 1));
+        out.push_back(rec_of(b, 0, 1));
+        break;
+    case FLIP_SCENE_SPHERE_POUR: {  // a RED wall sphere, then a box (r, c, material carried)
+        SceneObject s0 = rec_of(a, 3, 1);
+        out.push_back(s0);
+        out.push_back(carry(rec_of(b, 0, 1), s0, false, true, false, true));
+        break;
+    }
+    case FLIP_SCENE_CLIFF_POUR: {  // a RED wall box, then a box (material carried)
+        SceneObject s0 = rec_of(a, 3, 1);
+        out.push_back(s0);
+        out.push_back(carry(rec_of(b, 0, 1), s0, true, true, false, true));
+        break;
+    }
+    case FLIP_SCENE_WATER_DROP: {  // a box, then a sphere (p carried); visible never set
+        SceneObject s0 = rec_of(a, 0, 0);
+        out.push_back(s0);
+        out.push_back(carry(rec_of(b, 0, 0), s0, true, false, false, false));
+        break;
+    }
+    case FLIP_SCENE_BENCHMARK: {  // a box, then a sphere (p carried)
+        SceneObject s0 = rec_of(a, 0, 1);
+        out.push_back(s0);
+        out.push_back(carry(rec_of(b, 0, 1), s0, true, false, false, true));
+        break;
+    }
+    default:
+        for (size_t t = 6; t < o.size(); ++t) out.push_back(rec_of(o[t], 0, 1));
+    }
+}
+}  // namespace
+
 extern "C" int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np_out)
 {
     if (!ctx) return -1;
@@ -392,11 +477,15 @@ extern "C" int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int
             return flip_internal_fail(FLIP_E_ARG, "unknown scene id");
         }
         finish(s, N);
+        std::vector<SceneObject> recs;
+        scene_records(scene, o, recs);
         float md = calibrate_max_dens(ctx, c);
         int rc = flip_gpu_upload_particles(ctx, (int64_t)s.size(), s.p.data(), nullptr, s.nrm.data(), s.type.data(),
                                            nullptr, nullptr);
         if (rc) return rc;
         c->max_dens = md;
+        c->objects = recs;
+        c->objects_saved = false;
         if (np_out) *np_out = (int64_t)s.size();
         return FLIP_OK;
     } catch (const flip_hip_error &e) {

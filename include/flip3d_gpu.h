@@ -261,16 +261,21 @@ int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int64_t *np);
  * max_dens, int pourTime, float pourPos[2], float pourRad, count 72-byte
  * particle records (src/common.h:34-45) in creation order, the pressure grid
  * (n^3 floats, [i][j][k]), four wall-normal floats per cell.  Saving writes
- * the emitter fields as the reference's defaults (-1, {0,0}, 0.12) and the
- * wall-normal grid as compute_wall_normal (flip3D.cpp:75-164) builds it;
- * loading takes particles, max_dens and pressure, checks the file size
- * against n, and ignores the emitter fields and the wall-normal grid (wall
- * normals travel in the particle records).  objects.dat is not read or written. */
+ * the context's emitter (pour time, disc position and radius) and the
+ * wall-normal grid as compute_wall_normal (flip3D.cpp:75-164) builds it; on
+ * the context's first save it also writes objects.dat beside the file, the
+ * scene's object list in saveState's text format (flip3D.cpp:498-508; fields
+ * the reference leaves indeterminate are written as 0).  Loading takes the
+ * particles, max_dens, pressure and emitter, checks the file size against n,
+ * takes the wall normals from the particle records (the file's wall-normal
+ * grid is derived from them), and on the context's first load reads
+ * objects.dat from beside the file when it is present. */
 int flip_gpu_save_state(flip_ctx *ctx, const char *path);
 int flip_gpu_load_state(flip_ctx *ctx, const char *path);
 
 /* Weights file (flip_unet_save layout) of the coarse-level network used by
- * FLIP_PRECOND_LEARNED; the network runs on the (n/2)^3 coarse grid, so n
+ * FLIP_PRECOND_LEARNED; the network solves the coarsest level of the
+ * multigrid hierarchy (32^3 for n = 64, 128 and 256; 16^3 for n = 32), so n
  * must be a multiple of 16.  No counterpart in the reference. */
 int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path);
 

@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 SECTIONS = ["kernels", "sort", "p2g", "g2p", "correct", "solve", "scene_ref", "traj_ref", "traj_bench",
             "checkpoint", "scenes_test", "mesh", "exports", "scene_bench_ref",
-            "solve_unit"]
+            "solve_unit", "objects0", "objects1", "objects2", "objects3", "objects4"]
 
 
 def digest(a):
@@ -290,6 +290,20 @@ def sec_traj_bench(out):
     _trajectory(R, 100, (1, 10), out)
 
 
+def sec_objects(which):
+    """render/save/objects.dat as saveState writes it on its first save
+    (flip3D.cpp:498-508), for the scene `which` of ref_init_test (0 damBreak,
+    1 spherePour, 2 cliffPour, 3 waterDrop, 4 the benchmark objects): the
+    reference's own text, kept as tests/golden/objects_<which>.txt."""
+    R = ref()
+    with Quiet():
+        R.ref_init_test(which, 1)
+        R.ref_save_state(1)
+    text = open(os.path.join("render", "save", "objects.dat")).read()
+    with open(os.path.join(HERE, f"objects_{which}.txt"), "w") as fh:
+        fh.write(text)
+
+
 def sec_solve_unit(out):
     """solver::solve on a system with P == 1.0 cells (tests/pocket_system.py),
     where the reference's value-test couplings are live: its own result
@@ -427,7 +441,10 @@ def run_section(name):
     cwd = tempfile.mkdtemp(prefix="flipref_")
     os.chdir(cwd)  # the reference writes render/ into its working directory
     os.makedirs(os.path.join(cwd, "render", "log"), exist_ok=True)  # solver.cpp:209 logs there
-    globals()["sec_" + name](out if not name.startswith("traj_bench") else os.path.join(HERE, name))
+    if name.startswith("objects"):
+        sec_objects(int(name[len("objects"):]))
+    else:
+        globals()["sec_" + name](out if not name.startswith("traj_bench") else os.path.join(HERE, name))
     print("wrote", name)
 
 

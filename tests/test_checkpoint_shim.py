@@ -135,3 +135,63 @@ def test_shim_matches_c_abi():
     r = subprocess.run([SHIM_CHECK], cwd=d, capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
+
+
+# objects.dat (flip3D.cpp:498-508): per object a line "type shape material
+# visible" and a line "r c0 c1 c2 p00 p01 p02 p10 p11 p12".  Each reference
+# scene function fills one reused stack Object, so only the fields it sets
+# (or carries from an earlier object of the same function) are defined; the
+# masks below list them (1 = compared), from placeWalls (:300-362) and the
+# scene functions (:203-297; the benchmark objects of ref_harness.cpp).
+_WALL = ([1, 1, 1, 1], [0, 0, 0, 0, 1, 1, 1, 1, 1, 1])
+_OBJ_MASKS = {
+    0: [([1, 1, 0, 1], [0, 0, 0, 0, 1, 1, 1, 1, 1, 1])] * 2,                      # damBreakTest: two boxes
+    1: [([1, 1, 1, 1], [1, 1, 1, 1, 0, 0, 0, 0, 0, 0]),                         # spherePourTest: wall sphere,
+        ([1, 1, 1, 1], [1, 1, 1, 1, 1, 1, 1, 1, 1, 1])],                        #   box (material, r, c carried)
+    2: [([1, 1, 1, 1], [0, 0, 0, 0, 1, 1, 1, 1, 1, 1])] * 2,                      # cliffPourTest: wall box, box
+    3: [([1, 1, 0, 0], [0, 0, 0, 0, 1, 1, 1, 1, 1, 1]),                         # waterDropTest: box,
+        ([1, 1, 0, 0], [1, 1, 1, 1, 1, 1, 1, 1, 1, 1])],                        #   sphere (p carried)
+    4: [([1, 1, 0, 1], [0, 0, 0, 0, 1, 1, 1, 1, 1, 1]),                         # benchmark: box,
+        ([1, 1, 0, 1], [1, 1, 1, 1, 1, 1, 1, 1, 1, 1])],                        #   sphere (p carried)
+}
+_SCENE_OF = {0: fa.SCENE_REFERENCE, 1: fa.SCENE_SPHERE_POUR, 2: fa.SCENE_CLIFF_POUR, 3: fa.SCENE_WATER_DROP,
+             4: fa.SCENE_BENCHMARK}
+
+
+def _objects(text):
+    lines = text.strip().split("\n")
+    count = int(lines[0])
+    assert len(lines) == 1 + 2 * count
+    return [(lines[1 + 2 * t].split(), lines[2 + 2 * t].split()) for t in range(count)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", [0, 1, 2, 3, 4])
+def test_objects_dat_matches_reference(which, tmp_path):
+    """The first save of a context writes objects.dat beside the checkpoint:
+    the reference's eight objects (six walls, two scene objects) with every
+    field the reference defines equal to the reference's own file, token for
+    token; a context that loads the pair and saves again writes the same
+    file."""
+    ref = _objects(open(os.path.join(ROOT, "tests", "golden", f"objects_{which}.txt")).read())
+    masks = [_WALL] * 6 + _OBJ_MASKS[which]
+    d1, d2 = tmp_path / "a", tmp_path / "b"
+    d1.mkdir()
+    d2.mkdir()
+    with fa.FlipContext(32) as g:
+        g.build_scene(_SCENE_OF[which])
+        g.save_state(str(d1 / "1_scene.dat"))
+    mine_text = open(d1 / "objects.dat").read()
+    mine = _objects(mine_text)
+    assert len(mine) == len(ref) == 8
+    for t, ((a1, a2), (b1, b2), (m1, m2)) in enumerate(zip(mine, ref, masks)):
+        for u in range(4):
+            if m1[u]:
+                assert a1[u] == b1[u], (which, t, u, a1, b1)
+        for u in range(10):
+            if m2[u]:
+                assert a2[u] == b2[u], (which, t, u, a2, b2)
+    with fa.FlipContext(32) as g:
+        g.load_state(str(d1 / "1_scene.dat"))
+        g.save_state(str(d2 / "1_scene.dat"))
+    assert open(d2 / "objects.dat").read() == mine_text
