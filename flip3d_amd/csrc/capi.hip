@@ -439,6 +439,7 @@ static void run_stage(FlipCtx *c, int stage, flip_step_stats *st)
         k_launch_g2p(c);
         break;
     case FLIP_STAGE_ADVECT: {
+        FLIP_CHECK(hipMemsetAsync(c->ev_count + EV_NONFINITE, 0, sizeof(int32_t), c->stream));
         k_launch_advect(c);
         c->sorted = false;
         int m = k_stuck_and_reposition(c);  // leaves the particles sorted
@@ -917,11 +918,16 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         k_launch_extrapolate(c);
         k_launch_g2p(c, true);  // reads the extrapolation's (u, u - u_save) pairs
         run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
+        FLIP_CHECK(hipMemsetAsync(c->ev_count + EV_NONFINITE, 0, sizeof(int32_t), c->stream));
         if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
+        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 8, c->ev_count + EV_NONFINITE, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
         FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));
         FLIP_CHECK(hipEventSynchronize(c->ev_b));
         FLIP_CHECK(hipGetLastError());
         st->ms_step = elapsed(c->ev_a, c->ev_b);
+        if (c->h_pinned[8])
+            return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
         timing_collect(c);
         return FLIP_OK;
     });

@@ -22,6 +22,10 @@ constexpr int TB = 256;
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 __device__ __forceinline__ bool is_wall_w(float w) { return __builtin_signbit(w); }
+__device__ __forceinline__ bool finite3(float a, float b, float c)
+{
+    return __builtin_isfinite(a) && __builtin_isfinite(b) && __builtin_isfinite(c);
+}
 __device__ __forceinline__ float mass_of(float w) { return __builtin_fabsf(w); }
 
 // ------------------------------------------------------------- density
@@ -612,7 +616,7 @@ __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__rest
                                                const float *__restrict__ u0, const float *__restrict__ u1,
                                                const float *__restrict__ u2, double dt, float rlo, double re,
                                                const int32_t *__restrict__ wst, const float4 *__restrict__ wpos,
-                                               const float4 *__restrict__ wnrm)
+                                               const float4 *__restrict__ wnrm, int32_t *nonfinite)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
@@ -620,6 +624,9 @@ __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__rest
     if (is_wall_w(p.w)) return;
     float4 v = vel[i];
     float3 u = fetch_velocity(u0, u1, u2, n, p.x, p.y, p.z);
+    // the clamp below maps a NaN position into the domain, so the check
+    // looks at the inputs of the move
+    if (!finite3(u.x, u.y, u.z) || !finite3(v.x, v.y, v.z) || !finite3(p.x, p.y, p.z)) atomicOr(nonfinite, 1);
     p.x = (float)((double)p.x + dt * (double)u.x);
     p.y = (float)((double)p.y + dt * (double)u.y);
     p.z = (float)((double)p.z + dt * (double)u.z);
@@ -1257,7 +1264,7 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
 
 // corrector.cpp:103-114 update (tmp1 null: every velocity is kept)
 __global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const float4 *__restrict__ tmp0,
-                                 const float4 *__restrict__ tmp1)
+                                 const float4 *__restrict__ tmp1, int32_t *nonfinite)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
@@ -1265,10 +1272,14 @@ __global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const flo
     if (is_wall_w(p.w)) return;
     float4 a = tmp0[i];
     pos[i] = make_float4(a.x, a.y, a.z, p.w);
-    if (!tmp1) return;
-    float4 b = tmp1[i];
-    float4 v = vel[i];
-    vel[i] = make_float4(b.x, b.y, b.z, v.w);
+    bool ok = finite3(a.x, a.y, a.z);
+    if (tmp1) {
+        float4 b = tmp1[i];
+        float4 v = vel[i];
+        vel[i] = make_float4(b.x, b.y, b.z, v.w);
+        ok = ok && finite3(b.x, b.y, b.z);
+    }
+    if (!ok) atomicOr(nonfinite, 1);
 }
 
 // max |u|^2 over FLUID particles: non-negative doubles order like their bit
@@ -1428,7 +1439,7 @@ void k_launch_advect(FlipCtx *c)
     if (c->np > 0)
         launch_timed(c, "advect", k_advect, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0], c->ug[1],
                      c->ug[2], c->dt, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n, c->wall_start,
-                     c->wall_pos, c->wall_nrm);
+                     c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE);
 }
 
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s)
@@ -1465,9 +1476,11 @@ int k_stuck_and_reposition(FlipCtx *c)
         hipLaunchKernelGGL(k_stuck, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->V4(), c->M(), c->A,
                            c->ev_list, c->ev_count);
         t_end(c);
-        FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         FLIP_CHECK(hipStreamSynchronize(c->stream));
         m = c->h_pinned[0];
+        if (c->h_pinned[EV_NONFINITE])
+            throw flip_arg_error{FLIP_E_NUMERIC, "non-finite particle position or velocity in the advection"};
     }
     int moved = 0;
     if (m > 0) {
@@ -1617,6 +1630,6 @@ int k_correct(FlipCtx *c)
         launch_timed(c, "resample", k_resample, nblk(c->np, RESAMPLE_TB), RESAMPLE_TB, c->np, n, c->P4(), c->V4(),
                      c->cell_start, c->tmp0, c->tmp1, hh, reinterpret_cast<const P3 *>(c->xyz3));
     launch_timed(c, "correct_update", k_correct_update, nblk(c->np, TB), TB, c->np, c->P4(), c->V4(), c->tmp0,
-                 keep ? nullptr : c->tmp1);
+                 keep ? nullptr : c->tmp1, c->ev_count + EV_NONFINITE);
     return events;
 }

@@ -381,3 +381,23 @@ def test_p2g_general_mass():
         grids = g.map_p2g()
     for c, w in enumerate((po.G_U0, po.G_U1, po.G_U2)):
         assert np.array_equal(grids[c], s.grid(w)), c
+
+
+@pytest.mark.parametrize("where", ["velocity", "position"])
+def test_nonfinite_particle_is_reported(where):
+    """A NaN velocity or position in the particle state (forced on the host)
+    ends the step with FLIP_E_NUMERIC instead of running on: the advection
+    checks the values it moves, the corrector the values it writes."""
+    with fa.FlipContext(16, dt_policy=0) as g:
+        g.build_scene(fa.SCENE_BENCHMARK)
+        q = g.download()
+        fl = np.flatnonzero(q["type"] == po.FLUID)
+        p, u = q["p"].copy(), q["u"].copy()
+        if where == "velocity":
+            u[fl[7]] = [np.nan, 0.0, np.inf]
+        else:
+            p[fl[7], 1] = np.nan
+        g.upload(p, u, q["n"], q["type"], dens=q["dens"])
+        with pytest.raises(fa.FlipError) as e:
+            g.step()
+        assert "non-finite" in str(e.value) or "NaN" in str(e.value)
