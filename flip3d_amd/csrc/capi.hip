@@ -214,7 +214,12 @@ void alloc_grids(FlipCtx *c)
     dalloc(&c->tickets, 8);
     FLIP_CHECK(hipMemset(c->tickets, 0, 8 * sizeof(unsigned)));
     dalloc(&c->ev_count, 4);
+    dalloc(&c->dtv, 4);
     FLIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(int32_t)));
+    FLIP_CHECK(hipHostMalloc((void **)&c->h_status, sizeof(FlipCtx::SolveStatus), hipHostMallocMapped));
+    FLIP_CHECK(hipHostGetDevicePointer((void **)&c->d_status, c->h_status, 0));
+    memset(c->h_status, 0, sizeof(FlipCtx::SolveStatus));
+    for (auto &e : c->ev_batch) FLIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 void free_all(FlipCtx *c)
@@ -266,8 +271,13 @@ void free_all(FlipCtx *c)
     dfree(c->iflags);
     dfree(c->tickets);
     dfree(c->ev_count);
+    dfree(c->dtv);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     c->h_pinned = nullptr;
+    if (c->h_status) (void)hipHostFree(c->h_status);
+    c->h_status = c->d_status = nullptr;
+    for (auto &e : c->ev_batch)
+        if (e) (void)hipEventDestroy(e);
     for (auto e : c->tev) (void)hipEventDestroy(e);
     c->tev.clear();
     hipEvent_t evs[4] = {c->ev_a, c->ev_b, c->ev_s0, c->ev_s1};
@@ -424,9 +434,9 @@ static void run_stage(FlipCtx *c, int stage, flip_step_stats *st)
         }
         int rc = k_solve(c, st);
         if (c->ev_s0 && (c->profiling || st)) {
+            // read once the step (or stage call) has finished
             FLIP_CHECK(hipEventRecord(c->ev_s1, c->stream));
-            FLIP_CHECK(hipEventSynchronize(c->ev_s1));
-            if (st) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
+            c->solve_timed = true;
         }
         if (rc == FLIP_E_NUMERIC) throw flip_arg_error{FLIP_E_NUMERIC, "pressure solve produced NaN"};
         k_launch_gradient(c);
@@ -457,21 +467,6 @@ static void run_stage(FlipCtx *c, int stage, flip_step_stats *st)
     default:
         throw flip_arg_error{FLIP_E_ARG, "unknown stage"};
     }
-}
-
-static void choose_dt(FlipCtx *c)
-{
-    if (c->prm.dt_policy == 0) {
-        c->dt = c->prm.dt;
-        return;
-    }
-    double umax = k_max_speed(c);
-    double dt = c->prm.dt;
-    if (umax > 0.0) {
-        double cfl = c->prm.cfl / (c->n * umax);
-        if (cfl < dt) dt = cfl;
-    }
-    c->dt = dt;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -795,9 +790,13 @@ int flip_gpu_stage(flip_ctx *ctx, int stage, flip_step_stats *st)
         FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
         require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
         if (st) memset(st, 0, sizeof(*st));
+        k_launch_set_dt(c, c->dt);
+        c->solve_timed = false;
         run_stage(c, stage, st);
         FLIP_CHECK(hipStreamSynchronize(c->stream));
         FLIP_CHECK(hipGetLastError());
+        if (st && c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
+        c->solve_timed = false;
         return FLIP_OK;
     });
 }
@@ -905,8 +904,7 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         if (!c->begun) begin_step(c);
         c->begun = false;
         FLIP_CHECK(hipEventRecord(c->ev_a, c->stream));
-        choose_dt(c);
-        st->dt = c->dt;
+        k_launch_choose_dt(c);  // dt stays on the device; read back with the step's end
         // flip3D.cpp:1221-1222 sort + computeDensity
         k_launch_sort(c);
         c->sorted = true;
@@ -922,10 +920,15 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
         FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 8, c->ev_count + EV_NONFINITE, sizeof(int32_t), hipMemcpyDeviceToHost,
                                   c->stream));
+        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 16, c->dtv, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));
         FLIP_CHECK(hipEventSynchronize(c->ev_b));
         FLIP_CHECK(hipGetLastError());
+        memcpy(&c->dt, c->h_pinned + 16, sizeof(double));
+        st->dt = c->dt;
         st->ms_step = elapsed(c->ev_a, c->ev_b);
+        if (c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
+        c->solve_timed = false;
         if (c->h_pinned[8])
             return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
         timing_collect(c);

@@ -96,6 +96,15 @@ struct FlipCtx {
     uint32_t *flmeta = nullptr;  // per FLUID cell: neighbour kinds (2 bits x 6) + has-AIR bit
     hipGraphExec_t pcg_graph = nullptr;  // captured batch of PCG iterations
     int pcg_graph_subcell = -1;
+    // the solver's status after each batch of iterations, written by the
+    // device into mapped host memory (no copy, read after the batch's event)
+    struct SolveStatus {
+        int32_t done, iters, capped, unit, nfl, pad[3];
+        double err2, err2_0, bnorm2;
+    };
+    SolveStatus *h_status = nullptr, *d_status = nullptr;
+    hipEvent_t ev_batch[2] = {nullptr, nullptr};
+    bool solve_timed = false;  // ev_s0/ev_s1 bracket a solve whose time is not read yet
     hipGraphExec_t ic_graph = nullptr;  // captured batch of wavefront-preconditioned iterations
     int ic_graph_key = -1;              // subcell * 2 + preconditioner mode it was captured for
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
@@ -114,6 +123,7 @@ struct FlipCtx {
 
     // event buffers (host-drawn rand() consumers)
     int32_t *ev_list = nullptr;  // (slot) of particles with events
+    double *dtv = nullptr;        // [0] the time step the kernels use (chosen on the device)
     int32_t *ev_count = nullptr;  // [0] event count, [EV_NONFINITE] a non-finite particle was written
     float *ev_rand = nullptr;    // rand()%101 values, 3 per event
     int32_t *ev_off = nullptr;   // per listed particle: offset into ev_rand
@@ -198,11 +208,13 @@ int k_solve(FlipCtx *c, flip_step_stats *st);   // PCG on (A, L, press, div)
 // FLIP_PRECOND_LEARNED: flexible CG from the current press and r (after the
 // PCG init); returns its iterations, *converged when ||r||^2 <= eps
 int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *converged);
+int learned_fcg_iterations(FlipCtx *c);
 int learned_levels(int n);  // coarse levels of the learned preconditioner for side n
 void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const float *g2, int64_t m,
                          const float4 *p, float4 *out);
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s);
-double k_max_speed(FlipCtx *c);
+void k_launch_choose_dt(FlipCtx *c);            // the step's dt into dtv[0] on the device
+void k_launch_set_dt(FlipCtx *c, double dt);    // a given dt into dtv[0]
 
 // records the message for flip_gpu_last_error and returns code
 int flip_internal_fail(int code, const char *msg);

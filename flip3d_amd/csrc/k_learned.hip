@@ -431,7 +431,10 @@ __global__ __launch_bounds__(TB) void k_ml_post0_b(const int32_t *__restrict__ i
                                                    float2 *B0, float2 *B1, double *part, int32_t *hflag)
 {
     // the stop flag into mapped host memory (null outside the iteration)
-    if (hflag && blockIdx.x == 0 && threadIdx.x == 0) hflag[0] = fi[FI_DONE];
+    if (hflag && blockIdx.x == 0 && threadIdx.x == 0) {
+        hflag[1] = fi[FI_K];
+        hflag[0] = fi[FI_DONE];
+    }
     if (fi[FI_DONE]) return;
     const int nfl = iflags[I_NFL];
     float2 *B = ((fi[FI_K] + 1) & 1) ? B1 : B0;
@@ -453,6 +456,25 @@ __global__ __launch_bounds__(TB) void k_ml_post0_b(const int32_t *__restrict__ i
 }
 
 // ------------------------------------------------------------ flexible CG
+// the flexible CG's device state: iteration 0, the solve's stop threshold; it
+// does not start (done at once) when the factor has P == 1.0 cells (the
+// wavefront PCG then solves), when there is no FLUID cell, or when the start
+// residual already meets a relative tolerance
+__global__ void k_fcg_begin(const int32_t *__restrict__ iflags, const double *__restrict__ scal, double *fs,
+                            int32_t *fi)
+{
+    const int t = threadIdx.x;
+    if (t < 8) fs[t] = 0.0;
+    if (t < 4) fi[t] = 0;
+    __syncthreads();
+    if (t == 0) {
+        fs[F_EPS] = scal[S_EPS];
+        const bool skip = iflags[I_UNIT] > 0 || iflags[I_NFL] == 0 ||
+                          (iflags[I_TOLPOL] == 1 && scal[S_ERR2_0] <= scal[S_EPS]);
+        fi[FI_DONE] = skip ? 1 : 0;
+    }
+}
+
 // part a(k): z.r_k from the preconditioner partials, beta, s_k = z + beta
 // s_{k-1} (neighbours recomputed), q = A s_k, partials of q.s
 __global__ __launch_bounds__(TB) void k_fcg_a(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
@@ -705,10 +727,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
         hipLaunchKernelGGL(k_ml_coarsen, nb(L.C), TBL, 0, c->stream, L.R, Rf, Af, Lf, L.A, L.L);
         hipLaunchKernelGGL(k_ml_meta, nb(L.C), TBL, 0, c->stream, L.R, L.A, L.L, subcell, L.h2, L.meta, L.dinv);
     }
-    double hs[8] = {0, 0, 0, 0, eps, 0, 0, 0};
-    int32_t hf[4] = {0, 0, 0, 0};
-    FLIP_CHECK(hipMemcpyAsync(ls->fs, hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
-    FLIP_CHECK(hipMemcpyAsync(ls->fi, hf, sizeof(hf), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_fcg_begin, 1, 64, 0, c->stream, c->iflags, c->scal, ls->fs, ls->fi);
     t_end(c);
     vcycle(c, ls, G, h2, subcell, nullptr, pz, pb, false);  // z_0 into B_1.x
 
@@ -752,9 +771,16 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
             if (flag[0]) break;
         }
     }
-    int32_t hfi[4];
-    FLIP_CHECK(hipMemcpyAsync(hfi, ls->fi, sizeof(hfi), hipMemcpyDeviceToHost, c->stream));
-    FLIP_CHECK(hipStreamSynchronize(c->stream));
-    *converged = hfi[FI_CONV] != 0;
-    return hfi[FI_K];
+    // the iteration count and outcome are read after the solve's next wait
+    // (learned_fcg_iterations)
+    *converged = false;
+    (void)eps;
+    return 0;
+}
+
+// flexible-CG iterations of the last learned solve, from the stop flag's
+// mapped record (valid once the stream has passed its last iteration)
+int learned_fcg_iterations(FlipCtx *c)
+{
+    return c->learned ? c->learned->h_flag[1] : 0;
 }

@@ -47,12 +47,13 @@ struct P3 {
 template <bool POW2, bool FORCE, bool PACKED>
 __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, const int32_t *__restrict__ cs, float h,
-                                                float inv_hh, float max_dens, double gdt,
-                                                const P3 *__restrict__ xyz3)
+                                                float inv_hh, float max_dens, const double *__restrict__ dtv,
+                                                double gravity, const P3 *__restrict__ xyz3)
 {
     static_assert(!PACKED || POW2, "the packed pass uses the power-of-two kernel");
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
+    const double gdt = FORCE ? -dtv[0] * gravity : 0.0;  // add_ExtForce's -DT * GRAVITY
     float4 p = pos[i];
     if (is_wall_w(p.w)) {
         if (FORCE) {
@@ -107,10 +108,11 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
 }
 
 // flip3D.cpp:870-875 add_ExtForce: u_y = (float)((double)u_y + gdt), gdt = -dt*g
-__global__ void k_ext_force(int64_t np, float4 *vel, double gdt)
+__global__ void k_ext_force(int64_t np, float4 *vel, const double *__restrict__ dtv, double gravity)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
+    const double gdt = -dtv[0] * gravity;
     vel[i].y = (float)((double)vel[i].y + gdt);
 }
 
@@ -614,7 +616,8 @@ __global__ void k_g2p_points(int64_t m, int n, const float4 *__restrict__ p, con
 // move, so the static wall bins are the reference's post-advect wall buckets.
 __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__restrict__ pos, float4 *__restrict__ vel,
                                                const float *__restrict__ u0, const float *__restrict__ u1,
-                                               const float *__restrict__ u2, double dt, float rlo, double re,
+                                               const float *__restrict__ u2, const double *__restrict__ dtv,
+                                               float rlo, double re,
                                                const int32_t *__restrict__ wst, const float4 *__restrict__ wpos,
                                                const float4 *__restrict__ wnrm, int32_t *nonfinite)
 {
@@ -624,6 +627,7 @@ __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__rest
     if (is_wall_w(p.w)) return;
     float4 v = vel[i];
     float3 u = fetch_velocity(u0, u1, u2, n, p.x, p.y, p.z);
+    const double dt = dtv[0];
     // the clamp below maps a NaN position into the domain, so the check
     // looks at the inputs of the move
     if (!finite3(u.x, u.y, u.z) || !finite3(v.x, v.y, v.z) || !finite3(p.x, p.y, p.z)) atomicOr(nonfinite, 1);
@@ -1047,9 +1051,11 @@ constexpr int SPRING_NEAR = 16;
 __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const float4 *__restrict__ pos,
                                                       const int32_t *__restrict__ meta,
                                                       const float4 *__restrict__ nrm_id,
-                                                      const int32_t *__restrict__ cs, float dt, float re, float4 *tmp0,
+                                                      const int32_t *__restrict__ cs,
+                                                      const double *__restrict__ dtv, float re, float4 *tmp0,
                                                       int32_t *list, int32_t *count)
 {
+    const float dt = (float)dtv[0];
     __shared__ int32_t near[SPRING_NEAR][SPRING_TB];
     __shared__ int32_t s_off[9 * SPRING_TB / 64], s_idx[SPRING_TB];
     const int64_t base = (int64_t)blockIdx.x * SPRING_TB;
@@ -1152,11 +1158,12 @@ __global__ void k_spring_count(int m, const int32_t *__restrict__ list, int n, c
 
 __global__ void k_spring_fix(int m, const int32_t *__restrict__ list, const int32_t *__restrict__ off, int n,
                              const float4 *__restrict__ pos, const int32_t *__restrict__ meta,
-                             const float4 *__restrict__ nrm_id, const int32_t *__restrict__ cs, float dt, float re,
-                             const float *__restrict__ rv, float4 *tmp0)
+                             const float4 *__restrict__ nrm_id, const int32_t *__restrict__ cs,
+                             const double *__restrict__ dtv, float re, const float *__restrict__ rv, float4 *tmp0)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
+    const float dt = (float)dtv[0];
     spring_one<true>(list[t], n, pos, meta, nrm_id, cs, dt, re, rv + off[t], tmp0);
 }
 
@@ -1282,10 +1289,9 @@ __global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const flo
     if (!ok) atomicOr(nonfinite, 1);
 }
 
-// max |u|^2 over FLUID particles: non-negative doubles order like their bit
-// patterns, so a 64-bit atomicMax of the bits is exact and order-free
-__global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                         unsigned long long *out)
+// max |u|^2 over FLUID particles: per-block maxima (the FP64 max is exact and
+// order-free), then k_choose_dt forms the time step on the device
+__global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float4 *__restrict__ vel, double *part)
 {
     __shared__ double lds[TB / 64];
     double m = 0.0;
@@ -1300,7 +1306,31 @@ __global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float
     if (threadIdx.x == 0) {
         double r = 0.0;
         for (int w = 0; w < TB / 64; ++w) r = fmax(r, lds[w]);
-        if (r == r) atomicMax(out, (unsigned long long)__double_as_longlong(r));
+        part[blockIdx.x] = r;
+    }
+}
+
+// dt policy 1: dt = min(cap, cfl / (N max|u|)) (no FLUID motion: the cap);
+// nb == 0 writes the cap (policy 0).  fmax drops NaN speeds, as the max does.
+__global__ __launch_bounds__(TB) void k_choose_dt(int nb, const double *__restrict__ part, double cap, double cfl,
+                                                  int n, double *dtv)
+{
+    __shared__ double lds[TB / 64];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += TB) m = fmax(m, part[b]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < TB / 64; ++w) r = fmax(r, lds[w]);
+        const double umax = sqrt(r);
+        double dt = cap;
+        if (umax > 0.0) {
+            const double c = cfl / (n * umax);
+            if (c < dt) dt = c;
+        }
+        dtv[0] = dt;
     }
 }
 
@@ -1351,21 +1381,21 @@ void k_launch_density(FlipCtx *c, bool with_force)
     int e = 0;
     const bool pow2 = std::isnormal(hh) && std::frexp(hh, &e) == 0.5f;
     const float inv_hh = pow2 ? std::ldexp(1.0f, 1 - e) : 0.0f;  // hh = 2^(e-1)
-    const double gdt = -c->dt * c->prm.gravity;
     // the sort's packed records are current while the storage is sorted
     const bool packed = pow2 && c->unit_mass && c->sorted;
     auto kern = packed ? (with_force ? k_density<true, true, true> : k_density<true, false, true>)
                        : pow2 ? (with_force ? k_density<true, true, false> : k_density<true, false, false>)
                               : (with_force ? k_density<false, true, false> : k_density<false, false, false>);
     launch_timed(c, with_force ? "density_force" : "density", kern, nblk(c->np, TB), TB, c->np, c->n, c->P4(),
-                 c->V4(), c->cell_start, h, inv_hh, c->max_dens, gdt, reinterpret_cast<const P3 *>(c->xyz3));
+                 c->V4(), c->cell_start, h, inv_hh, c->max_dens, c->dtv, c->prm.gravity,
+                 reinterpret_cast<const P3 *>(c->xyz3));
 }
 
 void k_launch_ext_force(FlipCtx *c)
 {
     t_begin(c, "ext_force");
     if (c->np > 0)
-        hipLaunchKernelGGL(k_ext_force, nblk(c->np, TB), TB, 0, c->stream, c->np, c->V4(), -c->dt * c->prm.gravity);
+        hipLaunchKernelGGL(k_ext_force, nblk(c->np, TB), TB, 0, c->stream, c->np, c->V4(), c->dtv, c->prm.gravity);
     t_end(c);
 }
 
@@ -1438,7 +1468,7 @@ void k_launch_advect(FlipCtx *c)
 {
     if (c->np > 0)
         launch_timed(c, "advect", k_advect, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0], c->ug[1],
-                     c->ug[2], c->dt, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n, c->wall_start,
+                     c->ug[2], c->dtv, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n, c->wall_start,
                      c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE);
 }
 
@@ -1447,17 +1477,21 @@ void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hi
     if (m > 0) hipLaunchKernelGGL(k_selftest, nblk(m, TB), TB, 0, s, m, x, y, out);
 }
 
-double k_max_speed(FlipCtx *c)
+// the step's time step into c->dtv[0] on the device (no host round trip)
+void k_launch_choose_dt(FlipCtx *c)
 {
-    if (c->np == 0) return 0.0;
-    unsigned nb = std::min(nblk(c->np, TB), 1024u);  // one atomic per block
-    unsigned long long *out = (unsigned long long *)(c->scal + 15);
-    FLIP_CHECK(hipMemsetAsync(out, 0, sizeof(unsigned long long), c->stream));
-    hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), out);
-    double h = 0.0;
-    FLIP_CHECK(hipMemcpyAsync(&h, c->scal + 15, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    FLIP_CHECK(hipStreamSynchronize(c->stream));
-    return sqrt(h);
+    unsigned nb = 0;
+    if (c->prm.dt_policy == 1 && c->np > 0) {
+        nb = std::min(nblk(c->np, TB), 1024u);
+        hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), c->partials);
+    }
+    hipLaunchKernelGGL(k_choose_dt, 1, TB, 0, c->stream, (int)nb, c->partials, c->prm.dt, c->prm.cfl, c->n, c->dtv);
+}
+
+// a host-chosen time step (stage calls) into c->dtv[0]
+void k_launch_set_dt(FlipCtx *c, double dt)
+{
+    hipLaunchKernelGGL(k_choose_dt, 1, TB, 0, c->stream, 0, c->partials, dt, 0.0, c->n, c->dtv);
 }
 
 // Stuck detection and repositioning (flip3D.cpp:932-968, 405-451).  The
@@ -1581,11 +1615,10 @@ int k_correct(FlipCtx *c)
     // the sort's packed records (resample) are current while the storage is sorted
     if (!c->sorted) throw flip_arg_error{FLIP_E_STATE, "the corrector needs sorted particles"};
     int n = c->n;
-    float dt = (float)c->dt;
     float re = (float)(c->prm.density / n);
     FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, sizeof(int32_t), c->stream));
     launch_timed(c, "spring", k_spring, nblk(c->np, SPRING_TB), SPRING_TB, c->np, n, c->P4(), c->M(), c->nrm_id,
-                 c->cell_start, dt, re, c->tmp0, c->ev_list, c->ev_count);
+                 c->cell_start, c->dtv, re, c->tmp0, c->ev_list, c->ev_count);
     FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     FLIP_CHECK(hipStreamSynchronize(c->stream));
     int m = c->h_pinned[0];
@@ -1616,7 +1649,7 @@ int k_correct(FlipCtx *c)
         FLIP_CHECK(hipMemcpyAsync(c->ev_rand, rv.data(), sizeof(float) * rv.size(), hipMemcpyHostToDevice, c->stream));
         FLIP_CHECK(hipMemcpyAsync(c->ev_off, off.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_spring_fix, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_off, n, c->P4(), c->M(),
-                           c->nrm_id, c->cell_start, dt, re, c->ev_rand, c->tmp0);
+                           c->nrm_id, c->cell_start, c->dtv, re, c->ev_rand, c->tmp0);
         FLIP_CHECK(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
     }
     // With re^2 <= 1e-5 (N >= 159) every resample weight m * (re^2 /

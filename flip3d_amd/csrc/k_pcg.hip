@@ -92,18 +92,39 @@ __global__ void k_fluid_compact(long C, int n, const int8_t *__restrict__ A, con
 }
 
 // per-cell factorisation diagonal (solver.cpp:125-139 with no active coupling)
+// (nfl < 0: the FLUID count is read from iflags)
 __global__ void k_build_diag(int nfl, const int32_t *__restrict__ fl, int n, const int8_t *__restrict__ A,
                              const float *__restrict__ L, int subcell, double *P, int32_t *iflags)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nfl) return;
-    long c = fl[t];
-    double diag = (double)a_diag(n, c, A, L, subcell);
-    double e = diag - 0.0 - 0.0 - 0.0;
-    if (e < 0.25 * diag) e = diag;
-    double p = 1.0 / (double)sqrtf((float)e);
-    P[c] = p;
-    if (p == 1.0) atomicAdd(&iflags[I_UNIT], 1);
+    if (nfl < 0) nfl = iflags[I_NFL];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) {
+        long c = fl[t];
+        double diag = (double)a_diag(n, c, A, L, subcell);
+        double e = diag - 0.0 - 0.0 - 0.0;
+        if (e < 0.25 * diag) e = diag;
+        double p = 1.0 / (double)sqrtf((float)e);
+        P[c] = p;
+        if (p == 1.0) atomicAdd(&iflags[I_UNIT], 1);
+    }
+}
+
+// The solve's device flags and scalars (no host copy): the FLUID count from
+// the compaction's scan total (or nfl_host when nfl_src is null), the stop
+// policy and iteration cap, the iteration-0 test and speculation switches.
+__global__ void k_pcg_begin(int32_t *iflags, double *scal, const int32_t *__restrict__ nfl_src, int nfl_host,
+                            int tolpol, int cap, int start, int spec)
+{
+    const int t = threadIdx.x;
+    if (t < 16) iflags[t] = 0;
+    if (t < 16) scal[t] = 0.0;
+    __syncthreads();
+    if (t == 0) {
+        iflags[I_NFL] = nfl_src ? nfl_src[0] : nfl_host;
+        iflags[I_TOLPOL] = tolpol;
+        iflags[I_CAP] = cap;
+        iflags[I_START] = start;
+        iflags[I_SPEC] = spec;
+    }
 }
 
 // r = b - A x (op with a = -1), z = M r, s = z; partials r.r, z.r, b.b
@@ -113,8 +134,9 @@ __global__ __launch_bounds__(TB) void k_pcg_init(int nfl, const int32_t *__restr
                                                  const float *__restrict__ L, int subcell, double h2,
                                                  const float *__restrict__ x, const float *__restrict__ b,
                                                  const double *__restrict__ P, float *r, float2 *zs,
-                                                 double *part)
+                                                 double *part, const int32_t *__restrict__ iflags)
 {
+    if (nfl < 0) nfl = iflags[I_NFL];
     double rr = 0.0, zr = 0.0, bb = 0.0;
     for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
         long c = fl[t];
@@ -137,17 +159,26 @@ __global__ __launch_bounds__(TB) void k_pcg_init(int nfl, const int32_t *__restr
     store_partials(part, 3, rr, zr, bb, 3);
 }
 
-// single block: initial scalars from the init partials
-__global__ __launch_bounds__(TB) void k_pcg_init_fin(int G, const double *__restrict__ part, double *scal)
+// single block: initial scalars from the init partials.  first: the solve's
+// start (initial residual, |b|^2 and the stop threshold of solver.cpp:323,
+// 1e-2 n^3, or tol^2 |b|^2); otherwise a restart after the learned solve
+// (the true residual of its result, compared by pcg_a's iteration-0 test).
+__global__ __launch_bounds__(TB) void k_pcg_init_fin(int G, const double *__restrict__ part, double *scal,
+                                                     const int32_t *__restrict__ iflags, int first, double eps_abs,
+                                                     double tol2)
 {
     double rr = sum_partials(part, G, 3, 0);
     double zr = sum_partials(part, G, 3, 1);
     double bb = sum_partials(part, G, 3, 2);
     if (threadIdx.x == 0) {
-        scal[S_ERR2_0] = rr;
         scal[S_A0] = zr;
-        scal[S_BNORM2] = bb;
         scal[S_ERR2] = rr;
+        scal[S_RR0] = rr;
+        if (first) {
+            scal[S_ERR2_0] = rr;
+            scal[S_BNORM2] = bb;
+            scal[S_EPS] = iflags[I_TOLPOL] == 0 ? eps_abs : tol2 * bb;
+        }
     }
 }
 
@@ -221,6 +252,27 @@ __global__ __launch_bounds__(TB) void k_pcg_a(const int32_t *__restrict__ fl, co
     if (iflags[I_DONE]) return;
     const int k = iflags[I_KA];
     const int nfl = iflags[I_NFL];
+    if (k == 0) {
+        // speculative per-cell iterations on a factor with P == 1.0 cells:
+        // stop at once (the host reruns the solve with the wavefront sweeps)
+        const bool unit = iflags[I_SPEC] && iflags[I_UNIT] > 0;
+        // the iteration-0 test: no FLUID cell, or the start residual already
+        // meets the stop threshold (no iteration runs)
+        bool capped;
+        const bool start = !unit && (nfl == 0 || (iflags[I_START] && stop_test(scal[S_RR0], 0, iflags, scal, capped)));
+        if (unit || start) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                if (start) {
+                    const double r0 = nfl == 0 ? 0.0 : scal[S_RR0];
+                    scal[S_ERR2] = iflags[I_TOLPOL] == 0 ? (double)(float)r0 : r0;
+                }
+                iflags[I_ITERS] = 0;
+                iflags[I_CAPPED] = 0;
+                iflags[I_DONE] = 1;
+            }
+            return;
+        }
+    }
     const float2 *__restrict__ rd = (k & 1) ? B0 : B1;
     float2 *wr = (k & 1) ? B1 : B0;
     // the first item's loads go out before the partial-sum prologue
@@ -352,22 +404,35 @@ __global__ __launch_bounds__(TB) void k_pcg_zr(int nfl, const int32_t *__restric
     if (threadIdx.x == 0) part[blockIdx.x * stride + off] = v;
 }
 
-// single block: stop test after the last pcg_b of a batch
-__global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb, double *scal, int32_t *iflags, int G)
+// single block: stop test after the last pcg_b of a batch, then the solve's
+// status into mapped host memory (status may be null)
+__global__ __launch_bounds__(TB) void k_pcg_check(const double *__restrict__ pb, double *scal, int32_t *iflags, int G,
+                                                  FlipCtx::SolveStatus *status)
 {
-    if (iflags[I_DONE]) return;
-    const int k = iflags[I_KA];  // completed iterations
-    double rr, zr_unused;
-    sum_partials2(pb, G, rr, zr_unused);
-    bool capped;
-    bool stop = stop_test(rr, k, iflags, scal, capped);
-    if (threadIdx.x == 0) {
-        scal[S_ERR2] = iflags[I_TOLPOL] == 0 ? (double)(float)rr : rr;
-        iflags[I_ITERS] = k;
-        if (stop) {
-            iflags[I_DONE] = 1;
-            iflags[I_CAPPED] = capped ? 1 : 0;
+    if (!iflags[I_DONE]) {
+        const int k = iflags[I_KA];  // completed iterations
+        double rr, zr_unused;
+        sum_partials2(pb, G, rr, zr_unused);
+        bool capped;
+        bool stop = stop_test(rr, k, iflags, scal, capped);
+        if (threadIdx.x == 0) {
+            scal[S_ERR2] = iflags[I_TOLPOL] == 0 ? (double)(float)rr : rr;
+            iflags[I_ITERS] = k;
+            if (stop) {
+                iflags[I_DONE] = 1;
+                iflags[I_CAPPED] = capped ? 1 : 0;
+            }
         }
+    }
+    if (status && threadIdx.x == 0) {
+        status->iters = iflags[I_ITERS];
+        status->capped = iflags[I_CAPPED];
+        status->unit = iflags[I_UNIT];
+        status->nfl = iflags[I_NFL];
+        status->err2 = scal[S_ERR2];
+        status->err2_0 = scal[S_ERR2_0];
+        status->bnorm2 = scal[S_BNORM2];
+        status->done = iflags[I_DONE];
     }
 }
 
@@ -538,10 +603,21 @@ __global__ __launch_bounds__(WT3) void k_ic_tile(int n, int nt, int D, int mode,
 }
 
 // cells with P == 1.0 exactly (the value test's live factors)
-__global__ void k_count_unit(int nfl, const int32_t *__restrict__ fl, const double *__restrict__ P, int32_t *iflags)
+__global__ void k_count_unit(const int32_t *__restrict__ fl, const double *__restrict__ P, int32_t *iflags)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nfl && P[fl[t]] == 1.0) atomicAdd(&iflags[I_UNIT], 1);
+    const int nfl = iflags[I_NFL];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x)
+        if (P[fl[t]] == 1.0) atomicAdd(&iflags[I_UNIT], 1);
+}
+
+// I_UNIT <- 0 and the iteration state cleared for a restart with the wavefront sweeps
+__global__ void k_pcg_restart(int32_t *iflags)
+{
+    if (threadIdx.x == 0) {
+        iflags[I_DONE] = iflags[I_ITERS] = iflags[I_CAPPED] = iflags[I_KA] = iflags[I_KB] = 0;
+        iflags[I_UNIT] = 0;
+        iflags[I_SPEC] = 0;
+    }
 }
 
 
@@ -562,12 +638,18 @@ void launch_pair(FlipCtx *c, bool jacobi, int G, double h2, int subcell, double 
 
 // ordered list of FLUID cells and their neighbour metadata (zeroes non-FLUID
 // press); returns the count (host sync)
-int k_build_fluid_list(FlipCtx *c)
+// the list without reading its length back (the count stays at scan_tmp[C])
+static void fluid_list_async(FlipCtx *c)
 {
     hipLaunchKernelGGL(k_fluid_flag, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->cell_count, c->press);
     k_launch_scan(c, c->cell_count, c->scan_tmp, c->C);
     hipLaunchKernelGGL(k_fluid_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->n, c->A, c->scan_tmp, c->fl,
                        c->flmeta);
+}
+
+int k_build_fluid_list(FlipCtx *c)
+{
+    fluid_list_async(c);
     FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->scan_tmp + c->C, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     FLIP_CHECK(hipStreamSynchronize(c->stream));
     c->nfl = c->h_pinned[0];
@@ -599,183 +681,194 @@ static void build_factor_wavefront(FlipCtx *c, int mode, int subcell)
 }
 
 // PCG on the context's A, L, press (x) and div (b, negated in place).  With
-// setup == false the caller has negated b and built the FLUID list.
+// setup == false the caller has negated b and built the FLUID list (c->nfl).
+//
+// The host does not wait during the setup: the FLUID count, the stop
+// threshold and the flags are formed on the device, the per-cell
+// (reference-form) factor is used speculatively, and pcg_a(0) stops the
+// solve at once if the factor has P == 1.0 cells, which the host learns at
+// its first wait and answers by rerunning the solve with the wavefront
+// sweeps.  Each batch of iterations ends with k_pcg_check writing the
+// solve's status to mapped host memory; the host keeps the next batch queued
+// while it waits for the previous one, so the device does not idle between
+// batches.
 static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
 {
-    int n = c->n;
-    long C = c->C;
-    int subcell = c->prm.subcell;
-    double h2 = (double)(float)(1.0 / ((double)n * n));
-    int mode = c->prm.precond == FLIP_PRECOND_IC0 ? 1 : 0;
+    const int n = c->n;
+    const int subcell = c->prm.subcell;
+    const double h2 = (double)(float)(1.0 / ((double)n * n));
+    const int mode = c->prm.precond == FLIP_PRECOND_IC0 ? 1 : 0;
+    const bool learned = c->prm.precond == FLIP_PRECOND_LEARNED;
+    const long cap = c->prm.max_iters > 0 ? c->prm.max_iters : (long)n * n * n;
+    const double eps_abs = 1.0e-2 * ((double)n * n * n);  // solver.cpp:323
+    const double tol2 = c->prm.tol * c->prm.tol;
+    const int G = pcg_grid(c);
+    double *pi = c->pcg_part, *pa = c->pcg_part + 3 * GMAX, *pb = c->pcg_part + 4 * GMAX;
+    FlipCtx::SolveStatus *hs = c->h_status;
 
     t_begin(c, "pcg_setup");
     if (setup) {
-        hipLaunchKernelGGL(k_negate, nblk(C, TB), TB, 0, c->stream, C, c->div);  // solver.cpp:231 flip(b)
-        k_build_fluid_list(c);
+        hipLaunchKernelGGL(k_negate, nblk(c->C, TB), TB, 0, c->stream, c->C, c->div);  // solver.cpp:231 flip(b)
+        fluid_list_async(c);
     }
-    const int nfl = c->nfl;
-    long cap = c->prm.max_iters > 0 ? c->prm.max_iters : (long)n * n * n;
-    int32_t hf[16] = {0};
-    hf[I_TOLPOL] = c->prm.tol_policy;
-    hf[I_CAP] = (int32_t)std::min<long>(cap, 0x7fffffff);
-    hf[I_NFL] = nfl;
-    FLIP_CHECK(hipMemcpyAsync(c->iflags, hf, sizeof(hf), hipMemcpyHostToDevice, c->stream));
-    double hs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    FLIP_CHECK(hipMemcpyAsync(c->scal, hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
-    if (st) st->n_fluid_cells = nfl;
-    if (nfl == 0) {
-        t_end(c);
-        if (st) {
-            st->iterations = 0;
-            st->error2 = st->error2_0 = st->bnorm2 = 0.0;
-        }
-        return 0;
-    }
-    unsigned nb = nblk(nfl, TB);
-    const int G = pcg_grid(c);
-    double *pi = c->pcg_part, *pa = c->pcg_part + 3 * GMAX, *pb = c->pcg_part + 4 * GMAX;
+    const int32_t *nfl_src = setup ? c->scan_tmp + c->C : nullptr;
+    const int start = (learned || c->prm.tol_policy == 1) ? 1 : 0;
+    hipLaunchKernelGGL(k_pcg_begin, 1, 64, 0, c->stream, c->iflags, c->scal, nfl_src, setup ? 0 : c->nfl,
+                       c->prm.tol_policy, (int)std::min<long>(cap, 0x7fffffff), start, mode == 0 ? 1 : 0);
     bool jacobi = mode == 0;
-    if (mode == 0) {
-        hipLaunchKernelGGL(k_build_diag, nb, TB, 0, c->stream, nfl, c->fl, n, c->A, c->L, subcell, c->P, c->iflags);
-        int32_t unit = 0;
-        FLIP_CHECK(hipMemcpyAsync(&unit, c->iflags + I_UNIT, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        if (st) st->n_precond_unit = unit;
-        jacobi = unit == 0;
-    }
-    if (!jacobi) {
-        build_factor_wavefront(c, mode, subcell);
-        // the count of live factors (P == 1.0) of the whole factorization
-        int32_t zero = 0, unit = 0;
-        FLIP_CHECK(hipMemcpyAsync(c->iflags + I_UNIT, &zero, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_count_unit, nb, TB, 0, c->stream, nfl, c->fl, c->P, c->iflags);
-        FLIP_CHECK(hipMemcpyAsync(&unit, c->iflags + I_UNIT, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        if (st) st->n_precond_unit = unit;
-    }
-    // the initial z goes to B_1.x, where pcg_a(0) reads it
     if (jacobi) {
-        hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
-                           c->press, c->div, c->P, c->r, c->zs[1], pi);
-    } else {
-        hipLaunchKernelGGL(k_pcg_init<false>, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
-                           c->press, c->div, c->P, c->r, c->zs[1], pi);
-        precond_wavefront(c, mode, c->r, c->z, nullptr);
-        hipLaunchKernelGGL(k_copy_fluid_x, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[1], nullptr);
-        hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pi, 3, 1, c->iflags);
+        hipLaunchKernelGGL(k_build_diag, G, TB, 0, c->stream, -1, c->fl, n, c->A, c->L, subcell, c->P, c->iflags);
+        hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, -1, c->fl, c->flmeta, n, c->L, subcell, h2,
+                           c->press, c->div, c->P, c->r, c->zs[1], pi, c->iflags);
+        hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal, c->iflags, 1, eps_abs, tol2);
     }
-    hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal);
-    // stop threshold: solver.cpp:323, or the relative one
-    double e0 = 0.0, b2 = 0.0;
-    FLIP_CHECK(hipMemcpyAsync(&e0, c->scal + S_ERR2_0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    FLIP_CHECK(hipMemcpyAsync(&b2, c->scal + S_BNORM2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    FLIP_CHECK(hipStreamSynchronize(c->stream));
-    double eps = c->prm.tol_policy == 0 ? 1.0e-2 * ((double)n * n * n) : c->prm.tol * c->prm.tol * b2;
-    FLIP_CHECK(hipMemcpyAsync(c->scal + S_EPS, &eps, sizeof(double), hipMemcpyHostToDevice, c->stream));
     t_end(c);
 
-    bool done = c->prm.tol_policy == 1 && e0 <= eps;
     int learned_iters = 0;
-    if (c->prm.precond == FLIP_PRECOND_LEARNED && jacobi && !done) {
-        // flexible CG with the learned two-grid preconditioner, then the
-        // reference PCG from its x: a fresh true residual, which ends the
-        // solve at once when it meets the tolerance (else PCG finishes)
+    bool learned_ran = false;
+    if (learned && jacobi) {
+        // flexible CG with the learned two-grid preconditioner (it does not
+        // start when the factor has P == 1.0 cells or the start residual
+        // meets the tolerance), then the reference PCG from its x: a fresh
+        // true residual, which pcg_a(0) accepts at once when it meets the
+        // tolerance (else PCG finishes)
         bool conv = false;
-        learned_iters = learned_fcg(c, G, h2, subcell, eps, &conv);
-        hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, nfl, c->fl, c->flmeta, n, c->L, subcell, h2,
-                           c->press, c->div, c->P, c->r, c->zs[1], pi);
-        hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal);
-        double e1 = 0.0;
-        FLIP_CHECK(hipMemcpyAsync(&e1, c->scal + S_ERR2_0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        done = c->prm.tol_policy == 0 ? (double)(float)e1 <= eps : e1 <= eps;
-        if (done) {
-            double e1d = c->prm.tol_policy == 0 ? (double)(float)e1 : e1;
-            FLIP_CHECK(hipMemcpyAsync(c->scal + S_ERR2, &e1d, sizeof(double), hipMemcpyHostToDevice, c->stream));
-            int32_t zero = 0;
-            FLIP_CHECK(hipMemcpyAsync(c->iflags + I_CAPPED, &zero, sizeof(int32_t), hipMemcpyHostToDevice,
-                                      c->stream));
-        }
+        learned_fcg(c, G, h2, subcell, 0.0, &conv);
+        learned_ran = true;
+        hipLaunchKernelGGL(k_pcg_init<true>, G, TB, 0, c->stream, -1, c->fl, c->flmeta, n, c->L, subcell, h2,
+                           c->press, c->div, c->P, c->r, c->zs[1], pi, c->iflags);
+        hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal, c->iflags, 0, eps_abs, tol2);
     }
-    if (st) st->learned_iterations = learned_iters;
-    // Iterations run as replays of a captured graph of BATCH iterations
-    // (Jacobi) or WBATCH iterations (wavefront sweeps) plus the stop check;
-    // profiling or direct launch launch kernel by kernel (one iteration per
-    // host check when profiling).
+
+    // the wavefront path: the flag-keyed IC(0) factor, or a reference-form
+    // factor with P == 1.0 cells (known after the speculative start)
+    auto setup_wavefront = [&](bool restart) {
+        if (restart) hipLaunchKernelGGL(k_pcg_restart, 1, 64, 0, c->stream, c->iflags);
+        build_factor_wavefront(c, mode, subcell);
+        hipLaunchKernelGGL(k_count_unit, G, TB, 0, c->stream, c->fl, c->P, c->iflags);  // the whole factor's count
+        hipLaunchKernelGGL(k_pcg_init<false>, G, TB, 0, c->stream, -1, c->fl, c->flmeta, n, c->L, subcell, h2,
+                           c->press, c->div, c->P, c->r, c->zs[1], pi, c->iflags);
+        precond_wavefront(c, mode, c->r, c->z, nullptr);
+        hipLaunchKernelGGL(k_copy_fluid_x, G, TB, 0, c->stream, 0, c->fl, c->z, c->zs[1], c->iflags);
+        hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, 0, c->fl, c->z, c->r, pi, 3, 1, c->iflags);
+        hipLaunchKernelGGL(k_pcg_init_fin, 1, TB, 0, c->stream, G, pi, c->scal, c->iflags, restart ? 0 : 1, eps_abs,
+                           tol2);
+    };
+    if (!jacobi) setup_wavefront(false);
+
     const bool use_graph = !c->profiling && !c->direct_launch;
-    const int wkey = subcell * 2 + mode;
-    if (use_graph && !jacobi && (!c->ic_graph || c->ic_graph_key != wkey)) {
-        if (c->ic_graph) {
-            (void)hipGraphExecDestroy(c->ic_graph);
-            c->ic_graph = nullptr;
-        }
-        hipGraph_t g;
-        FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int t = 0; t < WBATCH; ++t) {
-            launch_pair(c, false, G, h2, subcell, pa, pb);
-            precond_wavefront(c, mode, c->r, c->z, c->iflags);
-            hipLaunchKernelGGL(k_copy_fluid_x, G, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[t & 1], c->iflags);
-            hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
-        }
-        hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G);
-        FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
-        FLIP_CHECK(hipGraphInstantiate(&c->ic_graph, g, nullptr, nullptr, 0));
-        FLIP_CHECK(hipGraphDestroy(g));
-        c->ic_graph_key = wkey;
-    }
-    if (use_graph && jacobi && (!c->pcg_graph || c->pcg_graph_subcell != subcell)) {
-        if (c->pcg_graph) {
-            (void)hipGraphExecDestroy(c->pcg_graph);
-            c->pcg_graph = nullptr;
-        }
-        hipGraph_t g;
-        FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int t = 0; t < BATCH; ++t) launch_pair(c, true, G, h2, subcell, pa, pb);
-        hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G);
-        FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
-        FLIP_CHECK(hipGraphInstantiate(&c->pcg_graph, g, nullptr, nullptr, 0));
-        FLIP_CHECK(hipGraphDestroy(g));
-        c->pcg_graph_subcell = subcell;
-    }
-    int iters = 0;
-    int hk = 0;  // host mirror of the iteration index (parity of the z buffer)
-    while (!done) {
-        if (use_graph) {
-            FLIP_CHECK(hipGraphLaunch(jacobi ? c->pcg_graph : c->ic_graph, c->stream));
-        } else {
-            int batch = c->profiling ? 1 : 4;
-            for (int t = 0; t < batch; ++t, ++hk) {
-                launch_pair(c, jacobi, G, h2, subcell, pa, pb);
-                if (!jacobi) {
-                    t_begin(c, "precond_wavefront");
-                    precond_wavefront(c, mode, c->r, c->z, c->iflags);
-                    hipLaunchKernelGGL(k_copy_fluid_x, nb, TB, 0, c->stream, nfl, c->fl, c->z, c->zs[hk & 1],
-                                       c->iflags);
-                    hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, nfl, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
-                    t_end(c);
-                }
+    auto ensure_graph = [&](bool jac) {
+        if (!use_graph) return;
+        if (!jac && (!c->ic_graph || c->ic_graph_key != subcell * 2 + mode)) {
+            if (c->ic_graph) {
+                (void)hipGraphExecDestroy(c->ic_graph);
+                c->ic_graph = nullptr;
             }
-            hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G);
+            hipGraph_t g;
+            FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int t = 0; t < WBATCH; ++t) {
+                launch_pair(c, false, G, h2, subcell, pa, pb);
+                precond_wavefront(c, mode, c->r, c->z, c->iflags);
+                hipLaunchKernelGGL(k_copy_fluid_x, G, TB, 0, c->stream, 0, c->fl, c->z, c->zs[t & 1], c->iflags);
+                hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, 0, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
+            }
+            hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G, c->d_status);
+            FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
+            FLIP_CHECK(hipGraphInstantiate(&c->ic_graph, g, nullptr, nullptr, 0));
+            FLIP_CHECK(hipGraphDestroy(g));
+            c->ic_graph_key = subcell * 2 + mode;
         }
-        FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->iflags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        done = c->h_pinned[I_DONE] != 0;
-        iters = c->h_pinned[I_ITERS];
+        if (jac && (!c->pcg_graph || c->pcg_graph_subcell != subcell)) {
+            if (c->pcg_graph) {
+                (void)hipGraphExecDestroy(c->pcg_graph);
+                c->pcg_graph = nullptr;
+            }
+            hipGraph_t g;
+            FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int t = 0; t < BATCH; ++t) launch_pair(c, true, G, h2, subcell, pa, pb);
+            hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G, c->d_status);
+            FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
+            FLIP_CHECK(hipGraphInstantiate(&c->pcg_graph, g, nullptr, nullptr, 0));
+            FLIP_CHECK(hipGraphDestroy(g));
+            c->pcg_graph_subcell = subcell;
+        }
+    };
+    int hk = 0;  // host mirror of the iteration index (parity of the z buffer, direct launches)
+    auto launch_batch = [&](bool jac) {
+        if (use_graph) {
+            FLIP_CHECK(hipGraphLaunch(jac ? c->pcg_graph : c->ic_graph, c->stream));
+            return;
+        }
+        const int batch = c->profiling ? 1 : 4;
+        for (int t = 0; t < batch; ++t, ++hk) {
+            launch_pair(c, jac, G, h2, subcell, pa, pb);
+            if (!jac) {
+                t_begin(c, "precond_wavefront");
+                precond_wavefront(c, mode, c->r, c->z, c->iflags);
+                hipLaunchKernelGGL(k_copy_fluid_x, G, TB, 0, c->stream, 0, c->fl, c->z, c->zs[hk & 1], c->iflags);
+                hipLaunchKernelGGL(k_pcg_zr, G, TB, 0, c->stream, 0, c->fl, c->z, c->r, pb, 2, 1, c->iflags);
+                t_end(c);
+            }
+        }
+        hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G, c->d_status);
+    };
+    // Batches: the first runs alone (a learned solve usually ends in it); after
+    // that one batch stays queued ahead of the one the host waits for.  A
+    // queued batch after convergence returns at once.
+    auto run_batches = [&](bool jac, bool solo_first) {
+        ensure_graph(jac);
+        // pipelining needs every batch to start where the last one ended:
+        // graph replays do (the iteration index lives on the device); direct
+        // launches track the z-buffer parity on the host, so they run one
+        // batch at a time
+        launch_batch(jac);
+        FLIP_CHECK(hipEventRecord(c->ev_batch[0], c->stream));
+        int cur = 0;
+        bool ahead = solo_first;  // a learned solve's check batch waits alone
+        for (;;) {
+            if (!ahead && use_graph) {
+                launch_batch(jac);
+                FLIP_CHECK(hipEventRecord(c->ev_batch[cur ^ 1], c->stream));
+                ahead = true;
+            }
+            FLIP_CHECK(hipEventSynchronize(c->ev_batch[cur]));
+            if (hs->done) break;
+            if (!use_graph) {
+                launch_batch(jac);
+                FLIP_CHECK(hipEventRecord(c->ev_batch[cur ^ 1], c->stream));
+            } else {
+                ahead = false;
+            }
+            cur ^= 1;
+        }
+        if (ahead && !(solo_first && cur == 0)) FLIP_CHECK(hipStreamSynchronize(c->stream));  // the queued batch
+    };
+    // the first wait: for a learned solve, the flexible CG's last iteration
+    // has finished by then; a speculative start learns here whether the
+    // factor has P == 1.0 cells
+    hs->done = 0;
+    run_batches(jacobi, learned_ran);
+    if (jacobi && hs->unit > 0) {
+        // P == 1.0 cells: the reference's value test makes the couplings live
+        jacobi = false;
+        hs->done = 0;
+        setup_wavefront(true);
+        run_batches(false, false);
     }
-    c->last_iters = iters;
-    int32_t capped = 0;
-    double e2 = 0.0;
-    FLIP_CHECK(hipMemcpyAsync(&capped, c->iflags + I_CAPPED, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    FLIP_CHECK(hipMemcpyAsync(&e2, c->scal + S_ERR2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    FLIP_CHECK(hipStreamSynchronize(c->stream));
+    if (learned_ran) learned_iters = learned_fcg_iterations(c);
+    c->last_iters = hs->iters;
     if (st) {
-        st->iterations = iters + learned_iters;
-        st->error2 = e2;
-        st->error2_0 = e0;
-        st->bnorm2 = b2;
+        st->n_fluid_cells = hs->nfl;
+        st->n_precond_unit = hs->unit;
+        st->learned_iterations = learned_iters;
+        st->iterations = hs->iters + learned_iters;
+        st->error2 = hs->err2;
+        st->error2_0 = hs->err2_0;
+        st->bnorm2 = hs->bnorm2;
     }
-    if (!(e2 == e2)) return FLIP_E_NUMERIC;
-    return capped ? 1 : 0;
+    c->nfl = hs->nfl;
+    if (!(hs->err2 == hs->err2)) return FLIP_E_NUMERIC;
+    return hs->capped ? 1 : 0;
 }
 
 // ------------------------------------------------------------------ FP64

@@ -15,10 +15,14 @@ constexpr int TB = 1024;   // 16-wave blocks, one per CU at the largest grid
 constexpr int GMAX = 256;  // grid of the iteration kernels (partials per reduction)
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// device scalars: a_k lives in S_A0 + (k & 1)
-enum { S_A0 = 0, S_A1, S_ALPHA, S_BETA, S_ERR2, S_ERR2_0, S_BNORM2, S_EPS };
-// device int flags; I_KA is the next pcg_a iteration, I_KB the next pcg_b one
-enum { I_DONE = 0, I_ITERS, I_UNIT, I_CAPPED, I_TOLPOL, I_CAP, I_NFL, I_KA, I_KB };
+// device scalars: a_k lives in S_A0 + (k & 1); S_RR0 is the residual the
+// iteration-0 test of pcg_a compares (the initial one, or after a learned solve)
+enum { S_A0 = 0, S_A1, S_ALPHA, S_BETA, S_ERR2, S_ERR2_0, S_BNORM2, S_EPS, S_RR0 };
+// device int flags; I_KA is the next pcg_a iteration, I_KB the next pcg_b one;
+// I_START: pcg_a(0) first tests S_RR0 against the stop threshold; I_SPEC: the
+// per-cell (Jacobi-form) iterations run before the host knows whether the
+// factor has P == 1.0 cells, and stop at once when it does (I_UNIT > 0)
+enum { I_DONE = 0, I_ITERS, I_UNIT, I_CAPPED, I_TOLPOL, I_CAP, I_NFL, I_KA, I_KB, I_START, I_SPEC };
 
 // neighbour kinds in the metadata word, 2 bits each in x_ref order
 // (+i, -i, +j, -j, +k, -k), solver.cpp:19-26 / 161-163
