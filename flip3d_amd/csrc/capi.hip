@@ -79,6 +79,7 @@ void free_particles(FlipCtx *c)
     dfree(c->deep);
     dfree(c->ev_list);
     dfree(c->ev_off);
+    dfree(c->ev_slot);
     dfree(c->ev_rand);
     dfree(c->partials);
     dfree(c->scan_blocks);
@@ -106,8 +107,9 @@ void alloc_particles(FlipCtx *c, int64_t cap)
     dalloc(&c->perm, n);
     size_t big = std::max<size_t>(n, (size_t)c->C) + 1;
     dalloc(&c->deep, big);
-    dalloc(&c->ev_list, 2 * n + 2);
+    dalloc(&c->ev_list, 3 * n + 3);  // the corrector's (creation index, slot, events) triples
     dalloc(&c->ev_off, big);
+    dalloc(&c->ev_slot, n);
     c->ev_cap = 2 * (int64_t)n + 1024;
     dalloc(&c->ev_rand, 3 * (size_t)c->ev_cap);
     dalloc(&c->partials, 4 * (big / 256 + 8));
@@ -215,6 +217,10 @@ void alloc_grids(FlipCtx *c)
     FLIP_CHECK(hipMemset(c->tickets, 0, 8 * sizeof(unsigned)));
     dalloc(&c->ev_count, 4);
     dalloc(&c->dtv, 4);
+    dalloc(&c->rng_dev, 64);
+    dalloc(&c->rng_table, (size_t)RNG_CHUNK * 31);
+    rng_upload_table(c);
+    FLIP_CHECK(hipHostMalloc((void **)&c->h_rng, 64 * sizeof(int32_t)));
     FLIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(int32_t)));
     FLIP_CHECK(hipHostMalloc((void **)&c->h_status, sizeof(FlipCtx::SolveStatus), hipHostMallocMapped));
     FLIP_CHECK(hipHostGetDevicePointer((void **)&c->d_status, c->h_status, 0));
@@ -272,6 +278,10 @@ void free_all(FlipCtx *c)
     dfree(c->tickets);
     dfree(c->ev_count);
     dfree(c->dtv);
+    dfree(c->rng_dev);
+    dfree(c->rng_table);
+    if (c->h_rng) (void)hipHostFree(c->h_rng);
+    c->h_rng = nullptr;
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     c->h_pinned = nullptr;
     if (c->h_status) (void)hipHostFree(c->h_status);
@@ -459,9 +469,9 @@ static void run_stage(FlipCtx *c, int stage, flip_step_stats *st)
     }
     case FLIP_STAGE_CORRECT: {
         ensure_sorted(c);
-        int ev = k_correct(c);
+        k_correct(c);  // its event count (ev_count[3]) is read when the call ends
         c->sorted = false;
-        if (st) st->n_jitter = ev;
+        c->corrected = true;
         break;
     }
     default:
@@ -721,7 +731,9 @@ int flip_gpu_get_max_dens(flip_ctx *ctx, float *v)
 int flip_gpu_srand(flip_ctx *ctx, unsigned seed)
 {
     if (!ctx) return fail(FLIP_E_ARG, "null context");
-    reinterpret_cast<FlipCtx *>(ctx)->rng.seed(seed);
+    FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+    c->rng.seed(seed);
+    c->rng_on_device = false;
     return FLIP_OK;
 }
 
@@ -797,6 +809,13 @@ int flip_gpu_stage(flip_ctx *ctx, int stage, flip_step_stats *st)
         FLIP_CHECK(hipGetLastError());
         if (st && c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
         c->solve_timed = false;
+        if (c->corrected) {
+            c->corrected = false;
+            int32_t ev[4];
+            FLIP_CHECK(hipMemcpy(ev, c->ev_count, sizeof(ev), hipMemcpyDeviceToHost));
+            if (st) st->n_jitter = ev[3];
+            if (ev[1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
+        }
         return FLIP_OK;
     });
 }
@@ -918,7 +937,7 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
         FLIP_CHECK(hipMemsetAsync(c->ev_count + EV_NONFINITE, 0, sizeof(int32_t), c->stream));
         if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
-        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 8, c->ev_count + EV_NONFINITE, sizeof(int32_t), hipMemcpyDeviceToHost,
+        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 8, c->ev_count, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
                                   c->stream));
         FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 16, c->dtv, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));
@@ -929,7 +948,12 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         st->ms_step = elapsed(c->ev_a, c->ev_b);
         if (c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
         c->solve_timed = false;
-        if (c->h_pinned[8])
+        if (c->corrected) {
+            st->n_jitter = c->h_pinned[8 + 3];
+            c->corrected = false;
+            if (c->h_pinned[8 + 1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
+        }
+        if (c->h_pinned[8 + EV_NONFINITE])
             return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
         timing_collect(c);
         return FLIP_OK;

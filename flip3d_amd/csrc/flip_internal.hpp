@@ -41,6 +41,7 @@ void learned_free(LearnedSolver *ls);
 
 // Device buffers of one context.  Particles live in cell-sorted order and are
 // re-permuted after every sort; meta carries creation index << 2 | type.
+constexpr int RNG_CHUNK = 1024;  // rand() draws per chunk of the device's generator (its table's rows)
 constexpr int EV_NONFINITE = 2;  // slot of ev_count set when a step writes a non-finite particle
 
 struct FlipCtx {
@@ -104,7 +105,8 @@ struct FlipCtx {
     };
     SolveStatus *h_status = nullptr, *d_status = nullptr;
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
-    bool solve_timed = false;  // ev_s0/ev_s1 bracket a solve whose time is not read yet
+    bool solve_timed = false;
+    bool corrected = false;    // a corrector ran; its jitter-event count is not read yet  // ev_s0/ev_s1 bracket a solve whose time is not read yet
     hipGraphExec_t ic_graph = nullptr;  // captured batch of wavefront-preconditioned iterations
     int ic_graph_key = -1;              // subcell * 2 + preconditioner mode it was captured for
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
@@ -127,6 +129,12 @@ struct FlipCtx {
     int32_t *ev_count = nullptr;  // [0] event count, [EV_NONFINITE] a non-finite particle was written
     float *ev_rand = nullptr;    // rand()%101 values, 3 per event
     int32_t *ev_off = nullptr;   // per listed particle: offset into ev_rand
+    int32_t *ev_slot = nullptr;  // listed particles' slots in creation order
+    // the glibc rand() state (s[31], f, r) on the device, and whether it (not
+    // rng) is the current one
+    int32_t *rng_dev = nullptr, *h_rng = nullptr;
+    uint32_t *rng_table = nullptr;  // draw coefficients of the device's chunked rand() (k_jitter_draw)
+    bool rng_on_device = false;
     int64_t ev_cap = 0;
     int32_t *h_pinned = nullptr;  // pinned host scalars
 
@@ -213,6 +221,9 @@ int learned_levels(int n);  // coarse levels of the learned preconditioner for s
 void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const float *g2, int64_t m,
                          const float4 *p, float4 *out);
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s);
+void rng_upload_table(FlipCtx *c);  // the chunked-draw coefficients (context creation)
+void rng_to_device(FlipCtx *c);  // the rand() state to the device, if it is on the host
+void rng_to_host(FlipCtx *c);    // and back (waits for the stream)
 void k_launch_choose_dt(FlipCtx *c);            // the step's dt into dtv[0] on the device
 void k_launch_set_dt(FlipCtx *c, double dt);    // a given dt into dtv[0]
 

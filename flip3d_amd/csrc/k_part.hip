@@ -11,6 +11,7 @@
 #include "flip_internal.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <unordered_map>
 
@@ -1110,7 +1111,7 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
                 sy += w * (p.y - q.y) / dist * re;
                 sz += w * (p.z - q.z) / dist * re;
             } else if (!is_wall_w(q.w)) {
-                r = -1;  // a random-jitter event: the host replays this particle
+                r = -1;  // a random-jitter event: the fix-up pass replays this particle
                 break;
             } else {
                 float4 nr = nrm_id[meta_id(meta[a])];
@@ -1123,49 +1124,276 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
     }
     if (r < 0) {
         int32_t at = atomicAdd(count, 1);
-        list[2 * at] = meta_id(meta[i]);
-        list[2 * at + 1] = (int32_t)i;
+        list[3 * at] = meta_id(meta[i]);
+        list[3 * at + 1] = (int32_t)i;
     }
 }
 
-// count the jitter events of listed particles (for the host's rand() draws)
-__global__ void k_spring_count(int m, const int32_t *__restrict__ list, int n, const float4 *__restrict__ pos,
-                               const int32_t *__restrict__ cs, float re, int32_t *events)
+// ------------------------------------------- corrector jitter events
+// The spring pass lists the particles with a jitter event as (creation index,
+// slot, event count) triples, count in mcount[0].  The reference draws 3
+// rand()%101 per event in particle order at one thread (corrector.cpp:73-77);
+// here the listed particles are ranked by creation index (k_jitter_order),
+// the draws made in that order from the device copy of the generator
+// (k_jitter_draw), and the springs finished (k_spring_fix_wave).  No host
+// round trip; every launch returns at once when nothing is listed.
+constexpr int JB = RNG_CHUNK;  // block of the single-block jitter kernels (one draw per lane)
+
+constexpr int OTB = 256;  // block of the ranking kernel (one listed particle per thread)
+__global__ __launch_bounds__(OTB) void k_jitter_order(const int32_t *__restrict__ mcount,
+                                                      const int32_t *__restrict__ trip, int32_t *slots,
+                                                      int32_t *events)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    int64_t i = list[t];
-    float4 p = pos[i];
-    float cx, cy, cz;
-    int ci, cj, ck;
-    spring_centre(n, p, cx, cy, cz, ci, cj, ck);
-    const double rre = 0.1 * (double)re;
-    const float far2 = re * re * 1.0001f;  // as in spring_one
-    const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
-    int ev = 0;
-    int rlo[9], rhi[9];
-    window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
-#pragma unroll
-    for (int r = 0; r < 9; ++r)
-        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
-            if (a == i) return;
-            if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
-            float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
-            if (!((double)dist > rre) && !is_wall_w(q.w)) ev++;
-        });
-    events[t] = ev;
+    __shared__ int32_t ids[OTB];
+    const int m = mcount[0];
+    for (int b = blockIdx.x * OTB; b < m; b += gridDim.x * OTB) {  // uniform over the block
+        const int t = b + (int)threadIdx.x;
+        const int32_t id = t < m ? trip[3 * t] : 0;
+        int rank = 0;
+        for (int t0 = 0; t0 < m; t0 += OTB) {
+            __syncthreads();
+            ids[threadIdx.x] = t0 + (int)threadIdx.x < m ? trip[3 * (t0 + threadIdx.x)] : 0x7fffffff;
+            __syncthreads();
+#pragma unroll 16
+            for (int u = 0; u < OTB; ++u) rank += ids[u] < id;  // creation indices are unique
+        }
+        if (t < m) {
+            slots[rank] = trip[3 * t + 1];
+            events[rank] = trip[3 * t + 2];
+        }
+    }
 }
 
-__global__ void k_spring_fix(int m, const int32_t *__restrict__ list, const int32_t *__restrict__ off, int n,
-                             const float4 *__restrict__ pos, const int32_t *__restrict__ meta,
-                             const float4 *__restrict__ nrm_id, const int32_t *__restrict__ cs,
-                             const double *__restrict__ dtv, float re, const float *__restrict__ rv, float4 *tmp0)
+// offsets (3 draws per event) by a block scan, then the draws.  The glibc
+// TYPE_3 generator (state s[31], indices f and r = f + 28 mod 31; a draw sets
+// s[f] += s[r], advances both, and rand() is the new word >> 1) only adds
+// words mod 2^32, so with the state held in canonical order z_j = s[(f + j)
+// mod 31] the k-th new word is sum_j table[k][j] z_j for a table fixed by the
+// recurrence alone (rng_table, built on the host for k < JB).  Each lane forms
+// one draw of a chunk of JB; the chunk's last 31 words are the next state.
+// mcount[1] is set when the draws would overflow rv (cap values).
+__global__ __launch_bounds__(JB) void k_jitter_draw(int32_t *mcount, const int32_t *__restrict__ events,
+                                                    int32_t *off, float *rv, long cap, int32_t *rng,
+                                                    const uint32_t *__restrict__ table)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    const float dt = (float)dtv[0];
-    spring_one<true>(list[t], n, pos, meta, nrm_id, cs, dt, re, rv + off[t], tmp0);
+    __shared__ int32_t wsum[JB / 64];
+    __shared__ uint32_t z[31], zn[31], y[JB];
+    __shared__ int32_t s_total;
+    const int m = mcount[0];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int carry = 0;
+    for (int b = 0; b < m; b += JB) {
+        const int t = b + (int)threadIdx.x;
+        const int v = t < m ? events[t] : 0;
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int x = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += x;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        int before = 0, tot = 0;
+        for (int u = 0; u < JB / 64; ++u) {
+            if (u < w) before += wsum[u];
+            tot += wsum[u];
+        }
+        if (t < m) off[t] = 3 * (carry + before + incl - v);
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) s_total = carry;
+    if (threadIdx.x < 31) z[threadIdx.x] = (uint32_t)rng[threadIdx.x];  // canonical order (f = 0)
+    __syncthreads();
+    const long draws = 3L * s_total;
+    if (threadIdx.x == 0) mcount[3] = s_total;
+    if (draws > cap) {
+        if (threadIdx.x == 0) mcount[1] = 1;
+        return;
+    }
+    for (long e0 = 0; e0 < draws; e0 += JB) {
+        const int cnt = (int)min((long)JB, draws - e0);
+        const int t = threadIdx.x;
+        if (t < cnt) {
+            const uint32_t *row = table + (size_t)t * 31;
+            uint32_t acc = 0u;
+#pragma unroll
+            for (int j = 0; j < 31; ++j) acc += row[j] * z[j];
+            y[t] = acc;
+            rv[e0 + t] = (float)((int)(acc >> 1) % 101);
+        }
+        __syncthreads();
+        if (t < 31) zn[t] = cnt >= 31 ? y[cnt - 31 + t] : (t < 31 - cnt ? z[t + cnt] : y[t - (31 - cnt)]);
+        __syncthreads();
+        if (t < 31) z[t] = zn[t];
+        __syncthreads();
+    }
+    if (threadIdx.x < 31) rng[threadIdx.x] = (int32_t)z[threadIdx.x];
+    if (threadIdx.x == 0) {
+        rng[31] = 0;
+        rng[32] = 28;
+    }
 }
+
+// The window of listed particle t as one wave sees it: its rows (culled as
+// spring_one culls them) and the length of their concatenation.
+struct SpringWindow {
+    float4 p;
+    int rlo[9], rhi[9], len;
+    __device__ __forceinline__ SpringWindow(int64_t i, int n, const float4 *__restrict__ pos,
+                                            const int32_t *__restrict__ cs, float far2)
+    {
+        p = pos[i];
+        float cx, cy, cz;
+        int ci, cj, ck;
+        spring_centre(n, p, cx, cy, cz, ci, cj, ck);
+        const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
+        window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
+        len = 0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) len += rhi[r] - rlo[r];
+    }
+    // the particle at window position e (< len)
+    __device__ __forceinline__ int at(int e) const
+    {
+        int a = -1;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const int L = rhi[r] - rlo[r];
+            if (a < 0 && e < L) a = rlo[r] + e;
+            if (a < 0) e -= L;
+        }
+        return a;
+    }
+};
+
+// the jitter events of each listed particle (FLUID neighbours within 0.1 re,
+// corrector.cpp:73-77) into the third slot of its triple, one wave each
+constexpr int FIX_TB = 256;
+__global__ __launch_bounds__(FIX_TB) void k_spring_events(const int32_t *__restrict__ mcount, int32_t *trip, int n,
+                                                          const float4 *__restrict__ pos,
+                                                          const int32_t *__restrict__ cs, float re)
+{
+    const int m = mcount[0];
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * (FIX_TB / 64);
+    const double rre = 0.1 * (double)re;
+    const float far2 = re * re * 1.0001f;
+    for (int t = blockIdx.x * (FIX_TB / 64) + (threadIdx.x >> 6); t < m; t += nw) {
+        const int64_t i = trip[3 * t + 1];
+        const SpringWindow win(i, n, pos, cs, far2);
+        int ev = 0;
+        for (int e0 = 0; e0 < win.len; e0 += 64) {
+            bool jit = false;
+            if (e0 + lane < win.len) {
+                const int a = win.at(e0 + lane);
+                if (a != i) {
+                    const float4 q = pos[a];
+                    if (!(length2(win.p.x, win.p.y, win.p.z, q.x, q.y, q.z) > far2)) {
+                        const float dist = length3(win.p.x, win.p.y, win.p.z, q.x, q.y, q.z);
+                        jit = !((double)dist > rre) && !is_wall_w(q.w);
+                    }
+                }
+            }
+            ev += __popcll(__ballot(jit));
+        }
+        if (lane == 0) trip[3 * t + 2] = ev;
+    }
+}
+
+// Listed particles' springs with the jitter draws, one wave per particle:
+// lanes form the terms of 64 window particles at a time (window order; the
+// event index of a jitter term is the count of events before it) and lane 0
+// adds them in that order, float adds for the distance terms and double
+// rounding for the jitter and wall terms as spring_one does.
+__global__ __launch_bounds__(FIX_TB) void k_spring_fix_wave(const int32_t *__restrict__ mcount,
+                                                            const int32_t *__restrict__ list,
+                                                            const int32_t *__restrict__ off, int n,
+                                                            const float4 *__restrict__ pos,
+                                                            const int32_t *__restrict__ meta,
+                                                            const float4 *__restrict__ nrm_id,
+                                                            const int32_t *__restrict__ cs,
+                                                            const double *__restrict__ dtv, float re,
+                                                            const float *__restrict__ rv, float4 *tmp0)
+{
+    __shared__ double tv[FIX_TB / 64][64][3];
+    __shared__ int8_t tk[FIX_TB / 64][64];
+    const int m = mcount[1] ? 0 : mcount[0];  // nothing when the draws overflowed
+    const float dt = (float)dtv[0];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nw = gridDim.x * (FIX_TB / 64);
+    const double rre = 0.1 * (double)re;
+    const double cj_ = 0.01 * (double)re / (double)dt;
+    const double cw = 0.05 * (double)re / (double)dt;
+    const float far2 = re * re * 1.0001f;
+    for (int t = blockIdx.x * (FIX_TB / 64) + wv; t < m; t += nw) {
+        const int64_t i = list[t];
+        const float *rvt = rv + off[t];
+        const SpringWindow win(i, n, pos, cs, far2);
+        const float4 p = win.p;
+        const int len = win.len;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        int events = 0;
+        for (int e0 = 0; e0 < len; e0 += 64) {
+            const int a = e0 + lane < len ? win.at(e0 + lane) : -1;
+            int kind = 0;  // 0 none, 1 distance term (float add), 2 jitter, 3 wall (double-rounded adds)
+            double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (a >= 0 && a != i) {
+                q = pos[a];
+                if (!(length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2)) {
+                    const float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+                    if ((double)dist > rre) {
+                        const float w = (float)(50.0 * (double)mass_of(q.w) * (double)smooth_kernel(dist * dist, re));
+                        kind = 1;
+                        v0 = (double)(w * (p.x - q.x) / dist * re);
+                        v1 = (double)(w * (p.y - q.y) / dist * re);
+                        v2 = (double)(w * (p.z - q.z) / dist * re);
+                    } else {
+                        kind = is_wall_w(q.w) ? 3 : 2;
+                    }
+                }
+            }
+            const unsigned long long jm = __ballot(kind == 2);
+            if (kind == 2) {
+                const int ev = events + __popcll(jm & ((1ull << lane) - 1ull));
+                v0 = cj_ * (double)rvt[3 * ev + 0] / 100.0;
+                v1 = cj_ * (double)rvt[3 * ev + 1] / 100.0;
+                v2 = cj_ * (double)rvt[3 * ev + 2] / 100.0;
+            } else if (kind == 3) {
+                const float4 nr = nrm_id[meta_id(meta[a])];
+                v0 = cw * (double)nr.x;
+                v1 = cw * (double)nr.y;
+                v2 = cw * (double)nr.z;
+            }
+            events += __popcll(jm);
+            tk[wv][lane] = (int8_t)kind;
+            tv[wv][lane][0] = v0;
+            tv[wv][lane][1] = v1;
+            tv[wv][lane][2] = v2;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                const int lim = min(64, len - e0);
+                for (int u = 0; u < lim; ++u) {
+                    const int kd = tk[wv][u];
+                    if (kd == 1) {
+                        sx += (float)tv[wv][u][0];
+                        sy += (float)tv[wv][u][1];
+                        sz += (float)tv[wv][u][2];
+                    } else if (kd >= 2) {
+                        sx = (float)((double)sx + tv[wv][u][0]);
+                        sy = (float)((double)sy + tv[wv][u][1]);
+                        sz = (float)((double)sz + tv[wv][u][2]);
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) tmp0[i] = make_float4(p.x + dt * sx, p.y + dt * sy, p.z + dt * sz, 0.0f);
+    }
+}
+
 
 // corrector.cpp:15-47 resample at the pseudo-moved position with the old
 // positions and velocities (re = DENSITY/N); keeps u when no weight.
@@ -1477,6 +1705,56 @@ void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hi
     if (m > 0) hipLaunchKernelGGL(k_selftest, nblk(m, TB), TB, 0, s, m, x, y, out);
 }
 
+// The rand() stream lives on the host (scene placement, reposition) or on the
+// device (the corrector's jitter draws); these move it where it is used next.
+// draw k of a chunk from canonical state e_j (unit vectors): the coefficient
+// of z_j in the k-th new word (k_jitter_draw), uploaded at context creation
+// on the context's stream
+void rng_upload_table(FlipCtx *c)
+{
+    static const std::vector<uint32_t> tab = [] {
+        std::vector<uint32_t> t((size_t)JB * 31);
+        for (int j = 0; j < 31; ++j) {
+            uint32_t st[31] = {};
+            st[j] = 1u;
+            int f = 0, r = 28;
+            for (int k = 0; k < JB; ++k) {
+                st[f] += st[r];
+                t[(size_t)k * 31 + j] = st[f];
+                f = f == 30 ? 0 : f + 1;
+                r = r == 30 ? 0 : r + 1;
+            }
+        }
+        return t;
+    }();
+    FLIP_CHECK(hipMemcpyAsync(c->rng_table, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              c->stream));
+    FLIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// On the device the state is kept in canonical order (f = 0, r = 28).
+void rng_to_device(FlipCtx *c)
+{
+    if (c->rng_on_device) return;
+    if ((c->rng.f + 28) % 31 != c->rng.r) throw flip_arg_error{FLIP_E_STATE, "rand() state out of phase"};
+    for (int j = 0; j < 31; ++j) c->h_rng[j] = c->rng.s[(c->rng.f + j) % 31];
+    c->h_rng[31] = 0;
+    c->h_rng[32] = 28;
+    FLIP_CHECK(hipMemcpyAsync(c->rng_dev, c->h_rng, 33 * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    c->rng_on_device = true;
+}
+
+void rng_to_host(FlipCtx *c)
+{
+    if (!c->rng_on_device) return;
+    FLIP_CHECK(hipMemcpyAsync(c->h_rng, c->rng_dev, 33 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    FLIP_CHECK(hipStreamSynchronize(c->stream));
+    std::memcpy(c->rng.s, c->h_rng, sizeof(c->rng.s));
+    c->rng.f = c->h_rng[31];
+    c->rng.r = c->h_rng[32];
+    c->rng_on_device = false;
+}
+
 // the step's time step into c->dtv[0] on the device (no host round trip)
 void k_launch_choose_dt(FlipCtx *c)
 {
@@ -1531,6 +1809,7 @@ int k_stuck_and_reposition(FlipCtx *c)
         FLIP_CHECK(hipMemcpyAsync(&nd, c->scan_tmp + c->C, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         FLIP_CHECK(hipStreamSynchronize(c->stream));
         if (nd > 0) {
+            rng_to_host(c);
             int take = std::min(nd, m);
             std::vector<int32_t> deep(take), waters(m);
             FLIP_CHECK(hipMemcpyAsync(deep.data(), c->ev_off, sizeof(int32_t) * take, hipMemcpyDeviceToHost, c->stream));
@@ -1616,42 +1895,21 @@ int k_correct(FlipCtx *c)
     if (!c->sorted) throw flip_arg_error{FLIP_E_STATE, "the corrector needs sorted particles"};
     int n = c->n;
     float re = (float)(c->prm.density / n);
-    FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, sizeof(int32_t), c->stream));
+    // ev_count[0]: listed particles, [1]: draw overflow, [3]: events (read
+    // with the step's end); [2] (the advection's check) is kept
+    FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, 2 * sizeof(int32_t), c->stream));
     launch_timed(c, "spring", k_spring, nblk(c->np, SPRING_TB), SPRING_TB, c->np, n, c->P4(), c->M(), c->nrm_id,
                  c->cell_start, c->dtv, re, c->tmp0, c->ev_list, c->ev_count);
-    FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    FLIP_CHECK(hipStreamSynchronize(c->stream));
-    int m = c->h_pinned[0];
-    int events = 0;
-    if (m > 0) {
-        // order the listed particles by creation index, count their events, and
-        // draw 3 rand()%101 per event in that order (corrector.cpp:74-76 at one thread)
-        std::vector<int32_t> pairs(2 * (size_t)m), slots(m), ev(m);
-        FLIP_CHECK(hipMemcpyAsync(pairs.data(), c->ev_list, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        std::vector<std::pair<int32_t, int32_t>> byid(m);
-        for (int t = 0; t < m; ++t) byid[t] = {pairs[2 * t], pairs[2 * t + 1]};
-        std::sort(byid.begin(), byid.end());
-        for (int t = 0; t < m; ++t) slots[t] = byid[t].second;
-        FLIP_CHECK(hipMemcpyAsync(c->ev_list, slots.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_spring_count, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, n, c->P4(), c->cell_start,
-                           re, c->ev_off);
-        FLIP_CHECK(hipMemcpyAsync(ev.data(), c->ev_off, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        std::vector<int32_t> off(m);
-        for (int t = 0; t < m; ++t) {
-            off[t] = 3 * events;
-            events += ev[t];
-        }
-        std::vector<float> rv(3 * (size_t)events + 1);
-        for (int e = 0; e < 3 * events; ++e) rv[e] = (float)(c->rng.next() % 101);
-        if ((int64_t)rv.size() > c->ev_cap * 3) throw flip_arg_error{FLIP_E_STATE, "too many corrector jitter events"};
-        FLIP_CHECK(hipMemcpyAsync(c->ev_rand, rv.data(), sizeof(float) * rv.size(), hipMemcpyHostToDevice, c->stream));
-        FLIP_CHECK(hipMemcpyAsync(c->ev_off, off.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_spring_fix, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_off, n, c->P4(), c->M(),
-                           c->nrm_id, c->cell_start, c->dtv, re, c->ev_rand, c->tmp0);
-        FLIP_CHECK(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
-    }
+    // the jitter events in creation order with the reference's rand() draws,
+    // on the device (corrector.cpp:74-76 at one thread)
+    rng_to_device(c);
+    launch_timed(c, "jitter_count", k_spring_events, 256, FIX_TB, c->ev_count, c->ev_list, n, c->P4(),
+                 c->cell_start, re);
+    launch_timed(c, "jitter_order", k_jitter_order, 64, OTB, c->ev_count, c->ev_list, c->ev_slot, c->ev_off);
+    launch_timed(c, "jitter_draw", k_jitter_draw, 1, JB, c->ev_count, c->ev_off, c->ev_off, c->ev_rand,
+                 (long)(3 * c->ev_cap), c->rng_dev, c->rng_table);
+    launch_timed(c, "jitter_fix", k_spring_fix_wave, 256, FIX_TB, c->ev_count, c->ev_slot, c->ev_off, n, c->P4(),
+                 c->M(), c->nrm_id, c->cell_start, c->dtv, re, c->ev_rand, c->tmp0);
     // With re^2 <= 1e-5 (N >= 159) every resample weight m * (re^2 /
     // fmax(r2, 1e-5) - 1) clamps to exactly 0 (r2 < re^2 puts the divisor at
     // 1e-5), so the weight sum is 0 and the reference keeps every velocity
@@ -1664,5 +1922,5 @@ int k_correct(FlipCtx *c)
                      c->cell_start, c->tmp0, c->tmp1, hh, reinterpret_cast<const P3 *>(c->xyz3));
     launch_timed(c, "correct_update", k_correct_update, nblk(c->np, TB), TB, c->np, c->P4(), c->V4(), c->tmp0,
                  keep ? nullptr : c->tmp1, c->ev_count + EV_NONFINITE);
-    return events;
+    return 0;
 }

@@ -377,6 +377,7 @@ extern "C" int flip_gpu_build_scene(flip_ctx *ctx, int scene, uint64_t seed, int
     if (!ctx) return -1;
     FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
     try {
+        rng_to_host(c);  // the reference scenes draw rand() on the host
         int N = c->n;
         double dens = c->prm.density;
         std::vector<Obj> o;
