@@ -26,6 +26,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from flip3d_amd import passes as P  # noqa: E402  (pure Python: loads no library)
+
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-5
 METRIC = "FLIP steps/s at 128^3 (~4M particles), learned vs plain PCG, rel resid 1e-5"
@@ -33,16 +35,21 @@ PROFILES = os.path.join(ROOT, "profiles")
 DEFAULT_WEIGHTS = os.path.join(ROOT, "flip3d_amd", "weights", "unet_coarse.bin")
 
 
-def measured_traffic(kernel, n=128):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    profiles/<kernel>_n<n>_traffic.txt (FETCH_SIZE + WRITE_SIZE, each scaled
-    by the same counter on a copy of known size); None when there is none."""
-    path = os.path.join(PROFILES, f"{kernel}_n{n}_traffic.txt")
+def log(msg):
+    """progress on stderr (the JSON line alone goes to stdout)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(pass_name, n=128):
+    """HBM bytes per invocation of a pass from the committed counter summary
+    profiles/traffic_n<n>.txt (FETCH_SIZE + WRITE_SIZE of its kernels, each
+    scaled by the same counter on a copy of known size); None when absent."""
+    path = os.path.join(PROFILES, f"traffic_n{n}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+        if len(parts) == 3 and parts[0] == "traffic_per_invocation" and parts[1] == pass_name:
             return int(float(parts[2]))
     return None
 
@@ -65,31 +72,6 @@ def copy_probe(mib):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-
-
-def kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid):
-    """Algorithmic bytes per launch (unique reads + writes, neighbour re-reads
-    cached) of the kernels bench reports on; per-unit figures in DESIGN.md."""
-    per = {
-        # PCG iteration kernels, per FLUID cell
-        "pcg_a": 25 * n_fluid,    # fl 4, A 1, L 4, z 4, s_old 4 -> s_new 4, q 4
-        "pcg_b": 41 * n_fluid,    # fl 4, A 1, x 4, r 4, s 4, q 4, P 8 -> x 4, r 4, z 4
-        # flexible-CG kernels of the learned solver, per FLUID cell
-        "fcg_a": 28 * n_fluid,    # fl 4, meta 4, L 4 (AIR-adjacent), (z, s) 8 -> (z, s) 8, q 4
-        "fcg_b": 28 * n_fluid,    # fl 4, x 4, s 4, r 4, q 4 -> x 4, r 4
-        # particle gathers, per particle (float4 pos/vel = 16 B); density reads
-        # its windows from the sort's 12-byte (x, y, z) records
-        "density": 16 * n_particles + 12 * n_particles + 4 * n_particles + 4 * n_cells,
-        # density with add_ExtForce fused in: the velocity is read and written whole
-        "density_force": 16 * n_particles + 12 * n_particles + 32 * n_particles + 4 * n_cells,
-        "p2g": 32 * n_particles + 4 * n_cells + 2 * 4 * n_faces + n_cells,
-        "spring": 16 * n_particles + 4 * n_particles + 16 * n_particles + 4 * n_cells,
-        # own pos, pseudo-position, window records (12 B), window velocities -> tmp1
-        "resample": 16 * n_particles * 2 + 12 * n_particles + 16 * n_particles + 16 * n_particles + 4 * n_cells,
-        "g2p_flip": 32 * n_particles + 16 * n_particles + 24 * n_faces,
-        "advect": 32 * n_particles + 32 * n_particles + 12 * n_faces,
-    }
-    return per.get(name)
 
 
 def mesh_leg(fa, n, weights, cpu=True):
@@ -140,23 +122,73 @@ def mesh_leg(fa, n, weights, cpu=True):
     return out
 
 
-def cpu_baseline(n, seconds_budget):
-    """The oracle (C restatement of the reference path, OpenMP where the
-    reference uses it) on this host: the first step of the same scene."""
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def oracle_steps(n, k):
+    """Seconds and PCG iterations of the oracle's first k steps of the
+    benchmark scene (the reference's timed region: density through the
+    corrector, no I/O), from the same initial state as the GPU runs."""
     from oracle import pyoracle as po
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     s = po.Sim(n, tol_policy=1, tol=TOL, dt_policy=1)
     s.scene_benchmark()
     s.srand(1)
-    t0 = time.perf_counter()
-    st = s.step()
-    dt = time.perf_counter() - t0
+    secs, its = [], []
+    for _ in range(k):
+        t0 = time.perf_counter()
+        st = s.step()
+        secs.append(time.perf_counter() - t0)
+        its.append(int(st["iterations"]))
+        log(f"oracle N={n} step {len(secs)}/{k}: {secs[-1]:.2f} s, {its[-1]} PCG iterations")
     s.close()
-    return {
-        "value": 1.0 / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-        "sample": f"oracle step 1 of the N={n} benchmark scene from its initial state "
-                  f"({st['iterations']} PCG iterations, {dt:.1f} s)",
+    return secs, its
+
+
+def cpu_baseline(n, steps, one_thread_steps, one_thread_timeout):
+    """The oracle (C restatement of the reference path, OpenMP where the
+    reference uses it) on this host's cores: the first `steps` steps on all
+    the threads OpenMP is given (mean and spread per step), then the first
+    `one_thread_steps` on one thread in a child process.  Runs before the GPU
+    is touched, so the child starts from a process without a HIP runtime."""
+    import statistics
+    import subprocess
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    secs, its = oracle_steps(n, steps)
+    mean = statistics.fmean(secs)
+    out = {
+        "value": round(1.0 / mean, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+        "sample": f"oracle, first {steps} steps of the N={n} benchmark scene from the GPU runs' initial "
+                  f"state, {threads} OpenMP threads; mean and spread per step",
+        "steps": steps, "s_per_step_mean": round(mean, 3),
+        "s_per_step_min": round(min(secs), 3), "s_per_step_max": round(max(secs), 3),
+        "s_per_step_stdev": round(statistics.stdev(secs), 3) if len(secs) > 1 else 0.0,
+        "s_per_step": [round(x, 3) for x in secs], "pcg_iterations": its,
+        "nproc": os.cpu_count(), "cpu_model": cpu_model(),
     }
+    if one_thread_steps > 0:
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        cmd = [sys.executable, os.path.abspath(__file__), "--oracle-worker", str(n), str(one_thread_steps)]
+        try:
+            r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=one_thread_timeout)
+            w = json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+        except (subprocess.TimeoutExpired, ValueError, IndexError):
+            w = None
+        if w:
+            m1 = sum(w["secs"]) / len(w["secs"])
+            out["one_thread"] = {"steps": one_thread_steps, "s_per_step_mean": round(m1, 3),
+                                 "s_per_step": [round(x, 3) for x in w["secs"]], "pcg_iterations": w["its"],
+                                 "steps_per_s": round(1.0 / m1, 5)}
+        else:
+            out["one_thread"] = {"steps": one_thread_steps, "error": f"no result within {one_thread_timeout} s"}
+    return out
 
 
 def run_mode(fa, n, precond, weights, steps, warmup, profile_step=True):
@@ -178,9 +210,10 @@ def run_mode(fa, n, precond, weights, steps, warmup, profile_step=True):
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ktimes = []
+    pst = None
     if profile_step:
         ctx.set_profiling(True)
-        ctx.step()
+        pst = ctx.step()
         ktimes = ctx.kernel_times()
         ctx.set_profiling(False)
     ctx.close()
@@ -189,35 +222,55 @@ def run_mode(fa, n, precond, weights, steps, warmup, profile_step=True):
         tot, cnt = agg.get(name, (0.0, 0))
         agg[name] = (tot + ms, cnt + 1)
     mean = lambda k: sum(st[k] for st in stats) / len(stats)  # noqa: E731
+    log(f"{precond}: {steps / elapsed:.2f} steps/s over {steps} steps")
     return {
         "precond": precond, "steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
         "particles": n_particles, "n_fluid": stats[-1]["n_fluid_cells"],
         "iterations_mean": mean("iterations"), "learned_iterations_mean": mean("learned_iterations"),
         "solve_ms_mean": mean("ms_solve"), "repositioned_mean": mean("n_reposition"),
         "rel_residual_max": max((st["error2"] / max(st["bnorm2"], 1e-300)) ** 0.5 for st in stats),
-        "agg": agg,
+        "agg": agg, "profiled_fluid": pst["n_fluid_cells"] if pst else None,
     }
 
 
-def roofline_of(agg, n, n_particles, n_fluid):
-    """HBM roofline of the kernel with the largest share of the profiled
-    step among those with an algorithmic-bytes model."""
-    n_cells, n_faces = n ** 3, (n + 1) * n * n
-    ranked = sorted(((tot, name, cnt) for name, (tot, cnt) in agg.items()), reverse=True)
-    dom = next((r for r in ranked if kernel_bytes(r[1], n_particles, n_cells, n_faces, n_fluid)), None)
-    if not dom:
+def roofline_of(passes, n):
+    """HBM roofline of the modelled pass with the largest share of the
+    profiled step: algorithmic bytes per invocation over the pass's average
+    invocation time (HIP events around its launches on the library's
+    stream)."""
+    name, p = P.dominant(passes)
+    if name is None:
         return None
-    tot, name, cnt = dom
-    avg_ms = tot / cnt
-    b = kernel_bytes(name, n_particles, n_cells, n_faces, n_fluid)
-    achieved = b / (avg_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, n),
-            "bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt}
+    achieved = p["bytes_per_invocation"] / (p["avg_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, n),
+            "pass": name, "kernels": P.by_name(name)["regions"], "bytes_per_invocation": p["bytes_per_invocation"],
+            "avg_invocation_ms": p["avg_ms"], "invocations_per_step": p["invocations"]}
+
+
+def step_roofline(passes):
+    """All modelled passes of the step together: their algorithmic bytes
+    over their summed time."""
+    b = sum(p["bytes_per_invocation"] * p["invocations"] for p in passes.values())
+    ms = sum(p["ms"] for p in passes.values())
+    if ms <= 0:
+        return None
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"bytes": b, "ms": round(ms, 4), "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def summary(r):
-    return {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in r.items() if k != "agg"}
+    return {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in r.items()
+            if k not in ("agg", "profiled_fluid")}
+
+
+def zero_network(fa, path):
+    """Weights of a network whose output is zero: the learned solver's
+    V-cycle with a zero coarsest-level correction."""
+    import numpy as np
+    with fa.UNet(32, capacity=1, seed=1) as net:
+        net.set_params(np.zeros(net.num_params, np.float32))
+        net.save(path)
 
 
 def main():
@@ -230,7 +283,14 @@ def main():
                     help="solver of the reported value; the plain (reference) PCG is always timed beside it")
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS,
                     help="coarse-level network weights for --precond learned")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="do not time the plain PCG beside the reported solver (counter runs)")
+    ap.add_argument("--leg-steps", type=int, default=10,
+                    help="timed steps of the IC(0) and zero-network legs (0: skip them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps timed on all threads")
+    ap.add_argument("--cpu-one-thread-steps", type=int, default=1, help="oracle steps timed on one thread")
+    ap.add_argument("--cpu-one-thread-timeout", type=float, default=170.0)
     ap.add_argument("--no-profile-step", action="store_true",
                     help="skip the per-launch timed step (for runs under an external profiler)")
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesher / preview leg")
@@ -239,30 +299,52 @@ def main():
                          "(for kernel tracers that cannot follow graph launches; slower)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (counter calibration)")
+    ap.add_argument("--oracle-worker", nargs=2, type=int, metavar=("N", "STEPS"), help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.oracle_worker:
+        secs, its = oracle_steps(*args.oracle_worker)
+        print(json.dumps({"secs": secs, "its": its}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the slab decomposition is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    n = args.n
+    # the CPU baseline first: its one-thread leg starts a child process,
+    # which must not be forked from a process that holds the GPU
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(n, args.cpu_steps, args.cpu_one_thread_steps, args.cpu_one_thread_timeout)
+
     if args.direct_launch:
         os.environ["FLIP_DIRECT_LAUNCH"] = "1"
     import flip3d_amd as fa
 
-    n = args.n
     prof = not args.no_profile_step
-    plain = run_mode(fa, n, "reference", None, args.steps, args.warmup, prof)
+    plain = None
+    if not args.skip_plain or args.precond == "reference":
+        plain = run_mode(fa, n, "reference", None, args.steps, args.warmup, prof)
     main_run = plain if args.precond == "reference" else run_mode(fa, n, args.precond, args.weights, args.steps,
                                                                  args.warmup, prof)
     n_particles = main_run["particles"]
-    roofline = roofline_of(main_run["agg"], n, n_particles, main_run["n_fluid"])
+    passes = P.step_passes(main_run["agg"], P.units(n, n_particles, main_run["profiled_fluid"] or main_run["n_fluid"]))
+    roofline = roofline_of(passes, n)
+
+    legs = {}
+    if args.leg_steps > 0:
+        if args.precond != "ic0":
+            legs["ic0"] = summary(run_mode(fa, n, "ic0", None, args.leg_steps, 2, False))
+        if args.precond == "learned":
+            import tempfile
+            with tempfile.TemporaryDirectory() as td:
+                zpath = os.path.join(td, "zero.bin")
+                zero_network(fa, zpath)
+                legs["learned_zero_network"] = summary(run_mode(fa, n, "learned", zpath, args.leg_steps, 2, False))
 
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(n, 30)
     mesh = None if args.no_mesh else mesh_leg(fa, n, args.weights, cpu=not args.no_cpu_baseline)
 
     value = main_run["steps_per_s"]
@@ -277,11 +359,14 @@ def main():
                    "n": n, "particles": n_particles, "precond": args.precond, "tol": TOL,
                    "solver_launch": "direct" if args.direct_launch else "graph"},
         "particles_per_s": round(value * n_particles, 1),
-        "speedup_vs_plain_step": round(value / plain["steps_per_s"], 3),
-        "speedup_vs_plain_solve": round(plain["solve_ms_mean"] / main_run["solve_ms_mean"], 3),
+        "speedup_vs_plain_step": round(value / plain["steps_per_s"], 3) if plain else None,
+        "speedup_vs_plain_solve": round(plain["solve_ms_mean"] / main_run["solve_ms_mean"], 3) if plain else None,
         "solver": summary(main_run),
-        "plain": summary(plain),
+        "plain": summary(plain) if plain else None,
+        "legs": legs,
         "roofline": roofline,
+        "step_roofline": step_roofline(passes),
+        "passes": passes,
         "cpu_baseline": cpu,
         "mesh": mesh,
         "profile_step_ms": {k: round(v[0], 4) for k, v in sorted(main_run["agg"].items(), key=lambda kv: -kv[1][0])},

@@ -156,11 +156,14 @@ def default_params(n, **kw):
 
 
 def selftest_math(x, y):
+    """Device rounding self-check: (x/y, sqrtf(x), the glibc hypotf form,
+    (float)(x/y in double)) per pair, and the device's sharp kernel of
+    r2 = x with h*h = y (for x < y)."""
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(y, np.float32)
-    out = np.empty((len(x), 4), np.float32)
+    out = np.empty(5 * len(x), np.float32)
     _check(lib().flip_gpu_selftest_math(len(x), _ptr(x), _ptr(y), _ptr(out)))
-    return out
+    return out[:4 * len(x)].reshape(len(x), 4), out[4 * len(x):]
 
 
 class FlipContext:

@@ -1160,12 +1160,12 @@ int flip_gpu_selftest_math(int64_t m, const float *x, const float *y, float *out
         float *dx = nullptr, *dy = nullptr, *dout = nullptr;
         dalloc(&dx, m);
         dalloc(&dy, m);
-        dalloc(&dout, 4 * m);
+        dalloc(&dout, 5 * m);
         FLIP_CHECK(hipMemcpy(dx, x, m * sizeof(float), hipMemcpyHostToDevice));
         FLIP_CHECK(hipMemcpy(dy, y, m * sizeof(float), hipMemcpyHostToDevice));
         k_launch_selftest(m, dx, dy, dout, nullptr);
         FLIP_CHECK(hipDeviceSynchronize());
-        FLIP_CHECK(hipMemcpy(out, dout, 4 * m * sizeof(float), hipMemcpyDeviceToHost));
+        FLIP_CHECK(hipMemcpy(out, dout, 5 * m * sizeof(float), hipMemcpyDeviceToHost));
         dfree(dx);
         dfree(dy);
         dfree(dout);

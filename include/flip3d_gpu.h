@@ -289,7 +289,9 @@ int flip_gpu_step_many(flip_ctx **ctxs, int count, int threads, flip_step_stats 
 
 /* Device math self-check for the ops whose rounding the parity argument
  * relies on: out[4*i+0] = x/y, +1 = sqrtf(x), +2 = (float)sqrt((double)x*x+(double)y*y),
- * +3 = (float)((double)x / (double)y). */
+ * +3 = (float)((double)x / (double)y); and out[4*m+i] = kernel.cpp:8-10's
+ * sharp_kernel(x, sqrt(y)) as the device forms it for x < y (with y = h*h;
+ * 0 otherwise).  out holds 5*m floats. */
 int flip_gpu_selftest_math(int64_t m, const float *x, const float *y, float *out);
 
 #ifdef __cplusplus

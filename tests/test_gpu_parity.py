@@ -35,7 +35,7 @@ def test_device_rounding_matches_ieee():
     m = 200000
     x = (rs.uniform(1e-6, 4.0, m) * 10.0 ** rs.randint(-6, 6, m)).astype(np.float32)
     y = (rs.uniform(1e-6, 4.0, m) * 10.0 ** rs.randint(-6, 6, m)).astype(np.float32)
-    out = fa.selftest_math(x, y)
+    out, _ = fa.selftest_math(x, y)
     assert np.array_equal(out[:, 0], (x / y).astype(np.float32))
     assert np.array_equal(out[:, 1], np.sqrt(x))
     hyp = np.sqrt(x.astype(np.float64) ** 2 + y.astype(np.float64) ** 2).astype(np.float32)
@@ -57,6 +57,33 @@ def test_scene_matches_oracle(scene):
         assert np.array_equal(a["p"], b["p"]) and np.array_equal(a["n"], b["n"])
         assert np.array_equal(a["type"], b["type"])
         assert np.float32(g.max_dens) == np.float32(s.max_dens)
+
+
+@pytest.mark.parametrize("hh", [np.float32(1.4) * np.float32(1.4), np.float32(0.5 / 128) ** 2,
+                                np.float32(0.5 / 32) ** 2, np.float32(1.0)])
+def test_device_sharp_kernel_matches_reference(hh):
+    """kernel.cpp:8-10 sharp_kernel as the device forms it (the FP64 division
+    as reciprocal, two Newton steps and one remainder correction, without the
+    scaling steps) equals the reference's double formula bit for bit: 4M
+    random r2 in [1e-5, hh), every float in the 20000 ulp below hh and above
+    1e-5, among them r2 whose value lies within 1e-3 ulp of a float rounding
+    midpoint."""
+    hh = np.float32(hh)
+    rs = np.random.RandomState(int(hh * 1e6) % 1000)
+    lo = np.float32(1e-5)
+    r2 = [rs.uniform(float(lo), float(hh), 4000000).astype(np.float32)]
+    top = hh.view(np.uint32)
+    r2.append((top - np.arange(1, 20001, dtype=np.uint32)).view(np.float32))
+    r2.append((lo.view(np.uint32) + np.arange(0, 20000, dtype=np.uint32)).view(np.float32))
+    r2 = np.concatenate(r2)
+    r2 = r2[(r2 >= lo) & (r2 < hh)]
+    ref = np.maximum(np.float64(hh) / np.maximum(r2.astype(np.float64), 1e-5) - 1.0, 0.0).astype(np.float32)
+    # operands whose double value sits within 1e-3 ulp of a float midpoint
+    mid = np.abs((np.maximum(np.float64(hh) / r2.astype(np.float64) - 1.0, 0.0) - ref.astype(np.float64))
+                 / np.spacing(ref).astype(np.float64))
+    assert (np.abs(mid - 0.5) < 1e-3).sum() > 0
+    _, got = fa.selftest_math(r2, np.full_like(r2, hh))
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
 def _ctx_with(gn, p, t, u=None, nrm=None, dens=None):
