@@ -1571,6 +1571,7 @@ __global__ void k_selftest(int64_t m, const float *x, const float *y, float *out
     out[4 * i + 1] = sqrtf(a);
     out[4 * i + 2] = hypotf_glibc(a, b);
     out[4 * i + 3] = (float)((double)a / (double)b);
+    out[4 * m + i] = a < b ? sharp_kernel_in(a, b) : 0.0f;  // a = r2, b = h*h
 }
 
 // flip3D.cpp:375-402 pourWater: the disc's particles appended in disc order
