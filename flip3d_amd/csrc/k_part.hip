@@ -36,22 +36,16 @@ __device__ __forceinline__ float mass_of(float w) { return __builtin_fabsf(w); }
 // (scaling by a power of two rounds nothing while the result stays finite).
 // FORCE: add_ExtForce fused in (flip3D.cpp:870-875, u_y += gdt for every
 // particle); it reads and writes only velocities, which density does not use.
-// PACKED (POW2 and every mass 1): the window is read from the sort's 12-byte
-// records xyz3, where a WALL particle sits at +inf.  Its weight is then
-// exactly +0, which leaves the sum unchanged (it starts at +0 and never
-// becomes -0), as skipping it does; a unit mass times the kernel is the
-// kernel.
 struct P3 {
     float x, y, z;
 };
 
-template <bool POW2, bool FORCE, bool PACKED>
+template <bool POW2, bool FORCE>
 __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, const int32_t *__restrict__ cs, float h,
                                                 float inv_hh, float max_dens, const double *__restrict__ dtv,
-                                                double gravity, const P3 *__restrict__ xyz3)
+                                                double gravity)
 {
-    static_assert(!PACKED || POW2, "the packed pass uses the power-of-two kernel");
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const double gdt = FORCE ? -dtv[0] * gravity : 0.0;  // add_ExtForce's -DT * GRAVITY
@@ -85,18 +79,12 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
     float wsum = 0.0f;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-        if (PACKED) {
-            for_range8(xyz3, rlo[r], rhi[r], [&](const P3 &q, int) {
-                wsum += smooth_kernel_pow2(length2(q.x, q.y, q.z, p.x, p.y, p.z), inv_hh);
-            });
-        } else {
-            for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int) {
-                if (is_wall_w(q.w)) return;
-                const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
-                float w = q.w * (POW2 ? smooth_kernel_pow2(r2, inv_hh) : smooth_kernel(r2, h));
-                wsum += w;
-            });
-        }
+        for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int) {
+            if (is_wall_w(q.w)) return;
+            const float r2 = length2(q.x, q.y, q.z, p.x, p.y, p.z);
+            float w = q.w * (POW2 ? smooth_kernel_pow2(r2, inv_hh) : smooth_kernel(r2, h));
+            wsum += w;
+        });
     }
     if (FORCE) {
         float4 v = vel[i];
@@ -106,6 +94,131 @@ __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 
     } else {
         vel[i].w = wsum / max_dens;
     }
+}
+
+// k_density<true, FORCE> for unit masses while the storage is sorted: the
+// window is read from the sort's 12-byte records xyz3, where a WALL particle
+// sits at +inf.  Its weight is then exactly +0, which leaves the sum
+// unchanged (it starts at +0 and never becomes -0), as skipping it does; a
+// unit mass times the kernel is the kernel.  The storage is sorted by cell, so the block's 256 particles lie in a few (i, j)
+// cell rows ("groups", k ascending inside each).  For every group the block
+// copies the union of its particles' windows, nine rows (i+-1, j+-1) over the
+// cells [kmin-1, kmax+1], from the sort's records into LDS with coalesced
+// loads; every particle then reads its own window rows out of LDS in the
+// reference's order, so the sums are those of k_density bit for bit.  The
+// lanes of one cell read the same LDS words together (a broadcast), where
+// each had issued its own global gather.  A block with more than DL_G groups
+// or more than DL_CAP staged records takes the global loop.
+constexpr int DL_CAP = 2560;  // staged records per block (30 KiB)
+constexpr int DL_G = 3;
+
+template <bool FORCE>
+__global__ __launch_bounds__(TB) void k_density_lds(int64_t np, int n, const float4 *__restrict__ pos,
+                                                    float4 *__restrict__ vel, const int32_t *__restrict__ cs,
+                                                    float inv_hh, float max_dens, const double *__restrict__ dtv,
+                                                    double gravity, const P3 *__restrict__ xyz3)
+{
+    __shared__ P3 st[DL_CAP];
+    __shared__ int skey[TB];
+    __shared__ int wcnt[TB / 64];
+    __shared__ int gkey[DL_G], gk0[DL_G], gk1[DL_G];
+    __shared__ int rgb[DL_G * 9], rlb[DL_G * 9], rlen[DL_G * 9];
+    __shared__ int s_ngrp, s_total;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int64_t i = (int64_t)blockIdx.x * TB + t;
+    const bool valid = i < np;
+    const double gdt = FORCE ? -dtv[0] * gravity : 0.0;
+    float4 p = valid ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int ci = cell_of(n, p.x), cj = cell_of(n, p.y), ck = cell_of(n, p.z);
+    const int key = ci * n + cj;
+    skey[t] = valid ? key : -1;
+    __syncthreads();
+    // groups: runs of equal (i, j) keys (nondecreasing along the storage)
+    const bool first = valid && (t == 0 || skey[t - 1] != key);
+    const bool last = valid && (t == TB - 1 || skey[t + 1] != key);
+    const uint64_t bal = __ballot(first);
+    if (lane == 0) wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    int g = __popcll(bal & ((2ull << lane) - 1)) - 1;  // starts up to and including this lane
+    for (int w = 0; w < wv; ++w) g += wcnt[w];
+    if (t == 0) {
+        int s = 0;
+        for (int w = 0; w < TB / 64; ++w) s += wcnt[w];
+        s_ngrp = s;
+    }
+    if (first && g < DL_G) {
+        gkey[g] = key;
+        gk0[g] = ck;
+    }
+    if (last && g < DL_G) gk1[g] = ck;
+    __syncthreads();
+    const int ngrp = s_ngrp;
+    bool staged = ngrp <= DL_G;
+    if (staged) {
+        if (t < ngrp * 9) {
+            const int gg = t / 9, r = t % 9;
+            const int gi = gkey[gg] / n, gj = gkey[gg] % n;
+            const int si = gi - 1 + r / 3, sj = gj - 1 + r % 3;
+            int b = 0, e = 0;
+            if (si >= 0 && si <= n - 1 && sj >= 0 && sj <= n - 1) {
+                const int klo = max(gk0[gg] - 1, 0), khi = min(gk1[gg] + 1, n - 1);
+                const long c0 = cidx(n, si, sj, klo);
+                b = cs[c0];
+                e = cs[c0 + (khi - klo) + 1];
+            }
+            rgb[t] = b;
+            rlen[t] = e - b;
+        }
+        __syncthreads();
+        if (t == 0) {
+            int s = 0;
+            for (int q = 0; q < ngrp * 9; ++q) {
+                rlb[q] = s;
+                s += rlen[q];
+            }
+            s_total = s;
+        }
+        __syncthreads();
+        staged = s_total <= DL_CAP;
+        if (staged) {
+            for (int q = 0; q < ngrp * 9; ++q) {
+                const int b = rgb[q], l = rlen[q], o = rlb[q];
+                for (int e = t; e < l; e += TB) st[o + e] = xyz3[b + e];
+            }
+            __syncthreads();
+        }
+    }
+    if (!valid) return;
+    if (is_wall_w(p.w)) {
+        float4 v = vel[i];
+        if (FORCE) v.y = (float)((double)v.y + gdt);
+        v.w = 1.0f;
+        vel[i] = v;
+        return;
+    }
+    const int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
+    float wsum = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int si = ci - 1 + r / 3, sj = cj - 1 + r % 3;
+        if (si < 0 || si > n - 1 || sj < 0 || sj > n - 1) continue;
+        const long c0 = cidx(n, si, sj, klo);
+        const int b = cs[c0], e = cs[c0 + (khi - klo) + 1];
+        if (staged) {
+            const int q = g * 9 + r;
+            for_range8(st + (rlb[q] + (b - rgb[q])), 0, e - b, [&](const P3 &s, int) {
+                wsum += smooth_kernel_pow2(length2(s.x, s.y, s.z, p.x, p.y, p.z), inv_hh);
+            });
+        } else {
+            for_range8(xyz3, b, e, [&](const P3 &s, int) {
+                wsum += smooth_kernel_pow2(length2(s.x, s.y, s.z, p.x, p.y, p.z), inv_hh);
+            });
+        }
+    }
+    float4 v = vel[i];
+    if (FORCE) v.y = (float)((double)v.y + gdt);
+    v.w = wsum / max_dens;
+    vel[i] = v;
 }
 
 // flip3D.cpp:870-875 add_ExtForce: u_y = (float)((double)u_y + gdt), gdt = -dt*g
@@ -1612,12 +1725,17 @@ void k_launch_density(FlipCtx *c, bool with_force)
     const float inv_hh = pow2 ? std::ldexp(1.0f, 1 - e) : 0.0f;  // hh = 2^(e-1)
     // the sort's packed records are current while the storage is sorted
     const bool packed = pow2 && c->unit_mass && c->sorted;
-    auto kern = packed ? (with_force ? k_density<true, true, true> : k_density<true, false, true>)
-                       : pow2 ? (with_force ? k_density<true, true, false> : k_density<true, false, false>)
-                              : (with_force ? k_density<false, true, false> : k_density<false, false, false>);
-    launch_timed(c, with_force ? "density_force" : "density", kern, nblk(c->np, TB), TB, c->np, c->n, c->P4(),
-                 c->V4(), c->cell_start, h, inv_hh, c->max_dens, c->dtv, c->prm.gravity,
-                 reinterpret_cast<const P3 *>(c->xyz3));
+    const char *name = with_force ? "density_force" : "density";
+    if (packed) {
+        launch_timed(c, name, with_force ? k_density_lds<true> : k_density_lds<false>, nblk(c->np, TB), TB, c->np,
+                     c->n, c->P4(), c->V4(), c->cell_start, inv_hh, c->max_dens, c->dtv, c->prm.gravity,
+                     reinterpret_cast<const P3 *>(c->xyz3));
+        return;
+    }
+    auto kern = pow2 ? (with_force ? k_density<true, true> : k_density<true, false>)
+                     : (with_force ? k_density<false, true> : k_density<false, false>);
+    launch_timed(c, name, kern, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->cell_start, h, inv_hh,
+                 c->max_dens, c->dtv, c->prm.gravity);
 }
 
 void k_launch_ext_force(FlipCtx *c)

@@ -39,7 +39,7 @@ PASSES = [
        r"^k_fill_i32$|^k_sort_", "sort_count", r"^k_sort_gather$"),
     _p("density", "density",
        lambda u: 16 * u["Np"],
-       ("density_force", "density", "ext_force"), r"^k_density|^k_ext_force$", None, r"^k_density<true, true"),
+       ("density_force", "density", "ext_force"), r"^k_density|^k_ext_force$", None, r"^k_density_lds<true>$|^k_density<true, true>$"),
     _p("flags", "flags + L",
        lambda u: 4 * u["Np"] + 10 * u["C"],
        ("flags",), r"^k_flags$", "flags", r"^k_flags$"),
