@@ -1134,7 +1134,6 @@ int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path)
     return guarded([&]() -> int {
         require(ctx && path, "null argument");
         FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
-        require(c->n % 16 == 0, "the learned preconditioner needs n to be a multiple of 16");
         FILE *f = fopen(path, "rb");
         require(f != nullptr, "cannot open U-Net weights file");
         fclose(f);

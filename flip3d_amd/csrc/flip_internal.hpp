@@ -217,6 +217,7 @@ int k_solve(FlipCtx *c, flip_step_stats *st);   // PCG on (A, L, press, div)
 // PCG init); returns its iterations, *converged when ||r||^2 <= eps
 int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *converged);
 int learned_fcg_iterations(FlipCtx *c);
+int learned_side(int n);    // the side the learned preconditioner's levels halve (n padded to 16)
 int learned_levels(int n);  // coarse levels of the learned preconditioner for side n
 void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const float *g2, int64_t m,
                          const float4 *p, float4 *out);

@@ -73,10 +73,10 @@ __device__ __forceinline__ IJK ijk_of(long c, int n)
 }
 
 // parent (side n/2) of the cell at p plus the offset of neighbour u (u = 6: the cell itself)
-__device__ __forceinline__ long parent_nb(const IJK &p, int u, int n)
+// the coarse cell (side h) over cell p's neighbour u (u = 6: p itself)
+__device__ __forceinline__ long parent_nb(const IJK &p, int u, int h)
 {
     const int i = p.i + (u == 0) - (u == 1), j = p.j + (u == 2) - (u == 3), k = p.k + (u == 4) - (u == 5);
-    const int h = n >> 1;
     return ((long)(i >> 1) * h + (j >> 1)) * h + (k >> 1);
 }
 
@@ -97,8 +97,11 @@ __global__ void k_ml_coarsen(int Rc, int Rf, const int8_t *__restrict__ Af, cons
     int fluid = 0, wall = 0, nw = 0;
     float lf = 0.0f, ls = 0.0f;
     for (int d = 0; d < 8; ++d) {
-        long f = ((long)(2 * I + (d >> 2)) * Rf + 2 * J + ((d >> 1) & 1)) * Rf + 2 * K + (d & 1);
-        int a = Af[f];
+        const int fi = 2 * I + (d >> 2), fj = 2 * J + ((d >> 1) & 1), fk = 2 * K + (d & 1);
+        // children past the fine grid's end (a padded coarse level) are WALL
+        const bool in = fi < Rf && fj < Rf && fk < Rf;
+        const long f = in ? ((long)fi * Rf + fj) * Rf + fk : 0;
+        const int a = in ? Af[f] : WALL;
         fluid += a == FLUID;
         wall += a == WALL;
         if (a == FLUID) lf += Lf[f];
@@ -295,18 +298,27 @@ __global__ __launch_bounds__(TBL) void k_ml_sum8(const int32_t *__restrict__ fi,
         const IJK p = ijk_of(g, Rc);
         const long f0 = ((long)(2 * p.i) * Rf + 2 * p.j) * Rf + 2 * p.k;
         const long di = (long)Rf * Rf, dj = Rf;
-        const float2 a = *reinterpret_cast<const float2 *>(res + f0);
-        const float2 b = *reinterpret_cast<const float2 *>(res + f0 + dj);
-        const float2 c = *reinterpret_cast<const float2 *>(res + f0 + di);
-        const float2 d = *reinterpret_cast<const float2 *>(res + f0 + di + dj);
-        sum += a.x;
-        sum += a.y;
-        sum += b.x;
-        sum += b.y;
-        sum += c.x;
-        sum += c.y;
-        sum += d.x;
-        sum += d.y;
+        if ((Rf & 1) == 0 && 2 * p.i + 1 < Rf && 2 * p.j + 1 < Rf && 2 * p.k + 1 < Rf) {
+            const float2 a = *reinterpret_cast<const float2 *>(res + f0);
+            const float2 b = *reinterpret_cast<const float2 *>(res + f0 + dj);
+            const float2 c = *reinterpret_cast<const float2 *>(res + f0 + di);
+            const float2 d = *reinterpret_cast<const float2 *>(res + f0 + di + dj);
+            sum += a.x;
+            sum += a.y;
+            sum += b.x;
+            sum += b.y;
+            sum += c.x;
+            sum += c.y;
+            sum += d.x;
+            sum += d.y;
+        } else {
+            // an odd fine side or a padded coarse level: the children inside
+            // the fine grid, in the same order
+            for (int d = 0; d < 8; ++d) {
+                const int ci = 2 * p.i + (d >> 2), cj = 2 * p.j + ((d >> 1) & 1), ck = 2 * p.k + (d & 1);
+                if (ci < Rf && cj < Rf && ck < Rf) sum += res[((long)ci * Rf + cj) * Rf + ck];
+            }
+        }
         bc[g] = sum;
     }
     if (!Ac) return;
@@ -357,10 +369,10 @@ __global__ __launch_bounds__(TBL) void k_ml_post_a(const int32_t *__restrict__ f
         const int kd = (m >> (2 * u)) & 3;
         const bool f = kd == NB_FLUID;
         const float xn = x[c + (f ? nb_offset(u, R) : 0)];
-        const float xp = xc[f ? parent_nb(p, u, R) : parent_nb(p, 6, R)];
+        const float xp = xc[f ? parent_nb(p, u, R >> 1) : parent_nb(p, 6, R >> 1)];
         xv[u] = f ? xn + ws * xp : 0.0f;
     }
-    float x3 = x[c] + ws * xc[parent_nb(p, 6, R)];
+    float x3 = x[c] + ws * xc[parent_nb(p, 6, R >> 1)];
     float ax = ax_row(m, xv, x3, c, R, L, subcell, h2);
     t[c] = x3 + MG_W * dinv[c] * (b[c] - ax);
 }
@@ -396,7 +408,7 @@ __global__ __launch_bounds__(TBL) void k_ml_post0_a(const int32_t *__restrict__ 
                                                     const float *__restrict__ L, int subcell, double h2,
                                                     const float *__restrict__ r, const float *__restrict__ dinv,
                                                     const float *__restrict__ z2, const float *__restrict__ x1,
-                                                    float *z4)
+                                                    int h1, float *z4)
 {
     if (fi[FI_DONE]) return;
     const int nfl = iflags[I_NFL];
@@ -411,10 +423,10 @@ __global__ __launch_bounds__(TBL) void k_ml_post0_a(const int32_t *__restrict__ 
             const int kd = (m >> (2 * u)) & 3;
             const bool f = kd == NB_FLUID;
             const float zn = z2[c + (f ? nb_offset(u, n) : 0)];
-            const float xp = x1[f ? parent_nb(p, u, n) : parent_nb(p, 6, n)];
+            const float xp = x1[f ? parent_nb(p, u, h1) : parent_nb(p, 6, h1)];
             zv[u] = f ? zn + ws * xp : 0.0f;
         }
-        float z3 = z2[c] + ws * x1[parent_nb(p, 6, n)];
+        float z3 = z2[c] + ws * x1[parent_nb(p, 6, h1)];
         float az = ax_row(m, zv, z3, c, n, L, subcell, h2);
         z4[c] = z3 + MG_W * dinv[c] * (r[c] - az);
     }
@@ -597,6 +609,8 @@ void learned_free(LearnedSolver *ls)
 
 // number of coarse levels for grid side n: halve while the next level is at
 // least MG_COARSE cells wide and a multiple of 8 (at least one level)
+int learned_side(int n) { return (n + 15) / 16 * 16; }
+
 int learned_levels(int n)
 {
     int K = 1;
@@ -608,17 +622,20 @@ static LearnedSolver *learned_get(FlipCtx *c)
 {
     if (c->learned) return c->learned;
     const int n = c->n;
-    if (n % 16 != 0) throw flip_arg_error{FLIP_E_ARG, "the learned preconditioner needs n to be a multiple of 16"};
     if (c->learned_weights.empty())
         throw flip_arg_error{FLIP_E_STATE, "FLIP_PRECOND_LEARNED needs weights (flip_gpu_set_learned_weights)"};
     LearnedSolver *ls = new LearnedSolver();
     try {
         ls->n = n;
-        ls->K = learned_levels(n);
+        // the coarse levels halve a side padded to a multiple of 16 (the
+        // network's level a multiple of 8); level 1 covers the fine grid,
+        // its cells past the grid's end WALL
+        const int np = learned_side(n);
+        ls->K = learned_levels(np);
         ls->lv.resize(ls->K + 1);
         for (int l = 1; l <= ls->K; ++l) {
             Level &L = ls->lv[l];
-            L.R = n >> l;
+            L.R = np >> l;
             L.C = (long)L.R * L.R * L.R;
             L.h2 = (double)(float)(1.0 / ((double)L.R * L.R));
             dmalloc(&L.A, L.C);
@@ -701,7 +718,7 @@ static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell,
                            L.dinv, L.t, L.x);
     }
     launch_timed(c, "ml_post0_a", k_ml_post0_a, G0, TBL, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
-                       c->r, ls->dinv0, ls->z2, ls->lv[1].x, ls->z4);
+                       c->r, ls->dinv0, ls->z2, ls->lv[1].x, ls->lv[1].R, ls->z4);
     launch_timed(c, "ml_post0_b", k_ml_post0_b, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
                        ls->dinv0, ls->z4, q, c->zs[0], c->zs[1], pz, check ? ls->d_flag : nullptr);
 }

@@ -5,8 +5,8 @@
 64 scenes with N drawn uniformly from {24..96} (seeded), each holding 1-3
 random liquid boxes (FLIP_SCENE_RANDOM_BOXES, seed = scene index) at 8
 particles per cell, CFL dt, pressure solve to relative residual 1e-5.
-Scenes whose N is a multiple of 16 use the learned preconditioner (the
-network's levels need it), the others the reference-preconditioned PCG.
+Every scene uses the learned preconditioner (--no-learned: the
+reference-preconditioned PCG).
 Every step advances all scenes through flip_gpu_step_many: host threads
 drive the contexts concurrently, each on its own HIP stream.
 
@@ -50,7 +50,7 @@ def main():
     sizes = np.array(ensemble_sizes(args.scenes, args.seed))
     ctxs, particles, learned = [], 0, 0
     for i, n in enumerate(sizes):
-        use_learned = (not args.no_learned) and n % 16 == 0
+        use_learned = not args.no_learned
         pc = fa.PRECOND_LEARNED if use_learned else fa.PRECOND_REFERENCE
         c = fa.FlipContext(int(n), tol_policy=1, tol=1e-5, dt_policy=1, precond=pc)
         if use_learned:

@@ -275,8 +275,9 @@ int flip_gpu_load_state(flip_ctx *ctx, const char *path);
 
 /* Weights file (flip_unet_save layout) of the coarse-level network used by
  * FLIP_PRECOND_LEARNED; the network solves the coarsest level of the
- * multigrid hierarchy (32^3 for n = 64, 128 and 256; 16^3 for n = 32), so n
- * must be a multiple of 16.  No counterpart in the reference. */
+ * multigrid hierarchy (32^3 for n = 64, 128 and 256; 16^3 for n = 32).  The
+ * levels halve n rounded up to a multiple of 16, the coarse cells past the
+ * grid's end being WALL, so any n works.  No counterpart in the reference. */
 int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path);
 
 /* Ensembles of independent scenes.  Steps every context of ctxs once (each
