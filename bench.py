@@ -191,13 +191,15 @@ def cpu_baseline(n, steps, one_thread_steps, one_thread_timeout):
     return out
 
 
-def run_mode(fa, n, precond, weights, steps, warmup, profile_step=True):
+def run_mode(fa, n, precond, weights, steps, warmup, profile_step=True, launch=(0, 0)):
     """Timed steps of one solver mode from the initial scene, then (unless
-    profile_step is off) one untimed step with every kernel launch timed."""
+    profile_step is off) one untimed step with every kernel launch timed.
+    launch: the solver kernels' (block count, PCG batch), 0 = built in."""
     code = {"reference": fa.PRECOND_REFERENCE, "ic0": fa.PRECOND_IC0, "learned": fa.PRECOND_LEARNED}[precond]
     ctx = fa.FlipContext(n, tol_policy=1, tol=TOL, dt_policy=1, precond=code)
     if precond == "learned":
         ctx.set_learned_weights(weights)
+    ctx.set_launch(*launch)
     n_particles = ctx.build_scene(fa.SCENE_BENCHMARK)
     ctx.srand(1)
     for _ in range(warmup):
@@ -230,6 +232,7 @@ def run_mode(fa, n, precond, weights, steps, warmup, profile_step=True):
         "solve_ms_mean": mean("ms_solve"), "repositioned_mean": mean("n_reposition"),
         "rel_residual_max": max((st["error2"] / max(st["bnorm2"], 1e-300)) ** 0.5 for st in stats),
         "agg": agg, "profiled_fluid": pst["n_fluid_cells"] if pst else None,
+        "launch": {"pcg_blocks": launch[0], "pcg_batch": launch[1]},
     }
 
 
@@ -287,6 +290,8 @@ def main():
                     help="do not time the plain PCG beside the reported solver (counter runs)")
     ap.add_argument("--leg-steps", type=int, default=10,
                     help="timed steps of the IC(0) and zero-network legs (0: skip them)")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="built-in solver launch shapes instead of flip3d_amd/tuning.json's")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps timed on all threads")
     ap.add_argument("--cpu-one-thread-steps", type=int, default=1, help="oracle steps timed on one thread")
@@ -322,11 +327,15 @@ def main():
     import flip3d_amd as fa
 
     prof = not args.no_profile_step
+    from flip3d_amd import autotune
+    table = None if args.no_tuning else autotune.load()
     plain = None
     if not args.skip_plain or args.precond == "reference":
-        plain = run_mode(fa, n, "reference", None, args.steps, args.warmup, prof)
+        plain = run_mode(fa, n, "reference", None, args.steps, args.warmup, prof,
+                         autotune.launch_for(table, "reference", n))
     main_run = plain if args.precond == "reference" else run_mode(fa, n, args.precond, args.weights, args.steps,
-                                                                 args.warmup, prof)
+                                                                 args.warmup, prof,
+                                                                 autotune.launch_for(table, args.precond, n))
     n_particles = main_run["particles"]
     passes = P.step_passes(main_run["agg"], P.units(n, n_particles, main_run["profiled_fluid"] or main_run["n_fluid"]))
     roofline = roofline_of(passes, n)

@@ -1129,6 +1129,20 @@ int flip_gpu_kernel_times(flip_ctx *ctx, char *names, int names_cap, double *ms,
     return cnt;
 }
 
+int flip_gpu_set_launch(flip_ctx *ctx, int pcg_blocks, int pcg_batch)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        // 256: the length of the solver's per-block partials arrays
+        require(pcg_blocks >= 0 && pcg_blocks <= 256, "pcg_blocks must lie in [0, 256]");
+        require(pcg_batch >= 0 && pcg_batch <= 64, "pcg_batch must lie in [0, 64]");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        c->tune_blocks = pcg_blocks;
+        c->tune_batch = pcg_batch;
+        return FLIP_OK;
+    });
+}
+
 int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path)
 {
     return guarded([&]() -> int {

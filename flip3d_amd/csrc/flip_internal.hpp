@@ -54,6 +54,10 @@ struct FlipCtx {
     // by kernel instead of replayed from captured HIP graphs (for tracing
     // tools that cannot follow graph launches); results are unchanged
     bool direct_launch = false;
+    // launch shape of the solver's FLUID-list kernels (flip_gpu_set_launch;
+    // 0 = built in): their block count (at most GMAX) and the PCG iterations
+    // per captured batch
+    int tune_blocks = 0, tune_batch = 0;
 
     // particles (double-buffered for the permutation gather)
     int64_t np = 0, cap = 0;
@@ -96,7 +100,7 @@ struct FlipCtx {
     double *pcg_part = nullptr;  // PCG per-block partials (8 * 1024)
     uint32_t *flmeta = nullptr;  // per FLUID cell: neighbour kinds (2 bits x 6) + has-AIR bit
     hipGraphExec_t pcg_graph = nullptr;  // captured batch of PCG iterations
-    int pcg_graph_subcell = -1;
+    long pcg_graph_key = -1;             // subcell, block count and batch it was captured for
     // the solver's status after each batch of iterations, written by the
     // device into mapped host memory (no copy, read after the batch's event)
     struct SolveStatus {
@@ -105,10 +109,10 @@ struct FlipCtx {
     };
     SolveStatus *h_status = nullptr, *d_status = nullptr;
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
-    bool solve_timed = false;
-    bool corrected = false;    // a corrector ran; its jitter-event count is not read yet  // ev_s0/ev_s1 bracket a solve whose time is not read yet
+    bool solve_timed = false;  // ev_s0/ev_s1 bracket a solve whose time is not read yet
+    bool corrected = false;    // a corrector ran; its jitter-event count is not read yet
     hipGraphExec_t ic_graph = nullptr;  // captured batch of wavefront-preconditioned iterations
-    int ic_graph_key = -1;              // subcell * 2 + preconditioner mode it was captured for
+    long ic_graph_key = -1;             // subcell, preconditioner mode and block count it was captured for
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
     // FP64 solve / iterative refinement vectors (allocated on first use)
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr, *s64 = nullptr, *q64 = nullptr;

@@ -585,7 +585,7 @@ struct LearnedSolver {
     int32_t *d_flag = nullptr;  // its device address
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipGraphExec_t graph = nullptr;
-    int graph_subcell = -1;
+    long graph_key = -1;  // subcell and block count the graph was captured for
 };
 
 void learned_free(LearnedSolver *ls)
@@ -749,7 +749,8 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
     vcycle(c, ls, G, h2, subcell, nullptr, pz, pb, false);  // z_0 into B_1.x
 
     const bool direct = c->profiling || c->direct_launch;
-    if (!ls->graph || ls->graph_subcell != subcell || direct) {
+    const long key = (long)G * 2 + subcell;
+    if (!ls->graph || ls->graph_key != key || direct) {
         if (ls->graph) {
             (void)hipGraphExecDestroy(ls->graph);
             ls->graph = nullptr;
@@ -770,7 +771,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
         FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
         FLIP_CHECK(hipGraphInstantiate(&ls->graph, g, nullptr, nullptr, 0));
         FLIP_CHECK(hipGraphDestroy(g));
-        ls->graph_subcell = subcell;
+        ls->graph_key = key;
     }
     // launch iteration k, then wait for iteration k-1 and read the stop flag
     // (a set flag stays set, so reading a later iteration's write is fine)

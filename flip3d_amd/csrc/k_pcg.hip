@@ -757,9 +757,12 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
     if (!jacobi) setup_wavefront(false);
 
     const bool use_graph = !c->profiling && !c->direct_launch;
+    const int batch = c->tune_batch > 0 ? c->tune_batch : BATCH;  // iterations per captured batch
     auto ensure_graph = [&](bool jac) {
         if (!use_graph) return;
-        if (!jac && (!c->ic_graph || c->ic_graph_key != subcell * 2 + mode)) {
+        const long ic_key = ((long)G * 2 + subcell) * 2 + mode;
+        const long pcg_key = ((long)batch * (GMAX + 1) + G) * 2 + subcell;
+        if (!jac && (!c->ic_graph || c->ic_graph_key != ic_key)) {
             if (c->ic_graph) {
                 (void)hipGraphExecDestroy(c->ic_graph);
                 c->ic_graph = nullptr;
@@ -776,21 +779,21 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
             FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
             FLIP_CHECK(hipGraphInstantiate(&c->ic_graph, g, nullptr, nullptr, 0));
             FLIP_CHECK(hipGraphDestroy(g));
-            c->ic_graph_key = subcell * 2 + mode;
+            c->ic_graph_key = ic_key;
         }
-        if (jac && (!c->pcg_graph || c->pcg_graph_subcell != subcell)) {
+        if (jac && (!c->pcg_graph || c->pcg_graph_key != pcg_key)) {
             if (c->pcg_graph) {
                 (void)hipGraphExecDestroy(c->pcg_graph);
                 c->pcg_graph = nullptr;
             }
             hipGraph_t g;
             FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int t = 0; t < BATCH; ++t) launch_pair(c, true, G, h2, subcell, pa, pb);
+            for (int t = 0; t < batch; ++t) launch_pair(c, true, G, h2, subcell, pa, pb);
             hipLaunchKernelGGL(k_pcg_check, 1, TB, 0, c->stream, pb, c->scal, c->iflags, G, c->d_status);
             FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
             FLIP_CHECK(hipGraphInstantiate(&c->pcg_graph, g, nullptr, nullptr, 0));
             FLIP_CHECK(hipGraphDestroy(g));
-            c->pcg_graph_subcell = subcell;
+            c->pcg_graph_key = pcg_key;
         }
     };
     int hk = 0;  // host mirror of the iteration index (parity of the z buffer, direct launches)
@@ -799,8 +802,8 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
             FLIP_CHECK(hipGraphLaunch(jac ? c->pcg_graph : c->ic_graph, c->stream));
             return;
         }
-        const int batch = c->profiling ? 1 : 4;
-        for (int t = 0; t < batch; ++t, ++hk) {
+        const int direct = c->profiling ? 1 : 4;
+        for (int t = 0; t < direct; ++t, ++hk) {
             launch_pair(c, jac, G, h2, subcell, pa, pb);
             if (!jac) {
                 t_begin(c, "precond_wavefront");

@@ -191,6 +191,10 @@ __device__ __forceinline__ float precond_cell(uint32_t meta, float r, double P)
 }
 
 // grid of the iteration kernels for a context (fixed so graphs stay valid)
-inline int pcg_grid(const FlipCtx *c) { return (int)std::min<unsigned>(nblk(c->C, TB), GMAX); }
+inline int pcg_grid(const FlipCtx *c)
+{
+    const int g = (int)std::min<unsigned>(nblk(c->C, TB), GMAX);
+    return c->tune_blocks > 0 ? std::min(g, c->tune_blocks) : g;
+}
 
 }  // namespace pcgdev

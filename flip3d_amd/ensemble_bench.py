@@ -6,7 +6,8 @@
 random liquid boxes (FLIP_SCENE_RANDOM_BOXES, seed = scene index) at 8
 particles per cell, CFL dt, pressure solve to relative residual 1e-5.
 Every scene uses the learned preconditioner (--no-learned: the
-reference-preconditioned PCG).
+reference-preconditioned PCG), with the solver launch shape the autotuner
+chose for its size (flip3d_amd/tuning.json; --no-tuning: the built-in one).
 Every step advances all scenes through flip_gpu_step_many: host threads
 drive the contexts concurrently, each on its own HIP stream.
 
@@ -27,6 +28,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 import flip3d_amd as fa  # noqa: E402
+from flip3d_amd import autotune  # noqa: E402
 
 WEIGHTS = os.path.join(ROOT, "flip3d_amd", "weights", "unet_coarse.bin")
 SEED = 20261203
@@ -45,14 +47,19 @@ def main():
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--seed", type=int, default=SEED)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="built-in solver launch shapes instead of flip3d_amd/tuning.json's")
     args = ap.parse_args()
+    table = None if args.no_tuning else autotune.load()
 
     sizes = np.array(ensemble_sizes(args.scenes, args.seed))
     ctxs, particles, learned = [], 0, 0
     for i, n in enumerate(sizes):
         use_learned = not args.no_learned
         pc = fa.PRECOND_LEARNED if use_learned else fa.PRECOND_REFERENCE
+        launch = autotune.launch_for(table, "learned" if use_learned else "reference", n)
         c = fa.FlipContext(int(n), tol_policy=1, tol=1e-5, dt_policy=1, precond=pc)
+        c.set_launch(*launch)
         if use_learned:
             c.set_learned_weights(WEIGHTS)
             learned += 1
@@ -79,6 +86,7 @@ def main():
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
         "particles": int(particles), "n_min": int(sizes.min()), "n_max": int(sizes.max()),
         "learned_scenes": learned, "pcg_iterations_mean": round(float(np.mean(iters)), 1),
+        "tuned": table is not None,
     }
     print(json.dumps(out))
 

@@ -280,6 +280,16 @@ int flip_gpu_load_state(flip_ctx *ctx, const char *path);
  * grid's end being WALL, so any n works.  No counterpart in the reference. */
 int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path);
 
+/* Launch shape of the pressure solver's FLUID-list kernels: pcg_blocks caps
+ * their block count (1..256; 0 = the built-in min(ceil(n^3 / 1024), 256)),
+ * pcg_batch sets the PCG iterations per captured graph replay (1..64; 0 =
+ * 8).  The block count fixes how the dot products' per-block partials are
+ * grouped, so results can differ in the last bits from the default; the
+ * batch size changes nothing but the launch pattern.  The values the
+ * autotuner (flip3d_amd/autotune.py) chose per grid size are in
+ * flip3d_amd/tuning.json.  No counterpart in the reference. */
+int flip_gpu_set_launch(flip_ctx *ctx, int pcg_blocks, int pcg_batch);
+
 /* Ensembles of independent scenes.  Steps every context of ctxs once (each
  * exactly as flip_gpu_step would), with `threads` host threads driving the
  * contexts concurrently, each context on its own stream; returns when all
