@@ -3,11 +3,12 @@
     python -m flip3d_amd.autotune [--sizes 24,32,...] [--precond learned,reference]
                                   [--steps 4] [--out flip3d_amd/tuning.json]
 
-For every grid size N and solver, a seeded scene (1-3 random liquid boxes,
-the ensemble's kind of scene) is stepped from the same initial state once
-per candidate launch shape: the kernels' block count (the built-in
-min(ceil(N^3 / 1024), 256) and smaller caps) times the PCG iterations per
-captured batch.  Each candidate runs 2 warm-up steps and `steps` timed ones
+For every grid size N and solver, a scene is stepped from the same initial
+state once per candidate launch shape: the benchmark dam break at N = 128
+and above (the metric's workload), elsewhere seeded random liquid boxes
+(the ensemble's kind of scene).  The shapes are the kernels' block count
+(the built-in min(ceil(N^3 / 1024), 256) and smaller caps) times the PCG
+iterations per captured batch.  Each candidate runs 2 warm-up steps and `steps` timed ones
 (wall clock around synchronized steps); the shape with the lowest mean step
 time is written to the table with the default's time beside it.  The table
 (flip3d_amd/tuning.json) is what bench.py and the ensemble bench apply
@@ -58,13 +59,20 @@ def launch_for(table, precond, n):
     return int(e["pcg_blocks"]), int(e["pcg_batch"])
 
 
+def scene_for(n):
+    return "benchmark" if n >= 128 else "random_boxes"
+
+
 def time_shape(fa, n, precond, shape, steps, seed):
     code = fa.PRECOND_LEARNED if precond == "learned" else fa.PRECOND_REFERENCE
     with fa.FlipContext(n, precond=code, dt_policy=1, tol_policy=1, tol=1e-5) as g:
         if precond == "learned":
             g.set_learned_weights(WEIGHTS)
         g.set_launch(*shape)
-        g.build_scene(fa.SCENE_RANDOM_BOXES, seed)
+        if scene_for(n) == "benchmark":
+            g.build_scene(fa.SCENE_BENCHMARK)
+        else:
+            g.build_scene(fa.SCENE_RANDOM_BOXES, seed)
         g.srand(1)
         for _ in range(2):
             g.step()
@@ -83,12 +91,17 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--out", default=TABLE)
+    ap.add_argument("--merge", action="store_true", help="update the entries of an existing --out table")
     a = ap.parse_args(argv)
     sys.path.insert(0, ROOT)
     import flip3d_amd as fa
     sizes = [int(s) for s in a.sizes.split(",") if s]
-    table = {"command": "python -m flip3d_amd.autotune " + " ".join(argv if argv is not None else sys.argv[1:]),
-             "steps": a.steps, "seed": a.seed, "shapes": {}}
+    cmd = "python -m flip3d_amd.autotune " + " ".join(argv if argv is not None else sys.argv[1:])
+    table = load(a.out) if a.merge else None
+    if table:
+        table["command"] = table.get("command", "") + " ; " + cmd
+    else:
+        table = {"command": cmd, "steps": a.steps, "seed": a.seed, "shapes": {}}
     for precond in a.precond.split(","):
         out = table["shapes"].setdefault(precond, {})
         for n in sizes:
@@ -97,7 +110,8 @@ def main(argv=None):
                 times[shape] = time_shape(fa, n, precond, shape, a.steps, a.seed)
             best = min(times, key=times.get)
             blocks = best[0] if best[0] else default_blocks(n)
-            out[str(n)] = {"pcg_blocks": blocks, "pcg_batch": best[1], "ms": round(times[best], 4),
+            out[str(n)] = {"scene": scene_for(n), "pcg_blocks": blocks, "pcg_batch": best[1],
+                           "ms": round(times[best], 4),
                            "default_ms": round(times[(0, 8)], 4),
                            "swept": {f"{b or default_blocks(n)}x{t}": round(v, 4) for (b, t), v in times.items()}}
             print(f"[autotune] {precond} N={n}: blocks {blocks} batch {best[1]} {times[best]:.3f} ms "
