@@ -36,7 +36,7 @@ EXPORTS = [
     "flip_gpu_step_begin",
     "flip_gpu_set_learned_weights", "flip_gpu_set_launch", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
     "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
-    "flip_unet_predict", "flip_unet_predict_fused", "flip_unet_save", "flip_unet_load",
+    "flip_unet_predict", "flip_unet_predict_fused", "flip_unet_solve_system", "flip_unet_save", "flip_unet_load",
 ]
 
 
@@ -130,6 +130,7 @@ def lib():
         L.flip_unet_train_step.argtypes = [P, C.c_int, P, P, P, P, C.c_float, C.POINTER(C.c_double)]
         L.flip_unet_predict.argtypes = [P, C.c_int, P, P, P, P, P]
         L.flip_unet_predict_fused.argtypes = [P, P, P, P, P]
+        L.flip_unet_solve_system.argtypes = [P, C.c_int, P, P, P, P]
         L.flip_unet_save.argtypes = [P, C.c_char_p]
         L.flip_unet_load.argtypes = [P, C.c_char_p]
         _lib = L
@@ -471,6 +472,17 @@ class UNet:
             raise ValueError("predict_fused takes one system")
         x = np.empty(b.shape, np.float32)
         _check(lib().flip_unet_predict_fused(self.h, _ptr(A), _ptr(L), _ptr(b), _ptr(x)))
+        return x
+
+    def solve_system(self, A, L, b, path=1):
+        """The network on one system as the learned solver runs it (sigma
+        from per-256-cell partials of b): path 0 through separate kernels,
+        path 1 through the seven fused layer kernels."""
+        A, L, b, _ = self._systems(A, L, b)
+        if A.size != self.R ** 3:
+            raise ValueError("solve_system takes one system")
+        x = np.empty(b.shape, np.float32)
+        _check(lib().flip_unet_solve_system(self.h, int(path), _ptr(A), _ptr(L), _ptr(b), _ptr(x)))
         return x
 
     def save(self, path):

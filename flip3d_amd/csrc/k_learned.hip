@@ -25,11 +25,12 @@
 //
 // Flexible CG (Polak-Ribiere: beta_k = -alpha_{k-1} (z_k.q_{k-1}) /
 // (z_{k-1}.r_{k-1})), because the network makes M nonlinear.  One iteration
-// (two CG kernels, the stop check, the V-cycle) is captured once as a HIP
-// graph: the iteration index, buffer parity, FLUID count and the stop flag
-// live in device memory, kernels after convergence return at once (the
-// network's own kernels still run), and the host launches iteration k+1
-// before it reads iteration k's stop flag.
+// (the direction and A s, the update fused with the first smoothing sweep,
+// the stop check in the level-0 residual, the rest of the V-cycle) is
+// captured once as a HIP graph: the iteration index, buffer parity, FLUID
+// count and the stop flag live in device memory, every kernel after
+// convergence returns at once, and the host launches iteration k+1 before it
+// reads iteration k's stop flag.
 #include "pcg_device.hpp"
 #include "unet_internal.hpp"
 
@@ -162,49 +163,171 @@ __global__ void k_ml_zero(long n, float *a)
 }
 
 // ------------------------------------------------------------ V-cycle
-// level 0 pre-smoothing: z1 = w D^-1 r, z2 = z1 + w D^-1 (r - A z1).  With
-// check set it first applies the flexible-CG stop test of the iteration
-// that has just updated r: every block sums the r.r partials in the same
-// order and stops on convergence or NaN; block 0 records the result,
-// advances the iteration index and also stops at kmax.
-__global__ __launch_bounds__(TB) void k_ml_pre0(const int32_t *__restrict__ iflags, int32_t *fi,
-                                                const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
-                                                int n, const float *__restrict__ L, int subcell, double h2,
-                                                const float *__restrict__ r, const float *__restrict__ dinv, float *z2,
-                                                const double *__restrict__ pb, double *fs, int G, int kmax, int check)
+// One FLUID cell's operands for the level-0 pre-smoothing, all loads issued
+// together: the six FLUID neighbours' (dinv, r), the cell's own, and for a
+// cell next to AIR the level set of the cell and of its AIR neighbours.
+struct Pre0Item {
+    long c;
+    uint32_t m;
+    float dq[6], rq[6];
+    float rc, dc, Lc;
+    float Ln[6];
+};
+
+__device__ __forceinline__ void pre0_load(Pre0Item &it, int t, const int32_t *__restrict__ fl,
+                                          const uint32_t *__restrict__ meta, const float *__restrict__ r,
+                                          const float *__restrict__ dinv, const float *__restrict__ L, int n)
 {
-    if (fi[FI_DONE]) return;
-    if (check) {
-        const double rr = sum_partials(pb, G, 1, 0);
-        const bool conv = rr <= fs[F_EPS];
-        const bool stop = conv || !(rr == rr);
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            const int k = fi[FI_K] + 1;
-            fs[F_RR] = rr;
-            fi[FI_K] = k;
-            if (stop || k >= kmax) {
-                fi[FI_DONE] = 1;
-                fi[FI_CONV] = conv ? 1 : 0;
-            }
-        }
-        if (stop) return;
+    it.c = fl[t];
+    it.m = meta[t];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (it.m >> (2 * u)) & 3;
+        const long q = it.c + (kd == NB_FLUID ? nb_offset(u, n) : 0);
+        it.dq[u] = dinv[q];
+        it.rq[u] = r[q];
     }
-    const int nfl = iflags[I_NFL];
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        uint32_t m = meta[t];
-        float zv[6];
+    it.rc = r[it.c];
+    it.dc = dinv[it.c];
+    it.Lc = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) it.Ln[u] = 0.0f;
+    if (it.m & META_HAS_AIR) {
+        it.Lc = L[it.c];
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
-            const int kd = (m >> (2 * u)) & 3;
-            const long q = c + (kd == NB_FLUID ? nb_offset(u, n) : 0);
-            zv[u] = kd == NB_FLUID ? MG_W * dinv[q] * r[q] : 0.0f;
+            const int kd = (it.m >> (2 * u)) & 3;
+            it.Ln[u] = L[it.c + (kd == NB_AIR ? nb_offset(u, n) : 0)];
         }
-        float rc = r[c], dc = dinv[c];
-        float z1 = MG_W * dc * rc;
-        float az = ax_row(m, zv, z1, c, n, L, subcell, h2);
-        z2[c] = z1 + MG_W * dc * (rc - az);
     }
+}
+
+// z1 = w D^-1 r (the neighbours' recomputed), z2 = z1 + w D^-1 (r - A z1):
+// the row of ax_row with the level sets already loaded
+__device__ __forceinline__ float pre0_row(const Pre0Item &it, int subcell, double h2)
+{
+    const float z1 = MG_W * it.dc * it.rc;
+    double acc = 6.0 * (double)z1;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (it.m >> (2 * u)) & 3;
+        float xr;
+        if (kd == NB_FLUID) xr = MG_W * it.dq[u] * it.rq[u];
+        else if (kd != NB_AIR) xr = z1;  // WALL, or clamped to the cell itself
+        else xr = subcell ? (float)((double)it.Ln[u] / fmin(1.0e-6, (double)it.Lc) * (double)z1) : 0.0f;
+        acc -= (double)xr;
+    }
+    const float az = (float)(acc / h2);
+    return z1 + MG_W * it.dc * (it.rc - az);
+}
+
+// level 0 pre-smoothing of the first V-cycle: z1 = w D^-1 r,
+// z2 = z1 + w D^-1 (r - A z1)
+__global__ __launch_bounds__(TB) void k_ml_pre0(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+                                                const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                                int n, const float *__restrict__ L, int subcell, double h2,
+                                                const float *__restrict__ r, const float *__restrict__ dinv, float *z2)
+{
+    if (fi[FI_DONE]) return;
+    const int nfl = iflags[I_NFL];
+    Pre0Item it;
+    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
+        pre0_load(it, t, fl, meta, r, dinv, L, n);
+        z2[it.c] = pre0_row(it, subcell, h2);
+    }
+}
+
+// Flexible-CG update fused with the level-0 pre-smoothing (iteration k):
+// alpha = zr_k / (q.s); x += alpha s; r' = r - alpha q into rn (the
+// neighbours' r' recomputed from r and q, so r itself is only read); z2 from
+// r' as k_ml_pre0 forms it; partials of r'.r'.  The values are those of the
+// update in a pass of its own followed by k_ml_pre0, bit for bit.  The first
+// item's loads go out before the partial sums.
+struct UpdItem {
+    long c;
+    uint32_t m;
+    float x, s, r, q, d;
+    float rq[6], qq[6], dq[6];
+    float Lc, Ln[6];
+};
+
+__device__ __forceinline__ void upd_load(UpdItem &it, int t, const int32_t *__restrict__ fl,
+                                         const uint32_t *__restrict__ meta, const float *__restrict__ x,
+                                         const float2 *__restrict__ bk, const float *__restrict__ r,
+                                         const float *__restrict__ q, const float *__restrict__ dinv,
+                                         const float *__restrict__ L, int n)
+{
+    it.c = fl[t];
+    it.m = meta[t];
+    it.x = x[it.c];
+    it.s = bk[it.c].y;
+    it.r = r[it.c];
+    it.q = q[it.c];
+    it.d = dinv[it.c];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int kd = (it.m >> (2 * u)) & 3;
+        const long o = it.c + (kd == NB_FLUID ? nb_offset(u, n) : 0);
+        it.rq[u] = r[o];
+        it.qq[u] = q[o];
+        it.dq[u] = dinv[o];
+    }
+    it.Lc = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) it.Ln[u] = 0.0f;
+    if (it.m & META_HAS_AIR) {
+        it.Lc = L[it.c];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int kd = (it.m >> (2 * u)) & 3;
+            it.Ln[u] = L[it.c + (kd == NB_AIR ? nb_offset(u, n) : 0)];
+        }
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_fcg_b_pre0(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+                                                   const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
+                                                   int n, const float *__restrict__ L, int subcell, double h2, float *x,
+                                                   const float *__restrict__ r, float *rn,
+                                                   const float2 *__restrict__ B0, const float2 *__restrict__ B1,
+                                                   const float *__restrict__ q, const float *__restrict__ dinv,
+                                                   float *z2, const double *__restrict__ pa, double *fs, double *pb,
+                                                   int G)
+{
+    if (fi[FI_DONE]) return;
+    const int k = fi[FI_K];
+    const int nfl = iflags[I_NFL];
+    const float2 *__restrict__ bk = (k & 1) ? B1 : B0;
+    const int t0 = blockIdx.x * TB + threadIdx.x;
+    UpdItem it;
+    if (t0 < nfl) upd_load(it, t0, fl, meta, x, bk, r, q, dinv, L, n);
+    double qs = sum_partials(pa, G, 1, 0);
+    double alpha = fs[F_ZR0 + (k & 1)] / qs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) fs[F_AL0 + (k & 1)] = alpha;
+    const float a = (float)alpha, na = (float)-alpha;
+    double rr = 0.0;
+    for (int t = t0; t < nfl; t += gridDim.x * TB) {
+        if (t != t0) upd_load(it, t, fl, meta, x, bk, r, q, dinv, L, n);
+        x[it.c] = it.x + a * it.s;
+        const float rc = it.r + na * it.q;
+        rn[it.c] = rc;
+        rr += (double)(rc * rc);
+        // k_ml_pre0 on r'
+        const float z1 = MG_W * it.d * rc;
+        double acc = 6.0 * (double)z1;
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int kd = (it.m >> (2 * u)) & 3;
+            float xr;
+            if (kd == NB_FLUID) xr = MG_W * it.dq[u] * (it.rq[u] + na * it.qq[u]);
+            else if (kd != NB_AIR) xr = z1;
+            else xr = subcell ? (float)((double)it.Ln[u] / fmin(1.0e-6, (double)it.Lc) * (double)z1) : 0.0f;
+            acc -= (double)xr;
+        }
+        const float az = (float)(acc / h2);
+        z2[it.c] = z1 + MG_W * it.d * (rc - az);
+    }
+    store_partials(pb, 1, rr, 0.0, 0.0, 1);
 }
 
 // dense level pre-smoothing from zero: t = w D^-1 b, x = t + w D^-1 (b - A t)
@@ -235,26 +358,78 @@ __global__ __launch_bounds__(TBL) void k_ml_pre(const int32_t *__restrict__ fi, 
     x[c] = t + MG_W * dc * (bc - at);
 }
 
-// level 0 residual: res = r - A z2 on FLUID cells (res is zero elsewhere)
-__global__ __launch_bounds__(TBL) void k_ml_resid0(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
+// level 0 residual: res = r - A z2 on FLUID cells (res is zero elsewhere).
+// With check set it first applies the flexible-CG stop test of the
+// iteration that has just updated r: every block sums k_fcg_b_pre0's r'.r'
+// partials in the same order and stops on convergence or NaN; block 0
+// records the result, advances the iteration index and also stops at kmax.
+__global__ __launch_bounds__(TBL) void k_ml_resid0(const int32_t *__restrict__ iflags, int32_t *fi,
                                                    const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
                                                    int n, const float *__restrict__ L, int subcell, double h2,
                                                    const float *__restrict__ r, const float *__restrict__ z2,
-                                                   float *res)
+                                                   float *res, const double *__restrict__ pb, double *fs, int G,
+                                                   int kmax, int check)
 {
     if (fi[FI_DONE]) return;
     const int nfl = iflags[I_NFL];
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfl; t += gridDim.x * blockDim.x) {
-        const long c = fl[t];
-        const uint32_t m = meta[t];
-        float zv[6];
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    long c0 = 0;
+    uint32_t m0 = 0;
+    float zv0[6], zc0 = 0.0f, rc0 = 0.0f;
+    if (t0 < nfl) {
+        c0 = fl[t0];
+        m0 = meta[t0];
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
-            const int kd = (m >> (2 * u)) & 3;
-            const float w = z2[c + (kd == NB_FLUID ? nb_offset(u, n) : 0)];
-            zv[u] = kd == NB_FLUID ? w : 0.0f;
+            const int kd = (m0 >> (2 * u)) & 3;
+            zv0[u] = z2[c0 + (kd == NB_FLUID ? nb_offset(u, n) : 0)];
         }
-        res[c] = r[c] - ax_row(m, zv, z2[c], c, n, L, subcell, h2);
+        zc0 = z2[c0];
+        rc0 = r[c0];
+    }
+    if (check) {
+        double a = 0.0;
+        for (int i = threadIdx.x; i < G; i += TBL) a += pb[i];
+        __shared__ double red[TBL / 64];
+        a = wave_sum(a);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+        __syncthreads();
+        double rr = 0.0;
+#pragma unroll
+        for (int w = 0; w < TBL / 64; ++w) rr += red[w];
+        const bool conv = rr <= fs[F_EPS];
+        const bool stop = conv || !(rr == rr);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const int k = fi[FI_K] + 1;
+            fs[F_RR] = rr;
+            fi[FI_K] = k;
+            if (stop || k >= kmax) {
+                fi[FI_DONE] = 1;
+                fi[FI_CONV] = conv ? 1 : 0;
+            }
+        }
+        if (stop) return;
+    }
+    for (int t = t0; t < nfl; t += gridDim.x * blockDim.x) {
+        long c = c0;
+        uint32_t m = m0;
+        float zv[6], zc = zc0, rc = rc0;
+#pragma unroll
+        for (int u = 0; u < 6; ++u) zv[u] = zv0[u];
+        if (t != t0) {
+            c = fl[t];
+            m = meta[t];
+#pragma unroll
+            for (int u = 0; u < 6; ++u) {
+                const int kd = (m >> (2 * u)) & 3;
+                zv[u] = z2[c + (kd == NB_FLUID ? nb_offset(u, n) : 0)];
+            }
+            zc = z2[c];
+            rc = r[c];
+        }
+#pragma unroll
+        for (int u = 0; u < 6; ++u) zv[u] = ((m >> (2 * u)) & 3) == NB_FLUID ? zv[u] : 0.0f;
+        res[c] = rc - ax_row(m, zv, zc, c, n, L, subcell, h2);
     }
 }
 
@@ -434,13 +609,16 @@ __global__ __launch_bounds__(TBL) void k_ml_post0_a(const int32_t *__restrict__ 
 
 // level 0 post-smoothing, second sweep: z = z4 + w D^-1 (r - A z4) into the
 // .x slots of the (z, s) buffer the next flexible-CG iteration reads (parity
-// (k + 1) & 1 with k = fi[FI_K]); partials z.r and, when q is given, z.q
+// (k + 1) & 1 with k = fi[FI_K]); partials z.r and, when q is given, z.q;
+// with rcopy given, r is also copied there (the iteration's r' into the
+// residual the next update reads)
 __global__ __launch_bounds__(TB) void k_ml_post0_b(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
                                                    const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
                                                    int n, const float *__restrict__ L, int subcell, double h2,
                                                    const float *__restrict__ r, const float *__restrict__ dinv,
                                                    const float *__restrict__ z4, const float *__restrict__ q,
-                                                   float2 *B0, float2 *B1, double *part, int32_t *hflag)
+                                                   float2 *B0, float2 *B1, double *part, int32_t *hflag,
+                                                   float *rcopy)
 {
     // the stop flag into mapped host memory (null outside the iteration)
     if (hflag && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -458,6 +636,7 @@ __global__ __launch_bounds__(TB) void k_ml_post0_b(const int32_t *__restrict__ i
 #pragma unroll
         for (int u = 0; u < 6; ++u) zv[u] = nbr_or_zero(m, u, z4, c, n);
         float rc = r[c];
+        if (rcopy) rcopy[c] = rc;  // r' becomes the residual the next update reads
         float az = ax_row(m, zv, z4[c], c, n, L, subcell, h2);
         float z = z4[c] + MG_W * dinv[c] * (rc - az);
         B[c].x = z;
@@ -488,7 +667,9 @@ __global__ void k_fcg_begin(const int32_t *__restrict__ iflags, const double *__
 }
 
 // part a(k): z.r_k from the preconditioner partials, beta, s_k = z + beta
-// s_{k-1} (neighbours recomputed), q = A s_k, partials of q.s
+// s_{k-1} (neighbours recomputed), q = A s_k, partials of q.s.  The first
+// item's loads go out before the partial sums (as in the reference PCG's
+// k_pcg_a, whose stencil row this is).
 __global__ __launch_bounds__(TB) void k_fcg_a(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
                                               const int32_t *__restrict__ fl, const uint32_t *__restrict__ meta,
                                               int n, const float *__restrict__ L, int subcell, double h2, float2 *B0,
@@ -500,57 +681,24 @@ __global__ __launch_bounds__(TB) void k_fcg_a(const int32_t *__restrict__ iflags
     const int nfl = iflags[I_NFL];
     const float2 *__restrict__ rd = (k & 1) ? B0 : B1;
     float2 *wr = (k & 1) ? B1 : B0;
+    const int t0 = blockIdx.x * TB + threadIdx.x;
+    StencilItem it;
+    if (t0 < nfl) stencil_load(it, t0, fl, meta, rd, L, n);
     double zr, zq;
     sum_partials2(pz, G, zr, zq);
     float beta = 0.0f;
     if (k > 0) beta = (float)(-fs[F_AL0 + ((k - 1) & 1)] * zq / fs[F_ZR0 + ((k - 1) & 1)]);
     if (blockIdx.x == 0 && threadIdx.x == 0) fs[F_ZR0 + (k & 1)] = zr;
     double qs = 0.0;
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        uint32_t m = meta[t];
-        float2 v = rd[c];
-        float sv[6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            const int kd = (m >> (2 * u)) & 3;
-            const float2 w = rd[c + (kd == NB_OUT ? 0 : nb_offset(u, n))];
-            sv[u] = kd == NB_FLUID ? (k == 0 ? w.x : w.x + beta * w.y) : 0.0f;
-        }
-        float ss = k == 0 ? v.x : v.x + beta * v.y;
-        float qc = ax_row(m, sv, ss, c, n, L, subcell, h2);
-        wr[c] = make_float2(v.x, ss);
-        q[c] = qc;
+    for (int t = t0; t < nfl; t += gridDim.x * TB) {
+        if (t != t0) stencil_load(it, t, fl, meta, rd, L, n);
+        float ss;
+        const float qc = stencil_row(it, k, beta, subcell, h2, ss);
+        wr[it.c] = make_float2(it.v.x, ss);
+        q[it.c] = qc;
         qs += (double)(qc * ss);
     }
     store_partials(pa, 1, qs, 0.0, 0.0, 1);
-}
-
-// part b(k): alpha = zr_k / (q.s); x += alpha s; r -= alpha q; partials r.r
-__global__ __launch_bounds__(TB) void k_fcg_b(const int32_t *__restrict__ iflags, const int32_t *__restrict__ fi,
-                                              const int32_t *__restrict__ fl, float *x, float *r,
-                                              const float2 *__restrict__ B0, const float2 *__restrict__ B1,
-                                              const float *__restrict__ q, const double *__restrict__ pa, double *fs,
-                                              double *pb, int G)
-{
-    if (fi[FI_DONE]) return;
-    const int k = fi[FI_K];
-    const int nfl = iflags[I_NFL];
-    const float2 *__restrict__ bk = (k & 1) ? B1 : B0;
-    double qs = sum_partials(pa, G, 1, 0);
-    double alpha = fs[F_ZR0 + (k & 1)] / qs;
-    if (blockIdx.x == 0 && threadIdx.x == 0) fs[F_AL0 + (k & 1)] = alpha;
-    const float a = (float)alpha, na = (float)-alpha;
-    double rr = 0.0;
-    for (int t = blockIdx.x * TB + threadIdx.x; t < nfl; t += gridDim.x * TB) {
-        long c = fl[t];
-        float xc = x[c], sc = bk[c].y, rc = r[c], qc = q[c];
-        x[c] = xc + a * sc;
-        rc = rc + na * qc;
-        r[c] = rc;
-        rr += (double)(rc * rc);
-    }
-    store_partials(pb, 1, rr, 0.0, 0.0, 1);
 }
 
 struct Level {
@@ -575,7 +723,7 @@ struct LearnedSolver {
     int n = 0, K = 0;
     std::vector<Level> lv;  // lv[1..K]
     UNet *net = nullptr;
-    float *dinv0 = nullptr, *z2 = nullptr, *z4 = nullptr, *res0 = nullptr;
+    float *dinv0 = nullptr, *z2 = nullptr, *z4 = nullptr, *res0 = nullptr, *rn = nullptr;
     float *sigma = nullptr;
     double *part = nullptr;  // [4 * GMAX]: pa, pb, pz (stride 2)
     double *fs = nullptr;
@@ -598,7 +746,7 @@ void learned_free(LearnedSolver *ls)
         for (void *b : bufs)
             if (b) (void)hipFree(b);
     }
-    void *bufs[] = {ls->dinv0, ls->z2, ls->z4, ls->res0, ls->sigma, ls->part, ls->fs, ls->fi, ls->upart};
+    void *bufs[] = {ls->dinv0, ls->z2, ls->z4, ls->res0, ls->rn, ls->sigma, ls->part, ls->fs, ls->fi, ls->upart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ls->h_flag) (void)hipHostFree(ls->h_flag);
@@ -659,6 +807,7 @@ static LearnedSolver *learned_get(FlipCtx *c)
         dmalloc(&ls->z2, C);
         dmalloc(&ls->z4, C);
         dmalloc(&ls->res0, C);
+        dmalloc(&ls->rn, C);
         dmalloc(&ls->sigma, 4);
         dmalloc(&ls->part, 4 * GMAX);
         dmalloc(&ls->fs, 8);
@@ -677,21 +826,29 @@ static LearnedSolver *learned_get(FlipCtx *c)
 }
 
 // z = M(r) into the (z, s) buffer of parity (fi[FI_K] + 1) & 1, partials (z.r,
-// z.q) in pz (q may be null)
+// z.q) in pz (q may be null).  iter == false: the first V-cycle, on c->r.
+// iter == true: flexible-CG iteration k's update fused with the
+// pre-smoothing (k_fcg_b_pre0, r' into ls->rn), the stop test in the level-0
+// residual, and r' copied back into c->r by the last kernel.
 static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell, const float *q, double *pz,
-                   const double *pb, bool check)
+                   const double *pa, double *pb, bool iter)
 {
     const int n = c->n, K = ls->K;
     int32_t *fi = ls->fi;
     const unsigned G0 = (unsigned)std::min<long>(nb(c->C), 4096);  // level-0 smoothing grid (grid-stride)
-    launch_timed(c, "ml_pre0", k_ml_pre0, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
-                       ls->dinv0, ls->z2, pb, ls->fs, G, FCG_MAX, check ? 1 : 0);
+    const float *r = iter ? ls->rn : c->r;
+    if (iter)
+        launch_timed(c, "fcg_b", k_fcg_b_pre0, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
+                     c->press, c->r, ls->rn, c->zs[0], c->zs[1], c->q, ls->dinv0, ls->z2, pa, ls->fs, pb, G);
+    else
+        launch_timed(c, "ml_pre0", k_ml_pre0, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
+                     ls->dinv0, ls->z2);
     // down: restrict level l-1's residual into b_l, smooth level l
     for (int l = 1; l <= K; ++l) {
         Level &L = ls->lv[l];
         if (l == 1) {
             launch_timed(c, "ml_resid0", k_ml_resid0, G0, TBL, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
-                               c->r, ls->z2, ls->res0);
+                         r, ls->z2, ls->res0, pb, ls->fs, G, FCG_MAX, iter ? 1 : 0);
             launch_timed(c, "ml_sum8", k_ml_sum8, nb(L.C), TBL, fi, L.R, n, ls->res0, L.b,
                                l == K ? L.A : nullptr, ls->upart);
         } else {
@@ -707,7 +864,8 @@ static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell,
     }
     Level &KL = ls->lv[K];
     t_begin(c, "unet");
-    unet_solve_partials(ls->net, KL.A, KL.L, KL.b, ls->upart, KL.x, ls->sigma);
+    unet_infer_layers(ls->net, KL.A, KL.L, KL.b, ls->upart, unet_partials_count(ls->net), fi + FI_DONE, KL.x,
+                      ls->sigma);
     t_end(c);
     // up: correct level l with level l+1, smooth
     for (int l = K - 1; l >= 1; --l) {
@@ -717,10 +875,10 @@ static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell,
         launch_timed(c, "ml_post_b", k_ml_post_b, nb(L.C), TBL, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
                            L.dinv, L.t, L.x);
     }
-    launch_timed(c, "ml_post0_a", k_ml_post0_a, G0, TBL, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2,
-                       c->r, ls->dinv0, ls->z2, ls->lv[1].x, ls->lv[1].R, ls->z4);
-    launch_timed(c, "ml_post0_b", k_ml_post0_b, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->r,
-                       ls->dinv0, ls->z4, q, c->zs[0], c->zs[1], pz, check ? ls->d_flag : nullptr);
+    launch_timed(c, "ml_post0_a", k_ml_post0_a, G0, TBL, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, r,
+                 ls->dinv0, ls->z2, ls->lv[1].x, ls->lv[1].R, ls->z4);
+    launch_timed(c, "ml_post0_b", k_ml_post0_b, G, TB, c->iflags, fi, c->fl, c->flmeta, n, c->L, subcell, h2, r,
+                 ls->dinv0, ls->z4, q, c->zs[0], c->zs[1], pz, iter ? ls->d_flag : nullptr, iter ? c->r : nullptr);
 }
 
 // Flexible CG with the multilevel preconditioner from the current x (press)
@@ -746,7 +904,7 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
     }
     hipLaunchKernelGGL(k_fcg_begin, 1, 64, 0, c->stream, c->iflags, c->scal, ls->fs, ls->fi);
     t_end(c);
-    vcycle(c, ls, G, h2, subcell, nullptr, pz, pb, false);  // z_0 into B_1.x
+    vcycle(c, ls, G, h2, subcell, nullptr, pz, pa, pb, false);  // z_0 into B_1.x
 
     const bool direct = c->profiling || c->direct_launch;
     const long key = (long)G * 2 + subcell;
@@ -759,10 +917,9 @@ int learned_fcg(FlipCtx *c, int G, double h2, int subcell, double eps, bool *con
     auto iteration = [&]() {
         launch_timed(c, "fcg_a", k_fcg_a, G, TB, c->iflags, ls->fi, c->fl, c->flmeta, n, c->L, subcell, h2, c->zs[0],
                      c->zs[1], c->q, pz, ls->fs, pa, G);
-        launch_timed(c, "fcg_b", k_fcg_b, G, TB, c->iflags, ls->fi, c->fl, c->press, c->r, c->zs[0], c->zs[1], c->q,
-                     pa, ls->fs, pb, G);
-        // its first kernel applies the stop test, its last writes the flag
-        vcycle(c, ls, G, h2, subcell, c->q, pz, pb, true);
+        // the update and the pre-smoothing, then the stop test (in the level-0
+        // residual); the V-cycle's last kernel writes the flag
+        vcycle(c, ls, G, h2, subcell, c->q, pz, pa, pb, true);
     };
     if (!direct && !ls->graph) {
         hipGraph_t g;

@@ -182,63 +182,6 @@ __global__ __launch_bounds__(TB) void k_pcg_init_fin(int G, const double *__rest
     }
 }
 
-// One FLUID cell's operands for the stencil, all loads issued together: the
-// cell's own (z, s) pair, the six neighbours' pairs (a non-FLUID neighbour's
-// slot is in range and ignored; outside the grid the cell's own slot is read)
-// and, for a cell next to AIR, the level set of the cell and of its
-// neighbours.
-struct StencilItem {
-    long c;
-    uint32_t m;
-    float2 v;
-    float2 w[6];
-    float Lc;
-    float Ln[6];
-};
-
-__device__ __forceinline__ void stencil_load(StencilItem &it, int t, const int32_t *__restrict__ fl,
-                                             const uint32_t *__restrict__ meta, const float2 *__restrict__ rd,
-                                             const float *__restrict__ L, int n)
-{
-    it.c = fl[t];
-    it.m = meta[t];
-    it.v = rd[it.c];
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-        const int kd = (it.m >> (2 * u)) & 3;
-        it.w[u] = rd[it.c + (kd == NB_OUT ? 0 : nb_offset(u, n))];
-    }
-    it.Lc = 0.0f;
-#pragma unroll
-    for (int u = 0; u < 6; ++u) it.Ln[u] = 0.0f;
-    if (it.m & META_HAS_AIR) {
-        it.Lc = L[it.c];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            const int kd = (it.m >> (2 * u)) & 3;
-            it.Ln[u] = L[it.c + (kd == NB_AIR ? nb_offset(u, n) : 0)];
-        }
-    }
-}
-
-// solver.cpp:19-41 compute_Ax row of s = z + beta*s_old (k > 0) or s = z (k == 0)
-__device__ __forceinline__ float stencil_row(const StencilItem &it, int k, float beta, int subcell, double h2,
-                                             float &ss)
-{
-    ss = k == 0 ? it.v.x : it.v.x + beta * it.v.y;
-    double acc = 6.0 * (double)ss;
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-        const int kd = (it.m >> (2 * u)) & 3;
-        float xr;
-        if (kd == NB_FLUID) xr = k == 0 ? it.w[u].x : it.w[u].x + beta * it.w[u].y;
-        else if (kd != NB_AIR) xr = ss;  // WALL, or clamped to the cell itself
-        else xr = subcell ? (float)((double)it.Ln[u] / fmin(1.0e-6, (double)it.Lc) * (double)ss) : 0.0f;
-        acc -= (double)xr;
-    }
-    return (float)(acc / h2);
-}
-
 // iteration part a (see the file comment).  Buffer B_p holds (z, s) pairs:
 // pcg_a(k) reads (z, s_{k-1}) from B_{(k+1)&1} and writes s_k into B_{k&1}.y;
 // pcg_b(k) reads s_k from B_{k&1}.y and writes the new z into B_{k&1}.x.

@@ -29,7 +29,7 @@
 
 namespace {
 
-constexpr float OUT_UNIT = 64.0f;
+constexpr float OUT_UNIT = UNET_OUT_UNIT;
 
 inline unsigned nb(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
@@ -229,6 +229,7 @@ struct Trainer {
     float *L = nullptr, *b = nullptr, *x = nullptr, *t = nullptr, *err = nullptr, *Aerr = nullptr, *At = nullptr;
     float *sigma = nullptr, *coef = nullptr;
     double *part = nullptr, *loss = nullptr;
+    double *spart = nullptr;  // per-256-cell right-hand-side partials of one system (solver inference)
     double *gpart = nullptr;  // gradient-norm partials (256)
     float *gscale = nullptr;  // clip factor of the last step
     int blocks = 64;
@@ -593,6 +594,7 @@ void trainer_alloc(Trainer &t, UNet *u)
     dmalloc(&t.sigma, u->cap);
     dmalloc(&t.coef, u->cap);
     dmalloc(&t.part, 2L * t.blocks * u->cap);
+    dmalloc(&t.spart, 2L * unet_partials_count(u));
     dmalloc(&t.loss, u->cap);
     dmalloc(&t.gpart, 256);
     dmalloc(&t.gscale, 1);
@@ -611,6 +613,7 @@ void trainer_free(Trainer &t)
     dfree(t.sigma);
     dfree(t.coef);
     dfree(t.part);
+    dfree(t.spart);
     dfree(t.loss);
     dfree(t.gpart);
     dfree(t.gscale);
@@ -797,6 +800,29 @@ int flip_unet_predict_fused(flip_unet *h, const int8_t *A, const float *L, const
         const long n = u->vox(0, 1);
         stage_systems(t, 1, A, L, b, nullptr);
         unet_infer(u, t.sigma, t.err);
+        FLIP_CHECK(hipMemcpyAsync(x, t.err, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        FLIP_CHECK(hipStreamSynchronize(s));
+        return FLIP_OK;
+    });
+}
+
+int flip_unet_solve_system(flip_unet *h, int path, const int8_t *A, const float *L, const float *b, float *x)
+{
+    return guarded_u([&]() -> int {
+        if (!h || !A || !L || !b || !x) return fail(FLIP_E_ARG, "null argument");
+        if (path != 0 && path != 1) return fail(FLIP_E_ARG, "path must be 0 or 1");
+        Trainer &t = h->impl.tr;
+        UNet *u = t.net;
+        hipStream_t s = u->stream;
+        const long n = u->vox(0, 1);
+        FLIP_CHECK(hipMemcpyAsync(t.A, A, n, hipMemcpyHostToDevice, s));
+        FLIP_CHECK(hipMemcpyAsync(t.L, L, n * sizeof(float), hipMemcpyHostToDevice, s));
+        FLIP_CHECK(hipMemcpyAsync(t.b, b, n * sizeof(float), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_rhs_partials, dim3(nb(n)), dim3(256), 0, s, t.A, t.b, n, t.spart);
+        if (path == 0)
+            unet_solve_partials(u, t.A, t.L, t.b, t.spart, t.err, t.sigma);
+        else
+            unet_infer_layers(u, t.A, t.L, t.b, t.spart, unet_partials_count(u), nullptr, t.err, t.sigma);
         FLIP_CHECK(hipMemcpyAsync(x, t.err, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         FLIP_CHECK(hipStreamSynchronize(s));
         return FLIP_OK;

@@ -47,6 +47,7 @@ void unet_to_f32(long n, const uint16_t *a, float *b, hipStream_t s);
 constexpr int UNET_FEATURES = 8;  // input channels (5 used, padded to 8)
 constexpr float UNET_LEAK = 0.1f;  // negative slope of the leaky ReLU
 constexpr int UNET_NCONV = 7;
+constexpr float UNET_OUT_UNIT = 64.0f;  // the network's output unit: x = out UNET_OUT_UNIT sigma / R^2
 constexpr float UNET_CLIP = 1.0f;  // global gradient-norm clip of the training step
 
 struct UNetConv {
@@ -97,4 +98,10 @@ void unet_solve(UNet *u, const int8_t *A, const float *L, const float *r, float 
 void unet_solve_partials(UNet *u, const int8_t *A, const float *L, const float *r, const double *part, float *x,
                          float *sigma);
 int unet_partials_count(const UNet *u);
+// the same forward in seven layer kernels (k_infer.hip): features built while
+// staging the first layer, poolings and the head in the layers' epilogues;
+// every kernel returns at once while *stop is nonzero (stop may be null).
+// Equals unet_solve_partials bit for bit.
+void unet_infer_layers(UNet *u, const int8_t *A, const float *L, const float *b, const double *part, int nparts,
+                       const int32_t *stop, float *x, float *sigma);
 bool unet_read_weights(const char *path, long nparam, std::vector<float> &w, std::string &err);

@@ -54,7 +54,7 @@ PASSES = [
        ("pcg_setup", "learned_setup"), None, None, None),
     _p("solver_iteration", "PCG iteration (over the fluid list; flexible CG for the learned solver)",
        lambda u: 117 * u["Cf"],
-       ("pcg_a", "pcg_b", "fcg_a", "fcg_b"), r"^k_pcg_a$|^k_pcg_b(<|$)|^k_fcg_a$|^k_fcg_b$", "pcg_a|fcg_a",
+       ("pcg_a", "pcg_b", "fcg_a", "fcg_b"), r"^k_pcg_a$|^k_pcg_b(<|$)|^k_fcg_a$|^k_fcg_b_pre0$", "pcg_a|fcg_a",
        r"^k_pcg_a$|^k_fcg_a$"),
     _p("gradient", "grad-sub + second enforce_boundary",
        lambda u: 8 * u["C"] + 8 * u["F"] + (8 * u["F"] + u["C"]),

@@ -53,6 +53,13 @@ int flip_unet_predict(flip_unet *net, int S, const int8_t *A, const float *L, co
  * convolutions.  The result equals predict()'s bit for bit; no counterpart
  * in the reference (the learned solver is this library's own). */
 int flip_unet_predict_fused(flip_unet *net, const int8_t *A, const float *L, const float *b, float *x);
+/* The network as the learned solver runs it on one system (S = 1), with
+ * sigma from per-256-cell partials of b: path 0 through separate kernels
+ * (features, seven convolutions, three poolings), path 1 through the seven
+ * layer kernels that stage each brick's halo in LDS and fuse the features,
+ * poolings and head.  The two paths agree bit for bit; no counterpart in the
+ * reference (the learned solver is this library's own). */
+int flip_unet_solve_system(flip_unet *net, int path, const int8_t *A, const float *L, const float *b, float *x);
 /* Weights file: "FLIPUNET", int32 version 1, int32 count, count floats. */
 int flip_unet_save(flip_unet *net, const char *path);
 int flip_unet_load(flip_unet *net, const char *path);

@@ -194,6 +194,30 @@ def test_training_fits_solver_systems():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("R", [16, 24, 32, 40, 48])
+def test_layer_kernels_equal_separate_kernels(R):
+    """The solver's seven layer kernels (each brick's halo and weights staged
+    in LDS; features, poolings and head fused) return the separate kernels'
+    result bit for bit: shipped weights at R=32, random weights at the other
+    coarse-level sides the solver uses (16-48; at 24, 40 and 48 bricks reach
+    past the cube)."""
+    import flip3d_amd as fa
+    A, L, b, _ = box_system(11 + R, R)
+    with fa.UNet(R, capacity=1, seed=R) as net:
+        if R == 32:
+            net.load(os.path.join(os.path.dirname(fa.__file__), "weights", "unet_coarse.bin"))
+        else:
+            net.set_params(random_params(R).astype(np.float32))
+        ref = net.solve_system(A, L, b, path=0)
+        got = net.solve_system(A, L, b, path=1)
+        assert np.array_equal(ref.view(np.uint32), got.view(np.uint32)), (R, np.abs(ref - got).max())
+        assert np.abs(ref).max() > 0 and np.all(got[A != 1] == 0)
+        # predict() forms sigma from other partial sums: equal up to its rounding
+        pr = net.predict(A[None], L[None], b[None])[0]
+        assert np.abs(pr - got).max() <= 1e-5 * np.abs(pr).max(), (R, np.abs(pr - got).max())
+
+
+@pytest.mark.gpu
 def test_fused_inference_equals_predict():
     """The solver's inference path (upsample-adds and head fused into the
     convolutions) returns predict()'s result bit for bit, with the shipped
