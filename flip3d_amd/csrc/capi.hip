@@ -79,6 +79,9 @@ void free_particles(FlipCtx *c)
     dfree(c->deep);
     dfree(c->ev_list);
     dfree(c->ev_off);
+    dfree(c->repo_ids);
+    dfree(c->repo_order);
+    dfree(c->repo_loff);
     dfree(c->ev_slot);
     dfree(c->ev_rand);
     dfree(c->partials);
@@ -110,6 +113,9 @@ void alloc_particles(FlipCtx *c, int64_t cap)
     dalloc(&c->ev_list, 3 * n + 3);  // the corrector's (creation index, slot, events) triples
     dalloc(&c->ev_off, big);
     dalloc(&c->ev_slot, n);
+    dalloc(&c->repo_ids, n);
+    dalloc(&c->repo_order, n);
+    dalloc(&c->repo_loff, n + 1);
     c->ev_cap = 2 * (int64_t)n + 1024;
     dalloc(&c->ev_rand, 3 * (size_t)c->ev_cap);
     dalloc(&c->partials, 4 * (big / 256 + 8));
@@ -215,7 +221,7 @@ void alloc_grids(FlipCtx *c)
     dalloc(&c->iflags, 16);
     dalloc(&c->tickets, 8);
     FLIP_CHECK(hipMemset(c->tickets, 0, 8 * sizeof(unsigned)));
-    dalloc(&c->ev_count, 4);
+    dalloc(&c->ev_count, EV_SLOTS);
     dalloc(&c->dtv, 4);
     dalloc(&c->rng_dev, 64);
     dalloc(&c->rng_table, (size_t)RNG_CHUNK * 31);
@@ -459,12 +465,12 @@ static void run_stage(FlipCtx *c, int stage, flip_step_stats *st)
         k_launch_g2p(c);
         break;
     case FLIP_STAGE_ADVECT: {
-        FLIP_CHECK(hipMemsetAsync(c->ev_count + EV_NONFINITE, 0, sizeof(int32_t), c->stream));
+        k_launch_clear_events(c, EV_NONFINITE_ADV, EV_NONFINITE_ADV + 1);
         k_launch_advect(c);
         c->sorted = false;
-        int m = k_stuck_and_reposition(c);  // leaves the particles sorted
+        k_stuck_and_reposition(c);  // leaves the particles sorted; counts read at the next wait
         c->sorted = true;
-        if (st) st->n_reposition = m;
+        c->advected = true;
         break;
     }
     case FLIP_STAGE_CORRECT: {
@@ -809,10 +815,16 @@ int flip_gpu_stage(flip_ctx *ctx, int stage, flip_step_stats *st)
         FLIP_CHECK(hipGetLastError());
         if (st && c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
         c->solve_timed = false;
+        int32_t ev[EV_SLOTS];
+        if (c->corrected || c->advected) FLIP_CHECK(hipMemcpy(ev, c->ev_count, sizeof(ev), hipMemcpyDeviceToHost));
+        if (c->advected) {
+            c->advected = false;
+            if (st) st->n_reposition = ev[EV_MOVED];
+            if (ev[EV_NONFINITE_ADV])
+                return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity in the advection");
+        }
         if (c->corrected) {
             c->corrected = false;
-            int32_t ev[4];
-            FLIP_CHECK(hipMemcpy(ev, c->ev_count, sizeof(ev), hipMemcpyDeviceToHost));
             if (st) st->n_jitter = ev[3];
             if (ev[1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
         }
@@ -935,9 +947,8 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         k_launch_extrapolate(c);
         k_launch_g2p(c, true);  // reads the extrapolation's (u, u - u_save) pairs
         run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
-        FLIP_CHECK(hipMemsetAsync(c->ev_count + EV_NONFINITE, 0, sizeof(int32_t), c->stream));
         if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
-        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 8, c->ev_count, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
+        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 8, c->ev_count, EV_SLOTS * sizeof(int32_t), hipMemcpyDeviceToHost,
                                   c->stream));
         FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 16, c->dtv, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));
@@ -948,13 +959,17 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         st->ms_step = elapsed(c->ev_a, c->ev_b);
         if (c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
         c->solve_timed = false;
+        c->advected = false;
+        st->n_reposition = c->h_pinned[8 + EV_MOVED];
+        if (c->h_pinned[8 + EV_NONFINITE_ADV])
+            return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity in the advection");
         if (c->corrected) {
             st->n_jitter = c->h_pinned[8 + 3];
             c->corrected = false;
             if (c->h_pinned[8 + 1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
+            if (c->h_pinned[8 + EV_NONFINITE])
+                return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
         }
-        if (c->h_pinned[8 + EV_NONFINITE])
-            return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
         timing_collect(c);
         return FLIP_OK;
     });

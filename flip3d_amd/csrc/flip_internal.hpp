@@ -42,7 +42,13 @@ void learned_free(LearnedSolver *ls);
 // Device buffers of one context.  Particles live in cell-sorted order and are
 // re-permuted after every sort; meta carries creation index << 2 | type.
 constexpr int RNG_CHUNK = 1024;  // rand() draws per chunk of the device's generator (its table's rows)
-constexpr int EV_NONFINITE = 2;  // slot of ev_count set when a step writes a non-finite particle
+// slots of ev_count (8 ints, zeroed where a step begins): the corrector's
+// listed particles [0], draw overflow [1], non-finite update [EV_NONFINITE]
+// and events [3]; the advection's stuck particles [EV_STUCK], the
+// repositioned ones [EV_MOVED], their resample levels [EV_NLEV] and its
+// non-finite flag [EV_NONFINITE_ADV]
+constexpr int EV_NONFINITE = 2;
+constexpr int EV_STUCK = 4, EV_MOVED = 5, EV_NLEV = 6, EV_NONFINITE_ADV = 7, EV_SLOTS = 8;
 
 struct FlipCtx {
     flip_params prm;
@@ -111,6 +117,7 @@ struct FlipCtx {
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
     bool solve_timed = false;  // ev_s0/ev_s1 bracket a solve whose time is not read yet
     bool corrected = false;    // a corrector ran; its jitter-event count is not read yet
+    bool advected = false;     // an advection ran; its reposition count and checks are not read yet
     hipGraphExec_t ic_graph = nullptr;  // captured batch of wavefront-preconditioned iterations
     long ic_graph_key = -1;             // subcell, preconditioner mode and block count it was captured for
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
@@ -130,7 +137,8 @@ struct FlipCtx {
     // event buffers (host-drawn rand() consumers)
     int32_t *ev_list = nullptr;  // (slot) of particles with events
     double *dtv = nullptr;        // [0] the time step the kernels use (chosen on the device)
-    int32_t *ev_count = nullptr;  // [0] event count, [EV_NONFINITE] a non-finite particle was written
+    int32_t *ev_count = nullptr;  // EV_SLOTS event counts and flags (see EV_STUCK)
+    int32_t *repo_ids = nullptr, *repo_order = nullptr, *repo_loff = nullptr;  // the reposition plan
     float *ev_rand = nullptr;    // rand()%101 values, 3 per event
     int32_t *ev_off = nullptr;   // per listed particle: offset into ev_rand
     int32_t *ev_slot = nullptr;  // listed particles' slots in creation order
@@ -192,7 +200,8 @@ void k_launch_gradient(FlipCtx *c);             // ghost-fluid gradient + enforc
 void k_launch_extrapolate(FlipCtx *c);          // extrapolate_velocity + subtract_grid
 void k_launch_g2p(FlipCtx *c, bool fresh = false);  // FLIP/PIC blend; fresh: right after k_launch_extrapolate
 void k_launch_advect(FlipCtx *c);               // advect + clamp + wall push-out
-int k_stuck_and_reposition(FlipCtx *c);         // returns repositioned count (host syncs)
+void k_stuck_and_reposition(FlipCtx *c);        // on the device; counts in ev_count[EV_STUCK..]
+void k_launch_clear_events(FlipCtx *c, int lo, int hi);  // ev_count[lo, hi) = 0 on the stream
 int k_correct(FlipCtx *c);                      // corrector; returns jitter events
 void k_launch_emit(FlipCtx *c);                 // append the emitter's disc (capacity already ensured)
 void ensure_particle_capacity(FlipCtx *c, int64_t need);  // grow, keeping the particles

@@ -812,9 +812,11 @@ __global__ void k_stuck(int64_t np, int n, const float4 *__restrict__ pos, const
     }
 }
 
-// flip3D.cpp:414-421 deep-water cells: FLUID with every in-range neighbour FLUID
-__global__ void k_deep(long C, int n, const int8_t *__restrict__ A, int32_t *flag)
+// flip3D.cpp:414-421 deep-water cells: FLUID with every in-range neighbour
+// FLUID (nothing to do when no particle is stuck: mcount[0] == 0)
+__global__ void k_deep(long C, int n, const int8_t *__restrict__ A, int32_t *flag, const int32_t *__restrict__ mcount)
 {
+    if (mcount[0] == 0) return;
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
@@ -828,26 +830,211 @@ __global__ void k_deep(long C, int n, const int8_t *__restrict__ A, int32_t *fla
     flag[c] = d ? 1 : 0;
 }
 
-__global__ void k_deep_compact(long C, const int32_t *__restrict__ flag, const int32_t *__restrict__ at, int limit,
-                               int32_t *out)
+// the first mcount[0] deep cells in scan order
+__global__ void k_deep_compact(long C, const int32_t *__restrict__ flag, const int32_t *__restrict__ at,
+                               const int32_t *__restrict__ mcount, int32_t *out)
 {
+    const int limit = mcount[0];
+    if (limit == 0) return;
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     if (flag[c] && at[c] < limit) out[at[c]] = (int32_t)c;
 }
 
-// write positions of particles given by creation index
-__global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const float *__restrict__ xyz,
-                                const int32_t *__restrict__ inv, float4 *pos)
+// write the positions of the mcount[0] particles given by creation index
+__global__ void k_set_pos_by_id(const int32_t *__restrict__ mcount, const int32_t *__restrict__ ids,
+                                const float *__restrict__ xyz, const int32_t *__restrict__ inv, float4 *pos)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    int32_t s = inv[ids[t]];
-    float4 p = pos[s];
-    p.x = xyz[3 * t];
-    p.y = xyz[3 * t + 1];
-    p.z = xyz[3 * t + 2];
-    pos[s] = p;
+    const int m = mcount[0];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m; t += gridDim.x * blockDim.x) {
+        int32_t s = inv[ids[t]];
+        float4 p = pos[s];
+        p.x = xyz[3 * t];
+        p.y = xyz[3 * t + 1];
+        p.z = xyz[3 * t + 2];
+        pos[s] = p;
+    }
+}
+
+// one glibc rand() draw from a TYPE_3 state held in s (31 words, indices f
+// and r = f + 28 mod 31)
+__device__ __forceinline__ int rand_step(uint32_t *s, int &f, int &r)
+{
+    const uint32_t v = s[f] + s[r];
+    s[f] = v;
+    f = f == 30 ? 0 : f + 1;
+    r = r == 30 ? 0 : r + 1;
+    return (int)(v >> 1);
+}
+
+// flip3D.cpp:405-437 the reposition plan, in one block: the stuck list in
+// creation order (the reference lists particle indices in order), the deep
+// cells repeated until there are as many as stuck particles (:409-429), the
+// libstdc++ random_shuffle of them with the reference's rand() (:432), three
+// rand() % 101 per particle for its position in its cell (:436-441), all
+// drawn in the reference's order from the device's generator (canonical
+// order at rest: f = 0, r = 28).  Then the dependency levels of the
+// resample (k_resample_levels): level(t) = 1 + the highest level of an
+// earlier listed particle within t's support (a few parallel rounds, then
+// list order for long chains), with the level offsets and the list ordered
+// by level.  ev[EV_MOVED] and ev[EV_NLEV] get the counts
+// (both 0 when nothing is stuck or there is no deep water: the reference
+// then moves nothing).
+constexpr int RP_TB = 1024, RP_LDS = 8192, RP_ROUNDS = 8;
+__global__ __launch_bounds__(RP_TB) void k_repo_plan(int32_t *ev, const int32_t *__restrict__ list,
+                                                     const int32_t *__restrict__ nd_src,
+                                                     const int32_t *__restrict__ deep, int32_t *ids,
+                                                     int32_t *waters, int32_t *rng, float *xyz, int32_t *level,
+                                                     int32_t *loff, int32_t *fill, int32_t *order, int n)
+{
+    __shared__ int32_t sw[RP_LDS];
+    __shared__ uint32_t st[31];
+    __shared__ int s_more, s_max;
+    const int tid = threadIdx.x;
+    const int m = ev[EV_STUCK];
+    const int nd = m > 0 ? nd_src[0] : 0;
+    if (m == 0 || nd == 0) {
+        if (tid == 0) {
+            ev[EV_MOVED] = 0;
+            ev[EV_NLEV] = 0;
+        }
+        return;
+    }
+    const int take = min(nd, m);
+    const bool in_lds = m <= RP_LDS;
+    int32_t *w = in_lds ? sw : waters;
+    for (int t = tid; t < m; t += RP_TB) {
+        const int32_t id = list[t];
+        int rank = 0;
+        for (int u = 0; u < m; ++u) rank += list[u] < id;  // creation indices are unique
+        ids[rank] = id;
+        w[t] = deep[t % take];
+    }
+    __threadfence_block();
+    __syncthreads();
+    const float h = (float)(1.0 / n);
+    if (tid == 0) {
+        for (int j = 0; j < 31; ++j) st[j] = (uint32_t)rng[j];
+        int f = 0, r = 28;
+        // libstdc++ random_shuffle: swap(a[i], a[rand() % (i + 1)]) for i >= 1
+        for (int i = 1; i < m; ++i) {
+            const int j = rand_step(st, f, r) % (i + 1);
+            if (i != j) {
+                const int32_t x = w[i];
+                w[i] = w[j];
+                w[j] = x;
+            }
+        }
+        for (int t = 0; t < m; ++t) {
+            const int c = w[t];
+            const int wi = c / (n * n), wj = (c / n) % n, wk = c % n;
+            const int r0 = rand_step(st, f, r) % 101;
+            xyz[3 * t + 0] = (float)((double)h * (wi + 0.25 + 0.5 * r0 / 100));
+            const int r1 = rand_step(st, f, r) % 101;
+            xyz[3 * t + 1] = (float)((double)h * (wj + 0.25 + 0.5 * r1 / 100));
+            const int r2 = rand_step(st, f, r) % 101;
+            xyz[3 * t + 2] = (float)((double)h * (wk + 0.25 + 0.5 * r2 / 100));
+        }
+        for (int j = 0; j < 31; ++j) rng[j] = (int32_t)st[(f + j) % 31];
+        rng[31] = 0;
+        rng[32] = 28;
+    }
+    for (int t = tid; t < m; t += RP_TB) level[t] = -1;
+    __threadfence_block();
+    __syncthreads();
+    // levels: a few rounds in which every particle whose earlier neighbours
+    // are all placed gets placed (the earliest unplaced one always is), then
+    // whatever is left (long chains) one particle at a time in list order
+    const float hh = h * h;
+    for (int round = 0; round < RP_ROUNDS; ++round) {
+        if (tid == 0) s_more = 0;
+        __syncthreads();
+        for (int t = tid; t < m; t += RP_TB) {
+            if (level[t] >= 0) continue;
+            const float px = xyz[3 * t], py = xyz[3 * t + 1], pz = xyz[3 * t + 2];
+            int mx = -1;
+            bool ready = true;
+            for (int u = 0; u < t && ready; ++u) {
+                const float dx = px - xyz[3 * u], dy = py - xyz[3 * u + 1], dz = pz - xyz[3 * u + 2];
+                if (dx * dx + dy * dy + dz * dz < hh) {
+                    const int lu = __atomic_load_n(&level[u], __ATOMIC_RELAXED);
+                    if (lu < 0) ready = false;
+                    else mx = max(mx, lu);
+                }
+            }
+            if (ready) __atomic_store_n(&level[t], mx + 1, __ATOMIC_RELAXED);
+            else s_more = 1;
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (!s_more) break;
+    }
+    if (s_more) {
+        for (int t = 0; t < m; ++t) {
+            if (level[t] >= 0) continue;  // the same for every thread after the last barrier
+            const float px = xyz[3 * t], py = xyz[3 * t + 1], pz = xyz[3 * t + 2];
+            int mx = -1;
+            for (int u = tid; u < t; u += RP_TB) {
+                const float dx = px - xyz[3 * u], dy = py - xyz[3 * u + 1], dz = pz - xyz[3 * u + 2];
+                if (dx * dx + dy * dy + dz * dz < hh) mx = max(mx, level[u]);
+            }
+            if (tid == 0) s_max = -1;
+            __syncthreads();
+            atomicMax(&s_max, mx);
+            __syncthreads();
+            if (tid == 0) level[t] = s_max + 1;
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+    if (tid == 0) s_max = 0;
+    __syncthreads();
+    for (int t = tid; t < m; t += RP_TB) atomicMax(&s_max, level[t]);
+    __syncthreads();
+    const int nlev = s_max + 1;
+    // level offsets: counts, then an exclusive scan over nlev + 1 entries
+    // (chunk per thread, the chunk sums scanned in LDS)
+    for (int l = tid; l <= nlev; l += RP_TB) loff[l] = 0;
+    __threadfence_block();
+    __syncthreads();
+    for (int t = tid; t < m; t += RP_TB) atomicAdd(&loff[level[t]], 1);
+    __threadfence_block();
+    __syncthreads();
+    const int chunk = (nlev + 1 + RP_TB - 1) / RP_TB, l0 = tid * chunk, l1 = min(l0 + chunk, nlev + 1);
+    int csum = 0;
+    for (int l = l0; l < l1; ++l) csum += loff[l];
+    sw[tid] = csum;  // the shuffle's LDS copy is no longer needed
+    __syncthreads();
+    for (int d = 1; d < RP_TB; d <<= 1) {
+        const int v = tid >= d ? sw[tid - d] : 0;
+        __syncthreads();
+        sw[tid] += v;
+        __syncthreads();
+    }
+    int run = sw[tid] - csum;
+    for (int l = l0; l < l1; ++l) {
+        const int cnt = loff[l];
+        loff[l] = run;
+        if (l < nlev) fill[l] = run;
+        run += cnt;
+    }
+    __threadfence_block();
+    __syncthreads();
+    // each particle at its level's next slot: the order inside a level is
+    // free, its particles never read each other
+    for (int t = tid; t < m; t += RP_TB) order[atomicAdd(&fill[level[t]], 1)] = t;
+    if (tid == 0) {
+        ev[EV_MOVED] = m;
+        ev[EV_NLEV] = nlev;
+    }
+}
+
+// zero slots [lo, hi) of the step's event counts (a kernel, so that steps
+// can be graph-captured)
+__global__ void k_clear_events(int32_t *ev, int lo, int hi)
+{
+    const int t = lo + (int)threadIdx.x;
+    if (t < hi) ev[t] = 0;
 }
 
 // flip3D.cpp:445-450: resample each repositioned particle in list order with
@@ -860,7 +1047,7 @@ __global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const fl
 // new positions).  Particles of one level never read each other, and every
 // particle they read from the list is on a lower level (earlier in the list,
 // new value) or a higher one (later in the list, old value), exactly as in
-// the serial loop.  One block walks the levels; within a level each wave
+// the serial loop (k_repo_plan forms the levels).  One block walks the levels; within a level each wave
 // resamples one particle: its lanes form the window's terms (w u, w) in
 // parallel into LDS, in window order, and lane 0 adds them up in that order,
 // so the sums are the reference's serial ones (a skipped particle stores +0
@@ -868,7 +1055,8 @@ __global__ void k_set_pos_by_id(int m, const int32_t *__restrict__ ids, const fl
 // -0).
 constexpr int RL_WAVES = 16, RL_CAP = 384;
 
-__global__ __launch_bounds__(64 * RL_WAVES) void k_resample_levels(int nlev, const int32_t *__restrict__ loff,
+__global__ __launch_bounds__(64 * RL_WAVES) void k_resample_levels(const int32_t *__restrict__ nlev_p,
+                                                                   const int32_t *__restrict__ loff,
                                                                    const int32_t *__restrict__ order,
                                                                    const int32_t *__restrict__ ids,
                                                                    const int32_t *__restrict__ inv, int n,
@@ -877,6 +1065,7 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_resample_levels(int nlev, con
 {
     __shared__ float4 terms[RL_WAVES][RL_CAP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nlev = nlev_p[0];
     for (int l = 0; l < nlev; ++l) {
         for (int x = loff[l] + wave; x < loff[l + 1]; x += RL_WAVES) {
             const int32_t slot = inv[ids[order[x]]];
@@ -1816,7 +2005,7 @@ void k_launch_advect(FlipCtx *c)
     if (c->np > 0)
         launch_timed(c, "advect", k_advect, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0], c->ug[1],
                      c->ug[2], c->dtv, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n, c->wall_start,
-                     c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE);
+                     c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE_ADV);
 }
 
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s)
@@ -1874,6 +2063,11 @@ void rng_to_host(FlipCtx *c)
     c->rng_on_device = false;
 }
 
+void k_launch_clear_events(FlipCtx *c, int lo, int hi)
+{
+    hipLaunchKernelGGL(k_clear_events, 1, 64, 0, c->stream, c->ev_count, lo, hi);
+}
+
 // the step's time step into c->dtv[0] on the device (no host round trip)
 void k_launch_choose_dt(FlipCtx *c)
 {
@@ -1891,119 +2085,45 @@ void k_launch_set_dt(FlipCtx *c, double dt)
     hipLaunchKernelGGL(k_choose_dt, 1, TB, 0, c->stream, 0, c->partials, dt, 0.0, c->n, c->dtv);
 }
 
-// Stuck detection and repositioning (flip3D.cpp:932-968, 405-451).  The
-// device flags and lists; the host sorts the list, shuffles the deep-water
-// cells and draws positions with the reference's rand() order; the device
-// re-sorts and resamples.  Returns the number of particles moved.  Always
-// leaves the particles sorted on their current positions.
-int k_stuck_and_reposition(FlipCtx *c)
+// Stuck detection and repositioning (flip3D.cpp:932-968, 405-451), all on
+// the device with no host round trip: the stuck list, the deep-water cells
+// (k_deep, a scan, the first ones compacted), the plan (k_repo_plan: the
+// list in creation order, the shuffle and positions drawn with the
+// reference's rand() sequence, the resample levels), the new positions, the
+// sort and the resample.  Kernels after the stuck test return at once when
+// nothing is stuck.  The counts stay in ev_count[EV_STUCK..EV_NLEV]; the
+// particles are left sorted on their current positions.
+void k_stuck_and_reposition(FlipCtx *c)
 {
     int n = c->n;
-    int m = 0;
-    std::vector<int32_t> ids;
+    hipLaunchKernelGGL(k_clear_events, 1, 64, 0, c->stream, c->ev_count, EV_STUCK, EV_NLEV + 1);
     if (c->prm.reposition && c->np > 0) {
         t_begin(c, "stuck");
-        FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, sizeof(int32_t), c->stream));
         hipLaunchKernelGGL(k_stuck, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->V4(), c->M(), c->A,
-                           c->ev_list, c->ev_count);
+                           c->ev_list, c->ev_count + EV_STUCK);
         t_end(c);
-        FLIP_CHECK(hipMemcpyAsync(c->h_pinned, c->ev_count, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        m = c->h_pinned[0];
-        if (c->h_pinned[EV_NONFINITE])
-            throw flip_arg_error{FLIP_E_NUMERIC, "non-finite particle position or velocity in the advection"};
-    }
-    int moved = 0;
-    if (m > 0) {
-        ids.resize(m);
-        FLIP_CHECK(hipMemcpyAsync(ids.data(), c->ev_list, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        std::sort(ids.begin(), ids.end());
-        // deep-water cells in scan order (flip3D.cpp:411-430)
-        hipLaunchKernelGGL(k_deep, nblk(c->C, TB), TB, 0, c->stream, c->C, n, c->A, c->cell_count);
+        t_begin(c, "reposition");
+        hipLaunchKernelGGL(k_deep, nblk(c->C, TB), TB, 0, c->stream, c->C, n, c->A, c->cell_count,
+                           c->ev_count + EV_STUCK);
         k_launch_scan(c, c->cell_count, c->scan_tmp, c->C);
-        hipLaunchKernelGGL(k_deep_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->cell_count, c->scan_tmp, m,
-                           c->ev_off);
-        int32_t nd = 0;
-        FLIP_CHECK(hipMemcpyAsync(&nd, c->scan_tmp + c->C, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        FLIP_CHECK(hipStreamSynchronize(c->stream));
-        if (nd > 0) {
-            rng_to_host(c);
-            int take = std::min(nd, m);
-            std::vector<int32_t> deep(take), waters(m);
-            FLIP_CHECK(hipMemcpyAsync(deep.data(), c->ev_off, sizeof(int32_t) * take, hipMemcpyDeviceToHost, c->stream));
-            FLIP_CHECK(hipStreamSynchronize(c->stream));
-            for (int t = 0; t < m; ++t) waters[t] = deep[t % take];
-            // libstdc++ random_shuffle: swap(a[i], a[rand() % (i+1)]) for i >= 1
-            for (int i = 1; i < m; ++i) {
-                int j = c->rng.next() % (i + 1);
-                if (i != j) std::swap(waters[i], waters[j]);
-            }
-            float h = (float)(1.0 / n);
-            std::vector<float> xyz(3 * (size_t)m);
-            for (int t = 0; t < m; ++t) {
-                int wi = waters[t] / (n * n), wj = (waters[t] / n) % n, wk = waters[t] % n;
-                int r0 = c->rng.next() % 101;
-                xyz[3 * t + 0] = (float)((double)h * (wi + 0.25 + 0.5 * r0 / 100));
-                int r1 = c->rng.next() % 101;
-                xyz[3 * t + 1] = (float)((double)h * (wj + 0.25 + 0.5 * r1 / 100));
-                int r2 = c->rng.next() % 101;
-                xyz[3 * t + 2] = (float)((double)h * (wk + 0.25 + 0.5 * r2 / 100));
-            }
-            FLIP_CHECK(hipMemcpyAsync(c->ev_list, ids.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
-            FLIP_CHECK(hipMemcpyAsync(c->ev_rand, xyz.data(), sizeof(float) * 3 * m, hipMemcpyHostToDevice,
-                                      c->stream));
-            hipLaunchKernelGGL(k_set_pos_by_id, nblk(m, TB), TB, 0, c->stream, m, c->ev_list, c->ev_rand, c->inv,
-                               c->P4());
-            // dependency levels of the list (see k_resample_levels)
-            const float hh = h * h;
-            std::vector<int32_t> lev(m, 0);
-            std::unordered_map<long, std::vector<int32_t>> bucket;
-            int nlev = 0;
-            for (int t = 0; t < m; ++t) {
-                const float *pt = &xyz[3 * t];
-                const int bi = (int)(pt[0] * (float)n), bj = (int)(pt[1] * (float)n), bk = (int)(pt[2] * (float)n);
-                int l = 0;
-                for (int di = -1; di <= 1; ++di)
-                    for (int dj = -1; dj <= 1; ++dj)
-                        for (int dk = -1; dk <= 1; ++dk) {
-                            auto it = bucket.find(((long)(bi + di) * (n + 2) + (bj + dj)) * (n + 2) + (bk + dk));
-                            if (it == bucket.end()) continue;
-                            for (int32_t u : it->second) {
-                                const float *pu = &xyz[3 * u];
-                                const float dx = pt[0] - pu[0], dy = pt[1] - pu[1], dz = pt[2] - pu[2];
-                                const float d2 = dx * dx + dy * dy + dz * dz;
-                                if (d2 < hh) l = std::max(l, lev[u] + 1);
-                            }
-                        }
-                lev[t] = l;
-                nlev = std::max(nlev, l + 1);
-                bucket[((long)bi * (n + 2) + bj) * (n + 2) + bk].push_back(t);
-            }
-            std::vector<int32_t> loff(nlev + 1, 0), order(m);
-            for (int t = 0; t < m; ++t) loff[lev[t] + 1]++;
-            for (int l = 0; l < nlev; ++l) loff[l + 1] += loff[l];
-            {
-                std::vector<int32_t> fill(loff.begin(), loff.end() - 1);
-                for (int t = 0; t < m; ++t) order[fill[lev[t]]++] = t;
-            }
-            k_launch_sort(c);
-            t_begin(c, "reposition_resample");
-            // ev_off (free again after the deep-cell list was read) holds the
-            // level offsets, deep (the sort's scratch) the order
-            FLIP_CHECK(hipMemcpyAsync(c->ev_off, loff.data(), sizeof(int32_t) * (nlev + 1), hipMemcpyHostToDevice,
-                                      c->stream));
-            FLIP_CHECK(hipMemcpyAsync(c->deep, order.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL(k_resample_levels, 1, 64 * RL_WAVES, 0, c->stream, nlev, c->ev_off, c->deep, c->ev_list, c->inv,
-                               n, c->P4(), c->V4(), c->cell_start, hh);
-            t_end(c);
-            FLIP_CHECK(hipStreamSynchronize(c->stream));
-            moved = m;
-            return moved;
-        }
+        hipLaunchKernelGGL(k_deep_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->cell_count, c->scan_tmp,
+                           c->ev_count + EV_STUCK, c->ev_off);
+        rng_to_device(c);
+        hipLaunchKernelGGL(k_repo_plan, 1, RP_TB, 0, c->stream, c->ev_count, c->ev_list, c->scan_tmp + c->C,
+                           c->ev_off, c->repo_ids, c->deep, c->rng_dev, c->ev_rand, c->pkey, c->repo_loff, c->pslot,
+                           c->repo_order, n);
+        hipLaunchKernelGGL(k_set_pos_by_id, 64, TB, 0, c->stream, c->ev_count + EV_MOVED, c->repo_ids, c->ev_rand,
+                           c->inv, c->P4());
+        t_end(c);
     }
     k_launch_sort(c);
-    return moved;
+    if (c->prm.reposition && c->np > 0) {
+        t_begin(c, "reposition_resample");
+        const float h = (float)(1.0 / n);
+        hipLaunchKernelGGL(k_resample_levels, 1, 64 * RL_WAVES, 0, c->stream, c->ev_count + EV_NLEV, c->repo_loff,
+                           c->repo_order, c->repo_ids, c->inv, n, c->P4(), c->V4(), c->cell_start, h * h);
+        t_end(c);
+    }
 }
 
 // corrector::correct (corrector.cpp:49-115); particles must be sorted.
@@ -2014,9 +2134,9 @@ int k_correct(FlipCtx *c)
     if (!c->sorted) throw flip_arg_error{FLIP_E_STATE, "the corrector needs sorted particles"};
     int n = c->n;
     float re = (float)(c->prm.density / n);
-    // ev_count[0]: listed particles, [1]: draw overflow, [3]: events (read
-    // with the step's end); [2] (the advection's check) is kept
-    FLIP_CHECK(hipMemsetAsync(c->ev_count, 0, 2 * sizeof(int32_t), c->stream));
+    // ev_count[0]: listed particles, [1]: draw overflow, [EV_NONFINITE]: a
+    // non-finite update, [3]: events (read with the step's end)
+    hipLaunchKernelGGL(k_clear_events, 1, 64, 0, c->stream, c->ev_count, 0, 4);
     launch_timed(c, "spring", k_spring, nblk(c->np, SPRING_TB), SPRING_TB, c->np, n, c->P4(), c->M(), c->nrm_id,
                  c->cell_start, c->dtv, re, c->tmp0, c->ev_list, c->ev_count);
     // the jitter events in creation order with the reference's rand() draws,

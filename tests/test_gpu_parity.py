@@ -210,6 +210,48 @@ def test_stage_parity_first_step(n):
         s.close()
 
 
+@pytest.mark.parametrize("n,deep", [(32, "scene"), (64, "scene"), (32, "one cell")])
+def test_reposition_many_stuck(n, deep):
+    """flip3D.cpp:932-968 and 405-451 with thousands of stuck particles: the
+    fluid cells of the layer above the floor are flagged WALL, so every
+    particle in them is stuck (about 3,300 at N=32 and 12,000 at N=64, past
+    the plan kernel's LDS shuffle).  With "one cell" the flags leave a
+    single deep-water cell: the deep list repeats, every stuck particle lands
+    in that cell, and the resample levels form one chain as long as the
+    list.  The device plan (creation order, shuffle, positions, levels) must
+    give the oracle's positions and velocities bit for bit."""
+    g, s = _pair(n=n)
+    try:
+        g.stage(fa.STAGE_DENSITY)
+        s.call("sort")
+        s.call("compute_density")
+        g.stage(fa.STAGE_EXT_FORCE)
+        s.call("add_ext_force")
+        g.stage(fa.STAGE_P2G)
+        for nm in ("sort", "map_p2g", "mark_water", "save_grid", "enforce_boundary"):
+            s.call(nm)
+        A = s.grid(po.G_A).copy()
+        layer = A[:, 1, :]
+        layer[layer == po.FLUID] = po.WALL
+        if deep == "one cell":
+            A[A == po.FLUID] = po.AIR
+            c = n // 2
+            A[c - 1:c + 2, c - 1:c + 2, c - 1:c + 2] = po.FLUID
+        s.set_grid(po.G_A, A)
+        g.set_grid(fa.GRID_A, A)
+        gst = g.stage(fa.STAGE_ADVECT)
+        sst = s.call_stats("advect")
+        assert sst["n_reposition"] > (8192 if n == 64 else 1000), sst["n_reposition"]
+        assert gst["n_reposition"] == sst["n_reposition"]
+        a, b = g.download(), s.particles()
+        assert np.array_equal(a["p"], b["p"])
+        fl = b["type"] == po.FLUID
+        assert np.array_equal(a["u"][fl], b["u"][fl])
+    finally:
+        g.close()
+        s.close()
+
+
 # step policies: the reference's (fixed dt 6e-3, absolute stop sum r^2 <= 1e-2 n^3)
 # and the benchmark configuration's (CFL dt, relative residual 1e-5)
 POLICIES = {"reference": {}, "config": dict(dt_policy=1, tol_policy=1, tol=1e-5)}
