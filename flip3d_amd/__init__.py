@@ -34,7 +34,7 @@ EXPORTS = [
     "flip_gpu_set_emitter", "flip_gpu_set_step", "flip_gpu_get_step", "flip_gpu_mesh", "flip_gpu_mesh_get",
     "flip_gpu_render_preview", "flip_gpu_write_preview", "flip_gpu_write_obj", "flip_gpu_write_sunflow",
     "flip_gpu_step_begin",
-    "flip_gpu_set_learned_weights", "flip_gpu_set_launch", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
+    "flip_gpu_set_learned_weights", "flip_gpu_set_launch", "flip_gpu_set_step_graphs", "flip_unet_create", "flip_unet_destroy", "flip_unet_num_params", "flip_unet_get_params",
     "flip_unet_set_params", "flip_unet_forward_features", "flip_unet_backward", "flip_unet_train_step",
     "flip_unet_predict", "flip_unet_predict_fused", "flip_unet_solve_system", "flip_unet_save", "flip_unet_load",
 ]
@@ -119,6 +119,7 @@ def lib():
         L.flip_gpu_load_state.argtypes = [P, C.c_char_p]
         L.flip_gpu_set_learned_weights.argtypes = [P, C.c_char_p]
         L.flip_gpu_set_launch.argtypes = [P, C.c_int, C.c_int]
+        L.flip_gpu_set_step_graphs.argtypes = [P, C.c_int]
         L.flip_unet_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]
         L.flip_unet_destroy.argtypes = [P]
         L.flip_unet_num_params.argtypes = [P]
@@ -367,6 +368,11 @@ class FlipContext:
         """Launch shape of the solver's FLUID-list kernels (0 = built in):
         their block count and the PCG iterations per captured batch."""
         _check(lib().flip_gpu_set_launch(self.h, int(pcg_blocks), int(pcg_batch)))
+
+    def set_step_graphs(self, on=True):
+        """Replay the step's kernels around the pressure solve from captured
+        HIP graphs (the default) or launch them one by one."""
+        _check(lib().flip_gpu_set_step_graphs(self.h, 1 if on else 0))
 
     def save_state(self, path):
         """Checkpoint in the reference's render/save/<step>_scene.dat layout."""

@@ -228,6 +228,9 @@ void alloc_grids(FlipCtx *c)
     rng_upload_table(c);
     FLIP_CHECK(hipHostMalloc((void **)&c->h_rng, 64 * sizeof(int32_t)));
     FLIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(int32_t)));
+    FLIP_CHECK(hipHostMalloc((void **)&c->h_rec, sizeof(FlipCtx::StepRecord), hipHostMallocMapped));
+    FLIP_CHECK(hipHostGetDevicePointer((void **)&c->d_rec, c->h_rec, 0));
+    memset(c->h_rec, 0, sizeof(FlipCtx::StepRecord));
     FLIP_CHECK(hipHostMalloc((void **)&c->h_status, sizeof(FlipCtx::SolveStatus), hipHostMallocMapped));
     FLIP_CHECK(hipHostGetDevicePointer((void **)&c->d_status, c->h_status, 0));
     memset(c->h_status, 0, sizeof(FlipCtx::SolveStatus));
@@ -292,6 +295,11 @@ void free_all(FlipCtx *c)
     c->h_pinned = nullptr;
     if (c->h_status) (void)hipHostFree(c->h_status);
     c->h_status = c->d_status = nullptr;
+    for (auto &g : c->step_gx)
+        if (g) (void)hipGraphExecDestroy(g);
+    c->step_gx[0] = c->step_gx[1] = nullptr;
+    if (c->h_rec) (void)hipHostFree(c->h_rec);
+    c->h_rec = c->d_rec = nullptr;
     for (auto &e : c->ev_batch)
         if (e) (void)hipEventDestroy(e);
     for (auto e : c->tev) (void)hipEventDestroy(e);
@@ -918,6 +926,40 @@ int flip_gpu_write_sunflow(flip_ctx *ctx, const char *path, const char *template
     });
 }
 
+// What the captured step parts bake into their launches: the particle count,
+// the parity of the double-buffered storage, the host scalars and parameters
+// the kernels take, and every device buffer that can be reallocated (particle
+// storage, its scratch, the wall bins).  A change means a new capture.
+static std::vector<uint64_t> step_graph_sig(const FlipCtx *c)
+{
+    std::vector<uint64_t> s;
+    auto ptr = [&](const void *p) { s.push_back((uint64_t)(uintptr_t)p); };
+    s.push_back((uint64_t)c->np);
+    s.push_back((uint64_t)c->cur);
+    s.push_back(c->unit_mass ? 1u : 0u);
+    uint32_t md;
+    memcpy(&md, &c->max_dens, sizeof(md));
+    s.push_back(md);
+    s.push_back((uint64_t)c->ev_cap);
+    const unsigned char *pb = reinterpret_cast<const unsigned char *>(&c->prm);
+    for (size_t i = 0; i < sizeof(c->prm); i += 8) {
+        uint64_t w = 0;
+        memcpy(&w, pb + i, std::min<size_t>(8, sizeof(c->prm) - i));
+        s.push_back(w);
+    }
+    for (int b = 0; b < 2; ++b) {
+        ptr(c->pos[b]);
+        ptr(c->vel[b]);
+        ptr(c->meta[b]);
+    }
+    const void *bufs[] = {c->nrm_id,   c->inv,       c->spos,      c->xyz3,     c->tmp0,     c->tmp1,
+                          c->pkey,     c->pslot,     c->perm,      c->sort_big, c->deep,     c->ev_list,
+                          c->ev_off,   c->ev_slot,   c->repo_ids,  c->repo_order, c->repo_loff, c->ev_rand,
+                          c->wall_start, c->wall_pos, c->wall_nrm};
+    for (const void *b : bufs) ptr(b);
+    return s;
+}
+
 int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
 {
     return guarded([&]() -> int {
@@ -934,40 +976,90 @@ int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
         }
         if (!c->begun) begin_step(c);
         c->begun = false;
+        const bool emitting = c->step <= c->pour_time && c->emit_n > 0;
+        const bool graphs = c->step_graphs && !c->profiling && !c->direct_launch && !emitting &&
+                            c->prm.refine == 0 && !c->prm.solve_fp64;
+        if (graphs) {
+            std::vector<uint64_t> sig = step_graph_sig(c);
+            if (sig != c->step_graph_sig) {
+                for (auto &g : c->step_gx)
+                    if (g) FLIP_CHECK(hipGraphExecDestroy(g));
+                c->step_gx[0] = c->step_gx[1] = nullptr;
+                c->step_graph_sig = std::move(sig);
+            }
+            rng_to_device(c);  // no host copy inside a captured part
+        }
+        // a part of the step: replayed from its graph, captured on first use
+        auto part = [&](int which, auto &&launch) {
+            if (!graphs) {
+                launch();
+                return;
+            }
+            if (!c->step_gx[which]) {
+                hipGraph_t g;
+                FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                launch();
+                FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
+                FLIP_CHECK(hipGraphInstantiate(&c->step_gx[which], g, nullptr, nullptr, 0));
+                FLIP_CHECK(hipGraphDestroy(g));
+            }
+            FLIP_CHECK(hipGraphLaunch(c->step_gx[which], c->stream));
+        };
         FLIP_CHECK(hipEventRecord(c->ev_a, c->stream));
-        k_launch_choose_dt(c);  // dt stays on the device; read back with the step's end
-        // flip3D.cpp:1221-1222 sort + computeDensity
-        k_launch_sort(c);
-        c->sorted = true;
-        k_launch_density(c, true);  // with add_ExtForce (:1230) fused in
-        // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
-        k_launch_flags(c);
-        k_launch_p2g(c);
-        run_stage(c, FLIP_STAGE_PROJECT, st);
-        k_launch_extrapolate(c);
-        k_launch_g2p(c, true);  // reads the extrapolation's (u, u - u_save) pairs
-        run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
-        if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
-        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 8, c->ev_count, EV_SLOTS * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                  c->stream));
-        FLIP_CHECK(hipMemcpyAsync(c->h_pinned + 16, c->dtv, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        part(0, [&]() {
+            k_launch_choose_dt(c);  // dt stays on the device; read back with the step's end
+            // flip3D.cpp:1221-1222 sort + computeDensity
+            k_launch_sort(c);
+            c->sorted = true;
+            k_launch_density(c, true);  // with add_ExtForce (:1230) fused in
+            // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
+            k_launch_flags(c);
+            k_launch_p2g(c);
+            k_launch_divergence(c);  // project (:807-868): div, solve, gradient
+        });
+        {
+            if (!c->ev_s0) {
+                FLIP_CHECK(hipEventCreate(&c->ev_s0));
+                FLIP_CHECK(hipEventCreate(&c->ev_s1));
+            }
+            FLIP_CHECK(hipEventRecord(c->ev_s0, c->stream));
+            const int rc = k_solve(c, st);
+            FLIP_CHECK(hipEventRecord(c->ev_s1, c->stream));
+            c->solve_timed = true;
+            if (rc == FLIP_E_NUMERIC) throw flip_arg_error{FLIP_E_NUMERIC, "pressure solve produced NaN"};
+        }
+        part(1, [&]() {
+            k_launch_gradient(c);
+            k_launch_extrapolate(c);
+            k_launch_g2p(c, true);  // reads the extrapolation's (u, u - u_save) pairs
+            run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
+            if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
+            k_launch_step_record(c);
+        });
+        // the host's view of the storage after the step (what the launches
+        // set when they are captured; a replay does not run them)
+        c->bucketed = true;
+        c->sorted = !c->prm.correction;
+        c->advected = true;
+        c->corrected = c->prm.correction != 0;
         FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));
         FLIP_CHECK(hipEventSynchronize(c->ev_b));
         FLIP_CHECK(hipGetLastError());
-        memcpy(&c->dt, c->h_pinned + 16, sizeof(double));
+        const FlipCtx::StepRecord &rec = *c->h_rec;
+        c->dt = rec.dt;
         st->dt = c->dt;
         st->ms_step = elapsed(c->ev_a, c->ev_b);
         if (c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
         c->solve_timed = false;
         c->advected = false;
-        st->n_reposition = c->h_pinned[8 + EV_MOVED];
-        if (c->h_pinned[8 + EV_NONFINITE_ADV])
+        st->n_reposition = rec.ev[EV_MOVED];
+        if (rec.ev[EV_NONFINITE_ADV])
             return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity in the advection");
         if (c->corrected) {
-            st->n_jitter = c->h_pinned[8 + 3];
+            st->n_jitter = rec.ev[3];
             c->corrected = false;
-            if (c->h_pinned[8 + 1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
-            if (c->h_pinned[8 + EV_NONFINITE])
+            if (rec.ev[1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
+            if (rec.ev[EV_NONFINITE])
                 return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
         }
         timing_collect(c);
@@ -1154,6 +1246,16 @@ int flip_gpu_set_launch(flip_ctx *ctx, int pcg_blocks, int pcg_batch)
         FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
         c->tune_blocks = pcg_blocks;
         c->tune_batch = pcg_batch;
+        return FLIP_OK;
+    });
+}
+
+int flip_gpu_set_step_graphs(flip_ctx *ctx, int on)
+{
+    return guarded([&]() -> int {
+        require(ctx, "null context");
+        FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
+        c->step_graphs = on != 0;
         return FLIP_OK;
     });
 }

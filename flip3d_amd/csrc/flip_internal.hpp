@@ -149,6 +149,20 @@ struct FlipCtx {
     bool rng_on_device = false;
     int64_t ev_cap = 0;
     int32_t *h_pinned = nullptr;  // pinned host scalars
+    // Step graphs: the step's kernels before the pressure solve (time step
+    // through divergence) and after it (gradient through the corrector and
+    // the step record) captured once per particle count and buffer parity
+    // and replayed; the solve between them keeps its own captured iterations.
+    bool step_graphs = true;
+    hipGraphExec_t step_gx[2] = {nullptr, nullptr};
+    std::vector<uint64_t> step_graph_sig;  // the pointers and scalars the captured launches bake in
+    // what the step's end reads, written by the device into mapped host
+    // memory (k_step_record): the event counts and the time step
+    struct StepRecord {
+        int32_t ev[EV_SLOTS];
+        double dt;
+    };
+    StepRecord *h_rec = nullptr, *d_rec = nullptr;
 
     std::string learned_weights;  // U-Net weights file for FLIP_PRECOND_LEARNED
     // the scene's object list as saveState writes it to objects.dat
@@ -202,6 +216,7 @@ void k_launch_g2p(FlipCtx *c, bool fresh = false);  // FLIP/PIC blend; fresh: ri
 void k_launch_advect(FlipCtx *c);               // advect + clamp + wall push-out
 void k_stuck_and_reposition(FlipCtx *c);        // on the device; counts in ev_count[EV_STUCK..]
 void k_launch_clear_events(FlipCtx *c, int lo, int hi);  // ev_count[lo, hi) = 0 on the stream
+void k_launch_step_record(FlipCtx *c);  // the event counts and the time step into c->d_rec (mapped host memory)
 int k_correct(FlipCtx *c);                      // corrector; returns jitter events
 void k_launch_emit(FlipCtx *c);                 // append the emitter's disc (capacity already ensured)
 void ensure_particle_capacity(FlipCtx *c, int64_t need);  // grow, keeping the particles

@@ -2068,6 +2068,19 @@ void k_launch_clear_events(FlipCtx *c, int lo, int hi)
     hipLaunchKernelGGL(k_clear_events, 1, 64, 0, c->stream, c->ev_count, lo, hi);
 }
 
+__global__ void k_step_record(const int32_t *__restrict__ ev, const double *__restrict__ dtv,
+                              FlipCtx::StepRecord *rec)
+{
+    const int t = threadIdx.x;
+    if (t < EV_SLOTS) rec->ev[t] = ev[t];
+    if (t == 0) rec->dt = dtv[0];
+}
+
+void k_launch_step_record(FlipCtx *c)
+{
+    hipLaunchKernelGGL(k_step_record, 1, 64, 0, c->stream, c->ev_count, c->dtv, c->d_rec);
+}
+
 // the step's time step into c->dtv[0] on the device (no host round trip)
 void k_launch_choose_dt(FlipCtx *c)
 {

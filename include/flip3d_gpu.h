@@ -290,6 +290,16 @@ int flip_gpu_set_learned_weights(flip_ctx *ctx, const char *path);
  * flip3d_amd/tuning.json.  No counterpart in the reference. */
 int flip_gpu_set_launch(flip_ctx *ctx, int pcg_blocks, int pcg_batch);
 
+/* Step graphs (on by default): flip_gpu_step replays the kernels before the
+ * pressure solve (time step through divergence) and after it (gradient
+ * through the corrector) from HIP graphs captured on first use, and
+ * captures them again when the particle storage, the wall bins or a kernel
+ * parameter change; the solve between them keeps its own captured
+ * iterations.  Steps of the pourWater emitter, profiled steps and contexts
+ * created with FLIP_DIRECT_LAUNCH=1 launch directly.  Results do not depend
+ * on the setting.  No counterpart in the reference. */
+int flip_gpu_set_step_graphs(flip_ctx *ctx, int on);
+
 /* Ensembles of independent scenes.  Steps every context of ctxs once (each
  * exactly as flip_gpu_step would), with `threads` host threads driving the
  * contexts concurrently, each context on its own stream; returns when all
