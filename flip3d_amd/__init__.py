@@ -30,7 +30,7 @@ EXPORTS = [
     "flip_gpu_map_p2g", "flip_gpu_map_g2p", "flip_gpu_solve", "flip_gpu_sorted_cells",
     "flip_gpu_sort_cells_noalloc",
     "flip_gpu_kernel_times", "flip_gpu_set_profiling", "flip_gpu_build_scene", "flip_gpu_selftest_math",
-    "flip_gpu_save_state", "flip_gpu_load_state", "flip_gpu_step_many",
+    "flip_gpu_save_state", "flip_gpu_load_state", "flip_gpu_step_many", "flip_gpu_step_batch",
     "flip_gpu_set_emitter", "flip_gpu_set_step", "flip_gpu_get_step", "flip_gpu_mesh", "flip_gpu_mesh_get",
     "flip_gpu_render_preview", "flip_gpu_write_preview", "flip_gpu_write_obj", "flip_gpu_write_sunflow",
     "flip_gpu_step_begin",
@@ -106,6 +106,7 @@ def lib():
         L.flip_gpu_step.argtypes = [P, C.POINTER(Stats)]
         L.flip_gpu_stage.argtypes = [P, C.c_int, C.POINTER(Stats)]
         L.flip_gpu_step_many.argtypes = [C.POINTER(P), C.c_int, C.c_int, C.POINTER(Stats), C.POINTER(C.c_double)]
+        L.flip_gpu_step_batch.argtypes = [C.POINTER(P), C.c_int, C.c_int, C.POINTER(Stats), C.POINTER(C.c_double)]
         L.flip_gpu_map_p2g.argtypes = [P, P, P, P]
         L.flip_gpu_map_g2p.argtypes = [P, P, P, P, C.c_int64, P, P]
         L.flip_gpu_solve.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(Stats)]
@@ -391,6 +392,19 @@ def step_many(contexts, threads=8):
     st = (Stats * n)()
     ms = (C.c_double * n)()
     _check(lib().flip_gpu_step_many(arr, n, threads, st, ms))
+    return [st[i].as_dict() for i in range(n)], [ms[i] for i in range(n)]
+
+
+def step_batch(contexts, threads=8):
+    """One step of every FlipContext in `contexts` with the kernels around
+    the pressure solves of all of them in one graph launch each
+    (flip_gpu_step_batch); returns (stats dicts, per-context ms = the
+    batch's wall clock)."""
+    n = len(contexts)
+    arr = (C.c_void_p * n)(*[c.h for c in contexts])
+    st = (Stats * n)()
+    ms = (C.c_double * n)()
+    _check(lib().flip_gpu_step_batch(arr, n, threads, st, ms))
     return [st[i].as_dict() for i in range(n)], [ms[i] for i in range(n)]
 
 

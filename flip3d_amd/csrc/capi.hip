@@ -295,9 +295,12 @@ void free_all(FlipCtx *c)
     c->h_pinned = nullptr;
     if (c->h_status) (void)hipHostFree(c->h_status);
     c->h_status = c->d_status = nullptr;
-    for (auto &g : c->step_gx)
-        if (g) (void)hipGraphExecDestroy(g);
-    c->step_gx[0] = c->step_gx[1] = nullptr;
+    for (int w = 0; w < 2; ++w) {
+        if (c->step_gx[w]) (void)hipGraphExecDestroy(c->step_gx[w]);
+        if (c->step_g[w]) (void)hipGraphDestroy(c->step_g[w]);
+        c->step_gx[w] = nullptr;
+        c->step_g[w] = nullptr;
+    }
     if (c->h_rec) (void)hipHostFree(c->h_rec);
     c->h_rec = c->d_rec = nullptr;
     for (auto &e : c->ev_batch)
@@ -960,110 +963,284 @@ static std::vector<uint64_t> step_graph_sig(const FlipCtx *c)
     return s;
 }
 
+// ---- the step in parts (flip_gpu_step, flip_gpu_step_batch)
+// flip3D.cpp:1194-1264 before the pressure solve
+static void step_launch_pre(FlipCtx *c)
+{
+    k_launch_choose_dt(c);  // dt stays on the device; read back with the step's end
+    // flip3D.cpp:1221-1222 sort + computeDensity
+    k_launch_sort(c);
+    c->sorted = true;
+    k_launch_density(c, true);  // with add_ExtForce (:1230) fused in
+    // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
+    k_launch_flags(c);
+    k_launch_p2g(c);
+    k_launch_divergence(c);  // project (:807-868): div, solve, gradient
+}
+
+// and after it, ending with the step record in mapped host memory
+static void step_launch_post(FlipCtx *c, flip_step_stats *st)
+{
+    k_launch_gradient(c);
+    k_launch_extrapolate(c);
+    k_launch_g2p(c, true);  // reads the extrapolation's (u, u - u_save) pairs
+    run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
+    if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
+    k_launch_step_record(c);
+}
+
+static std::atomic<uint64_t> g_step_capture_gen{0};
+
+// the start of a step: statistics, timing, step counter and emitter
+static void step_open(FlipCtx *c, flip_step_stats *st)
+{
+    require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
+    memset(st, 0, sizeof(*st));
+    timing_reset(c);
+    hipEvent_t *evs[4] = {&c->ev_a, &c->ev_b, &c->ev_s0, &c->ev_s1};
+    for (hipEvent_t *e : evs)
+        if (!*e) FLIP_CHECK(hipEventCreate(e));
+    if (!c->begun) begin_step(c);
+    c->begun = false;
+}
+
+// whether this step replays its captured parts; drops them when what they
+// bake in has changed
+static bool step_graphs_usable(FlipCtx *c)
+{
+    const bool emitting = c->step <= c->pour_time && c->emit_n > 0;
+    if (!c->step_graphs || c->profiling || c->direct_launch || emitting || c->prm.refine != 0 || c->prm.solve_fp64)
+        return false;
+    std::vector<uint64_t> sig = step_graph_sig(c);
+    if (sig != c->step_graph_sig) {
+        for (int w = 0; w < 2; ++w) {
+            if (c->step_gx[w]) FLIP_CHECK(hipGraphExecDestroy(c->step_gx[w]));
+            if (c->step_g[w]) FLIP_CHECK(hipGraphDestroy(c->step_g[w]));
+            c->step_gx[w] = nullptr;
+            c->step_g[w] = nullptr;
+        }
+        c->step_graph_sig = std::move(sig);
+    }
+    rng_to_device(c);  // no host copy inside a captured part
+    return true;
+}
+
+// capture part `which` (0: before the solve, 1: after it) unless it is
+// captured; the graph is kept for flip_gpu_step_batch
+static void step_capture(FlipCtx *c, int which, flip_step_stats *st)
+{
+    if (c->step_gx[which]) return;
+    hipGraph_t g;
+    FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    if (which == 0) step_launch_pre(c);
+    else step_launch_post(c, st);
+    FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
+    FLIP_CHECK(hipGraphInstantiate(&c->step_gx[which], g, nullptr, nullptr, 0));
+    c->step_g[which] = g;
+    c->step_gen = ++g_step_capture_gen;
+}
+
+// the pressure solve between the parts, on the context's stream
+static void step_solve(FlipCtx *c, flip_step_stats *st)
+{
+    FLIP_CHECK(hipEventRecord(c->ev_s0, c->stream));
+    const int rc = k_solve(c, st);
+    FLIP_CHECK(hipEventRecord(c->ev_s1, c->stream));
+    c->solve_timed = true;
+    if (rc == FLIP_E_NUMERIC) throw flip_arg_error{FLIP_E_NUMERIC, "pressure solve produced NaN"};
+}
+
+// the host's view of the storage after the step (what the launches set when
+// they are captured; a replay does not run them)
+static void step_host_state(FlipCtx *c)
+{
+    c->bucketed = true;
+    c->sorted = !c->prm.correction;
+    c->advected = true;
+    c->corrected = c->prm.correction != 0;
+}
+
+// the step's record, once the device has passed it
+static int step_close(FlipCtx *c, flip_step_stats *st)
+{
+    const FlipCtx::StepRecord &rec = *c->h_rec;
+    c->dt = rec.dt;
+    st->dt = c->dt;
+    if (c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
+    c->solve_timed = false;
+    c->advected = false;
+    st->n_reposition = rec.ev[EV_MOVED];
+    if (rec.ev[EV_NONFINITE_ADV])
+        return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity in the advection");
+    if (c->corrected) {
+        st->n_jitter = rec.ev[3];
+        c->corrected = false;
+        if (rec.ev[1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
+        if (rec.ev[EV_NONFINITE])
+            return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
+    }
+    timing_collect(c);
+    return FLIP_OK;
+}
+
 int flip_gpu_step(flip_ctx *ctx, flip_step_stats *st)
 {
     return guarded([&]() -> int {
         require(ctx, "null context");
         FlipCtx *c = reinterpret_cast<FlipCtx *>(ctx);
-        require(c->uploaded, "no particles uploaded", FLIP_E_STATE);
         flip_step_stats local;
         if (!st) st = &local;
-        memset(st, 0, sizeof(*st));
-        timing_reset(c);
-        if (!c->ev_a) {
-            FLIP_CHECK(hipEventCreate(&c->ev_a));
-            FLIP_CHECK(hipEventCreate(&c->ev_b));
-        }
-        if (!c->begun) begin_step(c);
-        c->begun = false;
-        const bool emitting = c->step <= c->pour_time && c->emit_n > 0;
-        const bool graphs = c->step_graphs && !c->profiling && !c->direct_launch && !emitting &&
-                            c->prm.refine == 0 && !c->prm.solve_fp64;
-        if (graphs) {
-            std::vector<uint64_t> sig = step_graph_sig(c);
-            if (sig != c->step_graph_sig) {
-                for (auto &g : c->step_gx)
-                    if (g) FLIP_CHECK(hipGraphExecDestroy(g));
-                c->step_gx[0] = c->step_gx[1] = nullptr;
-                c->step_graph_sig = std::move(sig);
-            }
-            rng_to_device(c);  // no host copy inside a captured part
-        }
-        // a part of the step: replayed from its graph, captured on first use
-        auto part = [&](int which, auto &&launch) {
-            if (!graphs) {
-                launch();
-                return;
-            }
-            if (!c->step_gx[which]) {
-                hipGraph_t g;
-                FLIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                launch();
-                FLIP_CHECK(hipStreamEndCapture(c->stream, &g));
-                FLIP_CHECK(hipGraphInstantiate(&c->step_gx[which], g, nullptr, nullptr, 0));
-                FLIP_CHECK(hipGraphDestroy(g));
-            }
-            FLIP_CHECK(hipGraphLaunch(c->step_gx[which], c->stream));
-        };
+        step_open(c, st);
+        const bool graphs = step_graphs_usable(c);
         FLIP_CHECK(hipEventRecord(c->ev_a, c->stream));
-        part(0, [&]() {
-            k_launch_choose_dt(c);  // dt stays on the device; read back with the step's end
-            // flip3D.cpp:1221-1222 sort + computeDensity
-            k_launch_sort(c);
-            c->sorted = true;
-            k_launch_density(c, true);  // with add_ExtForce (:1230) fused in
-            // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
-            k_launch_flags(c);
-            k_launch_p2g(c);
-            k_launch_divergence(c);  // project (:807-868): div, solve, gradient
-        });
-        {
-            if (!c->ev_s0) {
-                FLIP_CHECK(hipEventCreate(&c->ev_s0));
-                FLIP_CHECK(hipEventCreate(&c->ev_s1));
-            }
-            FLIP_CHECK(hipEventRecord(c->ev_s0, c->stream));
-            const int rc = k_solve(c, st);
-            FLIP_CHECK(hipEventRecord(c->ev_s1, c->stream));
-            c->solve_timed = true;
-            if (rc == FLIP_E_NUMERIC) throw flip_arg_error{FLIP_E_NUMERIC, "pressure solve produced NaN"};
+        if (graphs) {
+            step_capture(c, 0, st);
+            FLIP_CHECK(hipGraphLaunch(c->step_gx[0], c->stream));
+        } else {
+            step_launch_pre(c);
         }
-        part(1, [&]() {
-            k_launch_gradient(c);
-            k_launch_extrapolate(c);
-            k_launch_g2p(c, true);  // reads the extrapolation's (u, u - u_save) pairs
-            run_stage(c, FLIP_STAGE_ADVECT, st);  // :1236
-            if (c->prm.correction) run_stage(c, FLIP_STAGE_CORRECT, st);  // :1242
-            k_launch_step_record(c);
-        });
-        // the host's view of the storage after the step (what the launches
-        // set when they are captured; a replay does not run them)
-        c->bucketed = true;
-        c->sorted = !c->prm.correction;
-        c->advected = true;
-        c->corrected = c->prm.correction != 0;
+        step_solve(c, st);
+        if (graphs) {
+            step_capture(c, 1, st);
+            FLIP_CHECK(hipGraphLaunch(c->step_gx[1], c->stream));
+        } else {
+            step_launch_post(c, st);
+        }
+        step_host_state(c);
         FLIP_CHECK(hipEventRecord(c->ev_b, c->stream));
         FLIP_CHECK(hipEventSynchronize(c->ev_b));
         FLIP_CHECK(hipGetLastError());
-        const FlipCtx::StepRecord &rec = *c->h_rec;
-        c->dt = rec.dt;
-        st->dt = c->dt;
         st->ms_step = elapsed(c->ev_a, c->ev_b);
-        if (c->solve_timed) st->ms_solve = elapsed(c->ev_s0, c->ev_s1);
-        c->solve_timed = false;
-        c->advected = false;
-        st->n_reposition = rec.ev[EV_MOVED];
-        if (rec.ev[EV_NONFINITE_ADV])
-            return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity in the advection");
-        if (c->corrected) {
-            st->n_jitter = rec.ev[3];
-            c->corrected = false;
-            if (rec.ev[1]) return fail(FLIP_E_STATE, "too many corrector jitter events");
-            if (rec.ev[EV_NONFINITE])
-                return fail(FLIP_E_NUMERIC, "non-finite particle position or velocity after the corrector");
+        return step_close(c, st);
+    });
+}
+
+// flip_gpu_step_batch's graphs: one child graph per context for each part,
+// built again when the context list or any context's capture changes
+namespace {
+struct BatchGraphs {
+    std::vector<std::pair<const FlipCtx *, uint64_t>> key;  // (context, capture number)
+    hipGraphExec_t gx[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> done;  // one per context: its solve has been issued
+    hipEvent_t pre = nullptr, end = nullptr;
+    void clear()
+    {
+        for (auto &g : gx)
+            if (g) (void)hipGraphExecDestroy(g);
+        gx[0] = gx[1] = nullptr;
+        for (auto e : done) (void)hipEventDestroy(e);
+        done.clear();
+        if (pre) (void)hipEventDestroy(pre);
+        if (end) (void)hipEventDestroy(end);
+        pre = end = nullptr;
+        key.clear();
+    }
+};
+std::mutex g_batch_mu;
+BatchGraphs g_batch;
+}  // namespace
+
+int flip_gpu_step_batch(flip_ctx **ctxs, int count, int threads, flip_step_stats *stats, double *ms)
+{
+    if (!ctxs || count <= 0) return fail(FLIP_E_ARG, "no contexts");
+    if (threads <= 0) return fail(FLIP_E_ARG, "threads must be >= 1");
+    for (int i = 0; i < count; ++i)
+        if (!ctxs[i]) return fail(FLIP_E_ARG, "null context");
+    for (int i = 0; i < count; ++i)
+        for (int j = i + 1; j < count; ++j)
+            if (ctxs[i] == ctxs[j]) return fail(FLIP_E_ARG, "a context appears twice");
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    std::vector<flip_step_stats> local(stats ? 0 : count);
+    flip_step_stats *st = stats ? stats : local.data();
+    return guarded([&]() -> int {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<FlipCtx *> cs(count);
+        for (int i = 0; i < count; ++i) {
+            FlipCtx *c = reinterpret_cast<FlipCtx *>(ctxs[i]);
+            cs[i] = c;
+            step_open(c, &st[i]);
+            require(step_graphs_usable(c),
+                    "flip_gpu_step_batch needs step graphs on every context (no emitter step, profiling, direct "
+                    "launch, refinement or FP64 solve)",
+                    FLIP_E_STATE);
+            // both parts captured before any runs (their host bookkeeping
+            // ends where it began: two sorts)
+            step_capture(c, 0, &st[i]);
+            step_capture(c, 1, &st[i]);
         }
-        timing_collect(c);
-        return FLIP_OK;
+        BatchGraphs &B = g_batch;
+        std::vector<std::pair<const FlipCtx *, uint64_t>> key(count);
+        for (int i = 0; i < count; ++i) key[i] = {cs[i], cs[i]->step_gen};
+        if (key != B.key) {
+            B.clear();
+            for (int w = 0; w < 2; ++w) {
+                hipGraph_t g;
+                FLIP_CHECK(hipGraphCreate(&g, 0));
+                for (FlipCtx *c : cs) {
+                    hipGraphNode_t node;
+                    FLIP_CHECK(hipGraphAddChildGraphNode(&node, g, nullptr, 0, c->step_g[w]));
+                }
+                FLIP_CHECK(hipGraphInstantiate(&B.gx[w], g, nullptr, nullptr, 0));
+                FLIP_CHECK(hipGraphDestroy(g));
+            }
+            B.done.resize(count, nullptr);
+            for (auto &e : B.done) FLIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            FLIP_CHECK(hipEventCreateWithFlags(&B.pre, hipEventDisableTiming));
+            FLIP_CHECK(hipEventCreateWithFlags(&B.end, hipEventDisableTiming));
+            B.key = key;
+        }
+        hipStream_t master = cs[0]->stream;
+        // every context's kernels before its solve: one launch
+        FLIP_CHECK(hipGraphLaunch(B.gx[0], master));
+        FLIP_CHECK(hipEventRecord(B.pre, master));
+        for (int i = 1; i < count; ++i) FLIP_CHECK(hipStreamWaitEvent(cs[i]->stream, B.pre, 0));
+        // the solves, each on its context's stream, driven by host threads
+        threads = std::min(threads, count);
+        std::atomic<int> next{0};
+        std::mutex mu;
+        int first_rc = FLIP_OK, first_at = count;
+        std::string first_msg;
+        auto worker = [&]() {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= count) return;
+                const int rc = guarded([&]() -> int {
+                    step_solve(cs[i], &st[i]);
+                    FLIP_CHECK(hipEventRecord(B.done[i], cs[i]->stream));
+                    return FLIP_OK;
+                });
+                if (rc < 0) {
+                    std::lock_guard<std::mutex> l2(mu);
+                    if (i < first_at) {
+                        first_at = i;
+                        first_rc = rc;
+                        first_msg = flip_gpu_last_error();
+                    }
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+        worker();
+        for (auto &th : pool) th.join();
+        if (first_rc < 0) throw flip_arg_error{first_rc, "context " + std::to_string(first_at) + ": " + first_msg};
+        // every context's kernels after its solve: one launch
+        for (int i = 0; i < count; ++i) FLIP_CHECK(hipStreamWaitEvent(master, B.done[i], 0));
+        FLIP_CHECK(hipGraphLaunch(B.gx[1], master));
+        FLIP_CHECK(hipEventRecord(B.end, master));
+        FLIP_CHECK(hipEventSynchronize(B.end));
+        FLIP_CHECK(hipGetLastError());
+        const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        int rc = FLIP_OK;
+        for (int i = 0; i < count; ++i) {
+            FlipCtx *c = cs[i];
+            step_host_state(c);
+            st[i].ms_step = wall;
+            if (ms) ms[i] = wall;
+            const int r = step_close(c, &st[i]);
+            if (r < 0 && rc == FLIP_OK) rc = r;
+        }
+        return rc;
     });
 }
 

@@ -155,6 +155,8 @@ struct FlipCtx {
     // and replayed; the solve between them keeps its own captured iterations.
     bool step_graphs = true;
     hipGraphExec_t step_gx[2] = {nullptr, nullptr};
+    hipGraph_t step_g[2] = {nullptr, nullptr};  // the captured graphs (children of a batch's graphs)
+    uint64_t step_gen = 0;                       // process-wide number of the context's last capture
     std::vector<uint64_t> step_graph_sig;  // the pointers and scalars the captured launches bake in
     // what the step's end reads, written by the device into mapped host
     // memory (k_step_record): the event counts and the time step

@@ -8,8 +8,14 @@ particles per cell, CFL dt, pressure solve to relative residual 1e-5.
 Every scene uses the learned preconditioner (--no-learned: the
 reference-preconditioned PCG), with the solver launch shape the autotuner
 chose for its size (flip3d_amd/tuning.json; --no-tuning: the built-in one).
-Every step advances all scenes through flip_gpu_step_many: host threads
-drive the contexts concurrently, each on its own HIP stream.
+Every step advances all scenes at once.  --mode threads (the default):
+flip_gpu_step_many, host threads stepping the contexts concurrently, each on
+its own HIP stream, every step replaying its own captured parts around the
+solve.  --mode batch: flip_gpu_step_batch, the kernels before the pressure
+solves of all scenes in one graph launch (each scene's captured part a
+child graph), the solves concurrently on the scenes' streams, then the
+kernels after the solves in one more graph launch (slower here: every
+scene waits at each phase for the largest).
 
 Prints one JSON line: scene-steps/s over the timed steps, and the p50/p99 of
 the per-scene step latency (wall clock from the start of a scene's step to
@@ -46,6 +52,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--seed", type=int, default=SEED)
+    ap.add_argument("--mode", choices=["threads", "batch"], default="threads")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-tuning", action="store_true",
                     help="built-in solver launch shapes instead of flip3d_amd/tuning.json's")
@@ -66,13 +73,14 @@ def main():
         particles += c.build_scene(fa.SCENE_RANDOM_BOXES, i)
         c.srand(1)
         ctxs.append(c)
+    step = fa.step_batch if args.mode == "batch" else fa.step_many
     for _ in range(args.warmup):
-        fa.step_many(ctxs, args.threads)
+        step(ctxs, args.threads)
     lat = []
     iters = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st, ms = fa.step_many(ctxs, args.threads)
+        st, ms = step(ctxs, args.threads)
         lat += ms
         iters += [s["iterations"] for s in st]
     elapsed = time.perf_counter() - t0
@@ -81,6 +89,7 @@ def main():
     out = {
         "metric": "ensemble scene-steps/s", "value": round(args.scenes * args.steps / elapsed, 2),
         "unit": "scene-steps/s", "scenes": args.scenes, "steps": args.steps, "threads": args.threads,
+        "mode": args.mode,
         "ms_per_batch_step": round(1e3 * elapsed / args.steps, 3),
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),

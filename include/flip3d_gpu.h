@@ -308,6 +308,19 @@ int flip_gpu_set_step_graphs(flip_ctx *ctx, int on);
  * No counterpart in the reference, which simulates one scene per process. */
 int flip_gpu_step_many(flip_ctx **ctxs, int count, int threads, flip_step_stats *stats, double *ms);
 
+/* Ensembles in one launch per part of the step: steps every context of ctxs
+ * once, as flip_gpu_step would, with the kernels before the pressure solve
+ * of all contexts replayed as ONE graph launch (each context's captured part
+ * a child graph, the contexts independent of each other), then the solves
+ * (each on its context's stream, `threads` host threads driving them), then
+ * the kernels after the solves of all contexts as one more graph launch.
+ * Every context must be able to use step graphs (flip_gpu_set_step_graphs;
+ * no emitter step, profiling, direct launch, refinement or FP64 solve).
+ * ms[i] (optional) and stats[i].ms_step are the batch's wall time.  Calls
+ * are serialised (one batch's graphs are kept and rebuilt when the contexts
+ * or their captures change).  No counterpart in the reference. */
+int flip_gpu_step_batch(flip_ctx **ctxs, int count, int threads, flip_step_stats *stats, double *ms);
+
 /* Device math self-check for the ops whose rounding the parity argument
  * relies on: out[4*i+0] = x/y, +1 = sqrtf(x), +2 = (float)sqrt((double)x*x+(double)y*y),
  * +3 = (float)((double)x / (double)y); and out[4*m+i] = kernel.cpp:8-10's
