@@ -1359,7 +1359,7 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
                                                       int32_t *list, int32_t *count)
 {
     const float dt = (float)dtv[0];
-    __shared__ int32_t near[SPRING_NEAR][SPRING_TB];
+    __shared__ int32_t near[SPRING_NEAR + 1][SPRING_TB];  // slot SPRING_NEAR: scratch
     __shared__ int32_t s_off[9 * SPRING_TB / 64], s_idx[SPRING_TB];
     const int64_t base = (int64_t)blockIdx.x * SPRING_TB;
     float cx, cy, cz;
@@ -1388,13 +1388,16 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
     int cnt = 0;
     int rlo[9], rhi[9];
     window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
+    // Branch-free listing: every record is written to the next free slot
+    // (the scratch slot once the list is full) and counted only when it is
+    // a neighbour, so a later neighbour overwrites a non-neighbour's entry.
+    const int self = (int)i;
 #pragma unroll
     for (int r = 0; r < 9; ++r)
         for_range8(pos, rlo[r], rhi[r], [&](const float4 &q, int a) {
-            if (a == i) return;
-            if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
-            if (cnt < SPRING_NEAR) near[cnt][tid] = a;
-            ++cnt;
+            const bool nb = (a != self) & !(length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2);
+            near[min(cnt, SPRING_NEAR)][tid] = a;
+            cnt += nb;
         });
     int r = 0;
     if (cnt > SPRING_NEAR) {
@@ -1746,7 +1749,7 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
                                                           const float4 *__restrict__ tmp0, float4 *tmp1, float hh,
                                                           const P3 *__restrict__ xyz3)
 {
-    __shared__ int32_t near[RESAMPLE_NEAR][RESAMPLE_TB];
+    __shared__ int32_t near[RESAMPLE_NEAR + 1][RESAMPLE_TB];  // slot RESAMPLE_NEAR: scratch
     __shared__ int32_t s_off[9 * RESAMPLE_TB / 64], s_idx[RESAMPLE_TB];
     const int64_t base = (int64_t)blockIdx.x * RESAMPLE_TB;
     const float g = (float)n;
@@ -1769,12 +1772,13 @@ __global__ __launch_bounds__(RESAMPLE_TB) void k_resample(int64_t np, int n, con
     int cnt = 0;
     int rlo[9], rhi[9];
     window_rows(n, wc, ci, cj, ck, cs, rlo, rhi);
+    // branch-free listing, as in k_spring
 #pragma unroll
     for (int r = 0; r < 9; ++r)
         for_range8(xyz3, rlo[r], rhi[r], [&](const P3 &q, int a) {
-            if (!(length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh)) return;
-            if (cnt < RESAMPLE_NEAR) near[cnt][tid] = a;
-            ++cnt;
+            const bool nb = length2(p.x, p.y, p.z, q.x, q.y, q.z) < hh;
+            near[min(cnt, RESAMPLE_NEAR)][tid] = a;
+            cnt += nb;
         });
     float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
     if (cnt > RESAMPLE_NEAR) {
