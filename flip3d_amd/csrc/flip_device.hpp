@@ -106,6 +106,18 @@ __device__ __forceinline__ float length3(float ax, float ay, float az, float bx,
     return hypotf_glibc(hypotf_glibc(ax - bx, ay - by), az - bz);
 }
 
+// (i, j, k) of the flat index f = (i * nj + j) * nk + k.  Grids have n <= 1024,
+// so every cell or face index fits in 32 bits; two 32-bit divisions replace
+// the three 64-bit divisions and remainders of the plain decomposition.
+struct Split3 {
+    int i, j, k;
+};
+__device__ __forceinline__ Split3 split3(long f, int nj, int nk)
+{
+    const uint32_t u = (uint32_t)f, q = u / (uint32_t)nk, i = q / (uint32_t)nj;
+    return Split3{(int)i, (int)(q - i * (uint32_t)nj), (int)(u - q * (uint32_t)nk)};
+}
+
 // sorter.cpp:35-37 cell key (int)fmax(0, fmin(gn-1, gn*p)); NaN maps to gn-1.
 __device__ __forceinline__ int cell_of(int gn, float p)
 {

@@ -21,7 +21,8 @@ __global__ void k_divergence(int n, const int8_t *__restrict__ A, const float *_
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long C = (long)n * n * n;
     if (c >= C) return;
-    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
+    const Split3 s3 = split3(c, n, n);
+    const int i = s3.i, j = s3.j, k = s3.k;
     if (A[c] == FLUID) {
         float d = u0[fidx0(n, i + 1, j, k)] - u0[fidx0(n, i, j, k)];
         d = d + u1[fidx1(n, i, j + 1, k)];
@@ -45,9 +46,8 @@ __global__ void k_gradient(int n, const int8_t *__restrict__ A, const float *__r
     const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
     long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= (long)ni * nj * nk) return;
-    int k = (int)(f % nk);
-    int j = (int)((f / nk) % nj);
-    int i = (int)(f / ((long)nk * nj));
+    const Split3 s3 = split3(f, nj, nk);
+    const int i = s3.i, j = s3.j, k = s3.k;
     int a0 = COMP == 0 ? i : (COMP == 1 ? j : k);
     if (a0 == 0 || a0 == n) {
         u[f] = 0.0f;  // enforce_boundary: domain faces
@@ -91,9 +91,8 @@ __global__ void k_extrapolate(int n, const int8_t *__restrict__ A, float *u, flo
     const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
     long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= (long)ni * nj * nk) return;
-    int k = (int)(f % nk);
-    int j = (int)((f / nk) % nj);
-    int i = (int)(f / ((long)nk * nj));
+    const Split3 s3 = split3(f, nj, nk);
+    const int i = s3.i, j = s3.j, k = s3.k;
     bool mark, wmark;
     face_marks<COMP>(n, A, i, j, k, mark, wmark);
     float v = u[f];

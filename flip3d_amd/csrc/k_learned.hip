@@ -66,11 +66,8 @@ struct IJK {
 
 __device__ __forceinline__ IJK ijk_of(long c, int n)
 {
-    const long nn = (long)n * n;
-    const int i = (int)(c / nn);
-    const long rem = c - (long)i * nn;
-    const int j = (int)(rem / n);
-    return IJK{i, j, (int)(rem - (long)j * n)};
+    const Split3 s = split3(c, n, n);
+    return IJK{s.i, s.j, s.k};
 }
 
 // parent (side n/2) of the cell at p plus the offset of neighbour u (u = 6: the cell itself)
@@ -94,7 +91,8 @@ __global__ void k_ml_coarsen(int Rc, int Rf, const int8_t *__restrict__ Af, cons
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long Cc = (long)Rc * Rc * Rc;
     if (g >= Cc) return;
-    int K = (int)(g % Rc), J = (int)((g / Rc) % Rc), I = (int)(g / ((long)Rc * Rc));
+    const Split3 s3 = split3(g, Rc, Rc);
+    const int I = s3.i, J = s3.j, K = s3.k;
     int fluid = 0, wall = 0, nw = 0;
     float lf = 0.0f, ls = 0.0f;
     for (int d = 0; d < 8; ++d) {
@@ -127,7 +125,8 @@ __global__ void k_ml_meta(int R, const int8_t *__restrict__ A, const float *__re
         dinv[c] = 0.0f;
         return;
     }
-    int k = (int)(c % R), j = (int)((c / R) % R), i = (int)(c / ((long)R * R));
+    const Split3 s3 = split3(c, R, R);
+    const int i = s3.i, j = s3.j, k = s3.k;
     const bool inr[6] = {i + 1 <= R - 1, i - 1 >= 0, j + 1 <= R - 1, j - 1 >= 0, k + 1 <= R - 1, k - 1 >= 0};
     uint32_t m = 0;
 #pragma unroll

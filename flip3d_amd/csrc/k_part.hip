@@ -819,7 +819,8 @@ __global__ void k_deep(long C, int n, const int8_t *__restrict__ A, int32_t *fla
     if (mcount[0] == 0) return;
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
+    const Split3 s3 = split3(c, n, n);
+    const int i = s3.i, j = s3.j, k = s3.k;
     bool d = A[c] == FLUID;
     if (d && i > 0 && A[c - (long)n * n] != FLUID) d = false;
     if (d && i < n - 1 && A[c + (long)n * n] != FLUID) d = false;

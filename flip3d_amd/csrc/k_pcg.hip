@@ -73,7 +73,8 @@ __global__ void k_fluid_compact(long C, int n, const int8_t *__restrict__ A, con
 {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C || A[c] != FLUID) return;
-    int k = (int)(c % n), j = (int)((c / n) % n), i = (int)(c / ((long)n * n));
+    const Split3 s3 = split3(c, n, n);
+    const int i = s3.i, j = s3.j, k = s3.k;
     const bool inr[6] = {i + 1 <= n - 1, i - 1 >= 0, j + 1 <= n - 1, j - 1 >= 0, k + 1 <= n - 1, k - 1 >= 0};
     uint32_t m = 0;
 #pragma unroll
