@@ -971,10 +971,11 @@ static void step_launch_pre(FlipCtx *c)
     // flip3D.cpp:1221-1222 sort + computeDensity
     k_launch_sort(c);
     c->sorted = true;
-    k_launch_density(c, true);  // with add_ExtForce (:1230) fused in
+    // with add_ExtForce (:1230) fused in, and P2G's packed records written
+    const bool records = k_launch_density(c, true, true);
     // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
     k_launch_flags(c);
-    k_launch_p2g(c);
+    k_launch_p2g(c, records);
     k_launch_divergence(c);  // project (:807-868): div, solve, gradient
 }
 

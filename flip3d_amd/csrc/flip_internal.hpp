@@ -207,10 +207,12 @@ struct FlipCtx {
 
 // ---- kernels (launch wrappers; each enqueues on ctx->stream) ----
 void k_launch_sort(FlipCtx *c);                 // counting sort + per-cell id order + permute
-void k_launch_density(FlipCtx *c, bool with_force = false);  // with_force: add_ExtForce fused in
+// with_force: add_ExtForce fused in; p2g_records: also write P2G's packed
+// records (returns whether it did)
+bool k_launch_density(FlipCtx *c, bool with_force = false, bool p2g_records = false);
 void k_launch_ext_force(FlipCtx *c);
 void k_launch_flags(FlipCtx *c);                // A and L from the sorted buckets
-void k_launch_p2g(FlipCtx *c);                  // P2G + save_grid + enforce_boundary
+void k_launch_p2g(FlipCtx *c, bool records_ready = false);  // P2G + save_grid + enforce_boundary
 void k_launch_divergence(FlipCtx *c);           // div on FLUID cells
 void k_launch_gradient(FlipCtx *c);             // ghost-fluid gradient + enforce_boundary
 void k_launch_extrapolate(FlipCtx *c);          // extrapolate_velocity + subtract_grid

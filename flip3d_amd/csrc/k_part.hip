@@ -40,6 +40,30 @@ struct P3 {
     float x, y, z;
 };
 
+// The unit-mass P2G records of particle i: (x, y, z, u_c) per component with
+// the position scaled by n and clamped to [0, n] (mapper.cpp:27-29); a WALL
+// particle is placed at infinity, outside every face's support.
+__device__ __forceinline__ void p2g_records_wall(int64_t i, float4 *px, float4 *py, float4 *pz)
+{
+    const float inf = __builtin_inff();
+    const float4 far = make_float4(inf, inf, inf, 0.0f);
+    px[i] = far;
+    py[i] = far;
+    pz[i] = far;
+}
+
+__device__ __forceinline__ void p2g_records(int64_t i, int n, const float4 &p, const float4 &v, float4 *px,
+                                            float4 *py, float4 *pz)
+{
+    const float g = (float)n;
+    const float x = fmaxf(fminf(g * p.x, g), 0.0f);
+    const float y = fmaxf(fminf(g * p.y, g), 0.0f);
+    const float z = fmaxf(fminf(g * p.z, g), 0.0f);
+    px[i] = make_float4(x, y, z, v.x);
+    py[i] = make_float4(x, y, z, v.y);
+    pz[i] = make_float4(x, y, z, v.z);
+}
+
 template <bool POW2, bool FORCE>
 __global__ __launch_bounds__(TB) void k_density(int64_t np, int n, const float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, const int32_t *__restrict__ cs, float h,
@@ -116,7 +140,8 @@ template <bool FORCE>
 __global__ __launch_bounds__(TB) void k_density_lds(int64_t np, int n, const float4 *__restrict__ pos,
                                                     float4 *__restrict__ vel, const int32_t *__restrict__ cs,
                                                     float inv_hh, float max_dens, const double *__restrict__ dtv,
-                                                    double gravity, const P3 *__restrict__ xyz3)
+                                                    double gravity, const P3 *__restrict__ xyz3, float4 *px,
+                                                    float4 *py, float4 *pz)
 {
     __shared__ P3 st[DL_CAP];
     __shared__ int skey[TB];
@@ -194,6 +219,7 @@ __global__ __launch_bounds__(TB) void k_density_lds(int64_t np, int n, const flo
         if (FORCE) v.y = (float)((double)v.y + gdt);
         v.w = 1.0f;
         vel[i] = v;
+        if (px) p2g_records_wall(i, px, py, pz);
         return;
     }
     const int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
@@ -219,6 +245,7 @@ __global__ __launch_bounds__(TB) void k_density_lds(int64_t np, int n, const flo
     if (FORCE) v.y = (float)((double)v.y + gdt);
     v.w = wsum / max_dens;
     vel[i] = v;
+    if (px) p2g_records(i, n, p, v, px, py, pz);
 }
 
 // flip3D.cpp:870-875 add_ExtForce: u_y = (float)((double)u_y + gdt), gdt = -dt*g
@@ -318,23 +345,12 @@ __global__ void k_p2g_prep_unit(int64_t np, int n, const float4 *__restrict__ po
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    float4 p = pos[i];
+    const float4 p = pos[i];
     if (is_wall_w(p.w)) {
-        const float inf = __builtin_inff();
-        float4 far = make_float4(inf, inf, inf, 0.0f);
-        px[i] = far;
-        py[i] = far;
-        pz[i] = far;
+        p2g_records_wall(i, px, py, pz);
         return;
     }
-    float4 v = vel[i];
-    float g = (float)n;
-    float x = fmaxf(fminf(g * p.x, g), 0.0f);
-    float y = fmaxf(fminf(g * p.y, g), 0.0f);
-    float z = fmaxf(fminf(g * p.z, g), 0.0f);
-    px[i] = make_float4(x, y, z, v.x);
-    py[i] = make_float4(x, y, z, v.y);
-    pz[i] = make_float4(x, y, z, v.z);
+    p2g_records(i, n, p, vel[i], px, py, pz);
 }
 
 // mapper.cpp:17-88 window sum of one face with unit masses and packed
@@ -1910,9 +1926,9 @@ void k_launch_emit(FlipCtx *c)
     c->sorted = false;
     c->bucketed = false;
 }
-void k_launch_density(FlipCtx *c, bool with_force)
+bool k_launch_density(FlipCtx *c, bool with_force, bool p2g_records)
 {
-    if (c->np == 0) return;
+    if (c->np == 0) return false;
     const float h = (float)(4.0 * c->prm.density / c->n), hh = h * h;
     int e = 0;
     const bool pow2 = std::isnormal(hh) && std::frexp(hh, &e) == 0.5f;
@@ -1921,15 +1937,19 @@ void k_launch_density(FlipCtx *c, bool with_force)
     const bool packed = pow2 && c->unit_mass && c->sorted;
     const char *name = with_force ? "density_force" : "density";
     if (packed) {
+        // the P2G records are formed here only from the velocities P2G will see
+        const bool rec = p2g_records && with_force;
         launch_timed(c, name, with_force ? k_density_lds<true> : k_density_lds<false>, nblk(c->np, TB), TB, c->np,
                      c->n, c->P4(), c->V4(), c->cell_start, inv_hh, c->max_dens, c->dtv, c->prm.gravity,
-                     reinterpret_cast<const P3 *>(c->xyz3));
-        return;
+                     reinterpret_cast<const P3 *>(c->xyz3), rec ? c->spos : nullptr, rec ? c->tmp0 : nullptr,
+                     rec ? c->tmp1 : nullptr);
+        return rec;
     }
     auto kern = pow2 ? (with_force ? k_density<true, true> : k_density<true, false>)
                      : (with_force ? k_density<false, true> : k_density<false, false>);
     launch_timed(c, name, kern, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->cell_start, h, inv_hh,
                  c->max_dens, c->dtv, c->prm.gravity);
+    return false;
 }
 
 void k_launch_ext_force(FlipCtx *c)
@@ -1948,14 +1968,14 @@ void k_launch_flags(FlipCtx *c)
                  c->L, 0.2 * (double)n0);
 }
 
-void k_launch_p2g(FlipCtx *c)
+void k_launch_p2g(FlipCtx *c, bool records_ready)
 {
     const float re = (float)1.4;  // RE, mapper.cpp:14
     const float hh = re * re;
     if (c->unit_mass) {
         // packed (x, y, z, u_c) per component in spos / tmp0 / tmp1 (free until the corrector)
         t_begin(c, "p2g_prep");
-        if (c->np > 0)
+        if (c->np > 0 && !records_ready)
             hipLaunchKernelGGL(k_p2g_prep_unit, nblk(c->np, TB), TB, 0, c->stream, c->np, c->n, c->P4(), c->V4(),
                                c->spos, c->tmp0, c->tmp1);
         t_end(c);
