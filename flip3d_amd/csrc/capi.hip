@@ -203,6 +203,8 @@ void alloc_grids(FlipCtx *c)
     dalloc(&c->cell_count, c->C + 1);
     dalloc(&c->cell_start, c->C + 1);
     dalloc(&c->wall_start, c->C + 1);
+    dalloc(&c->wall_near, c->C);
+    FLIP_CHECK(hipMemset(c->wall_near, 0, c->C * sizeof(int8_t)));
     dalloc(&c->scan_tmp, c->C + 1);
     dalloc(&c->fl, c->C);
     dalloc(&c->P, c->C);
@@ -255,6 +257,7 @@ void free_all(FlipCtx *c)
     dfree(c->cell_count);
     dfree(c->cell_start);
     dfree(c->wall_start);
+    dfree(c->wall_near);
     dfree(c->scan_tmp);
     dfree(c->fl);
     dfree(c->P);
@@ -336,11 +339,22 @@ void build_wall_bins(FlipCtx *c, const std::vector<float4> &pos, const std::vect
         wp[t] = pos[walls[t].second];
         wn[t] = nrm_id[(uint32_t)meta[walls[t].second] >> 2];
     }
+    // cells whose 3x3x3 window holds a wall particle (the advection's push-out
+    // visits only these windows)
+    std::vector<int8_t> near(c->C, 0);
+    for (long k = 0; k < c->C; ++k) {
+        if (start[k + 1] == 0) continue;
+        const int ci = (int)(k / ((long)n * n)), cj = (int)((k / n) % n), ck = (int)(k % n);
+        for (int a = std::max(ci - 1, 0); a <= std::min(ci + 1, n - 1); ++a)
+            for (int b = std::max(cj - 1, 0); b <= std::min(cj + 1, n - 1); ++b)
+                for (int d = std::max(ck - 1, 0); d <= std::min(ck + 1, n - 1); ++d) near[((long)a * n + b) * n + d] = 1;
+    }
     for (long k = 0; k < c->C; ++k) start[k + 1] += start[k];
     c->nwall = (int64_t)walls.size();
     dalloc(&c->wall_pos, walls.size());
     dalloc(&c->wall_nrm, walls.size());
     FLIP_CHECK(hipMemcpy(c->wall_start, start.data(), (c->C + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    FLIP_CHECK(hipMemcpy(c->wall_near, near.data(), c->C * sizeof(int8_t), hipMemcpyHostToDevice));
     if (!walls.empty()) {
         FLIP_CHECK(hipMemcpy(c->wall_pos, wp.data(), wp.size() * sizeof(float4), hipMemcpyHostToDevice));
         FLIP_CHECK(hipMemcpy(c->wall_nrm, wn.data(), wn.size() * sizeof(float4), hipMemcpyHostToDevice));
@@ -958,7 +972,7 @@ static std::vector<uint64_t> step_graph_sig(const FlipCtx *c)
     const void *bufs[] = {c->nrm_id,   c->inv,       c->spos,      c->xyz3,     c->tmp0,     c->tmp1,
                           c->pkey,     c->pslot,     c->perm,      c->sort_big, c->deep,     c->ev_list,
                           c->ev_off,   c->ev_slot,   c->repo_ids,  c->repo_order, c->repo_loff, c->ev_rand,
-                          c->wall_start, c->wall_pos, c->wall_nrm};
+                          c->wall_start, c->wall_near, c->wall_pos, c->wall_nrm};
     for (const void *b : bufs) ptr(b);
     return s;
 }

@@ -85,6 +85,7 @@ struct FlipCtx {
     // cells
     int32_t *cell_count = nullptr, *cell_start = nullptr;
     int32_t *wall_start = nullptr;  // static wall bins
+    int8_t *wall_near = nullptr;    // per cell: a wall particle lies in its 3x3x3 window
     float4 *wall_pos = nullptr, *wall_nrm = nullptr;
     int64_t nwall = 0;
 

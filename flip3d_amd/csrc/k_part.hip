@@ -748,8 +748,9 @@ __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__rest
                                                const float *__restrict__ u0, const float *__restrict__ u1,
                                                const float *__restrict__ u2, const double *__restrict__ dtv,
                                                float rlo, double re,
-                                               const int32_t *__restrict__ wst, const float4 *__restrict__ wpos,
-                                               const float4 *__restrict__ wnrm, int32_t *nonfinite)
+                                               const int32_t *__restrict__ wst, const int8_t *__restrict__ wnear,
+                                               const float4 *__restrict__ wpos, const float4 *__restrict__ wnrm,
+                                               int32_t *nonfinite)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
@@ -770,7 +771,9 @@ __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__rest
     p.z = (float)fmax(lo, fmin(hi, (double)p.z));
     int ci = cell_of_b(n, p.x), cj = cell_of_b(n, p.y), ck = cell_of_b(n, p.z);
     int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
-    for (int si = ci - 1; si <= ci + 1; ++si) {
+    // a window with no wall particle pushes nothing
+    const bool walls = wnear[cidx(n, ci, cj, ck)] != 0;
+    for (int si = ci - 1; walls && si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
             if (sj < 0 || sj > n - 1) continue;
@@ -2030,7 +2033,7 @@ void k_launch_advect(FlipCtx *c)
     if (c->np > 0)
         launch_timed(c, "advect", k_advect, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0], c->ug[1],
                      c->ug[2], c->dtv, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n, c->wall_start,
-                     c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE_ADV);
+                     c->wall_near, c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE_ADV);
 }
 
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s)
