@@ -40,11 +40,10 @@ __device__ __forceinline__ int wall_chk(int8_t a) { return a == WALL ? 1 : -1; }
 // flip3D.cpp:833-867 ghost-fluid pressure gradient, then the second
 // enforce_boundary (flip3D.cpp:1067) on the same face.
 template <int COMP>
-__global__ void k_gradient(int n, const int8_t *__restrict__ A, const float *__restrict__ L,
-                           const float *__restrict__ press, float *u, float h, int subcell)
+__device__ __forceinline__ void gradient_face(int n, const int8_t *__restrict__ A, const float *__restrict__ L,
+                                              const float *__restrict__ press, float *u, float h, int subcell, long f)
 {
     const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
-    long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= (long)ni * nj * nk) return;
     const Split3 s3 = split3(f, nj, nk);
     const int i = s3.i, j = s3.j, k = s3.k;
@@ -64,6 +63,22 @@ __global__ void k_gradient(int n, const int8_t *__restrict__ A, const float *__r
     float v = u[f] - (pf - pb) / h;
     if (wall_chk(A[cf]) * wall_chk(A[cb]) < 0) v = 0.0f;
     u[f] = v;
+}
+
+// the three face components in one launch, nb blocks each (the components
+// are independent)
+__global__ void k_gradient3(int n, const int8_t *__restrict__ A, const float *__restrict__ L,
+                            const float *__restrict__ press, float *u0, float *u1, float *u2, float h, int subcell,
+                            unsigned nb)
+{
+    const unsigned b = blockIdx.x;
+    const long t = threadIdx.x;
+    if (b < nb)
+        gradient_face<0>(n, A, L, press, u0, h, subcell, (long)b * blockDim.x + t);
+    else if (b < 2 * nb)
+        gradient_face<1>(n, A, L, press, u1, h, subcell, (long)(b - nb) * blockDim.x + t);
+    else
+        gradient_face<2>(n, A, L, press, u2, h, subcell, (long)(b - 2 * nb) * blockDim.x + t);
 }
 
 // flip3D.cpp:1011-1054 mark / wall_mark of a face of component n
@@ -86,10 +101,10 @@ __device__ __forceinline__ void face_marks(int n, const int8_t *__restrict__ A, 
 // u_save = u - u_save.  Extrapolated faces read only marked faces, which are
 // never written here.
 template <int COMP>
-__global__ void k_extrapolate(int n, const int8_t *__restrict__ A, float *u, float *us, float2 *ud)
+__device__ __forceinline__ void extrapolate_face(int n, const int8_t *__restrict__ A, float *u, float *us,
+                                                 float2 *ud, long f)
 {
     const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
-    long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= (long)ni * nj * nk) return;
     const Split3 s3 = split3(f, nj, nk);
     const int i = s3.i, j = s3.j, k = s3.k;
@@ -122,6 +137,21 @@ __global__ void k_extrapolate(int n, const int8_t *__restrict__ A, float *u, flo
     ud[f] = make_float2(v, d);
 }
 
+// the three face components in one launch, nb blocks each (an extrapolated
+// face reads only faces of its own component)
+__global__ void k_extrapolate3(int n, const int8_t *__restrict__ A, float *u0, float *u1, float *u2, float *s0,
+                               float *s1, float *s2, float2 *d0, float2 *d1, float2 *d2, unsigned nb)
+{
+    const unsigned b = blockIdx.x;
+    const long t = threadIdx.x;
+    if (b < nb)
+        extrapolate_face<0>(n, A, u0, s0, d0, (long)b * blockDim.x + t);
+    else if (b < 2 * nb)
+        extrapolate_face<1>(n, A, u1, s1, d1, (long)(b - nb) * blockDim.x + t);
+    else
+        extrapolate_face<2>(n, A, u2, s2, d2, (long)(b - 2 * nb) * blockDim.x + t);
+}
+
 }  // namespace
 
 void k_launch_divergence(FlipCtx *c)
@@ -137,9 +167,8 @@ void k_launch_gradient(FlipCtx *c)
     t_begin(c, "gradient");
     float h = (float)(1.0 / c->n);
     unsigned nb = nblk(c->F, TB);
-    hipLaunchKernelGGL(k_gradient<0>, nb, TB, 0, c->stream, c->n, c->A, c->L, c->press, c->ug[0], h, c->prm.subcell);
-    hipLaunchKernelGGL(k_gradient<1>, nb, TB, 0, c->stream, c->n, c->A, c->L, c->press, c->ug[1], h, c->prm.subcell);
-    hipLaunchKernelGGL(k_gradient<2>, nb, TB, 0, c->stream, c->n, c->A, c->L, c->press, c->ug[2], h, c->prm.subcell);
+    hipLaunchKernelGGL(k_gradient3, 3 * nb, TB, 0, c->stream, c->n, c->A, c->L, c->press, c->ug[0], c->ug[1],
+                       c->ug[2], h, c->prm.subcell, nb);
     t_end(c);
 }
 
@@ -147,8 +176,7 @@ void k_launch_extrapolate(FlipCtx *c)
 {
     t_begin(c, "extrapolate");
     unsigned nb = nblk(c->F, TB);
-    hipLaunchKernelGGL(k_extrapolate<0>, nb, TB, 0, c->stream, c->n, c->A, c->ug[0], c->us[0], c->ud[0]);
-    hipLaunchKernelGGL(k_extrapolate<1>, nb, TB, 0, c->stream, c->n, c->A, c->ug[1], c->us[1], c->ud[1]);
-    hipLaunchKernelGGL(k_extrapolate<2>, nb, TB, 0, c->stream, c->n, c->A, c->ug[2], c->us[2], c->ud[2]);
+    hipLaunchKernelGGL(k_extrapolate3, 3 * nb, TB, 0, c->stream, c->n, c->A, c->ug[0], c->ug[1], c->ug[2], c->us[0],
+                       c->us[1], c->us[2], c->ud[0], c->ud[1], c->ud[2], nb);
     t_end(c);
 }

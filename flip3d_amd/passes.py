@@ -45,7 +45,7 @@ PASSES = [
        ("flags",), r"^k_flags$", "flags", r"^k_flags$"),
     _p("p2g", "P2G + save_grid + first enforce_boundary",
        lambda u: 24 * u["Np"] + 4 * u["C"] + 4 * u["F"] + 8 * u["F"] + (8 * u["F"] + u["C"]),
-       ("p2g_prep", "p2g"), r"^k_p2g", "p2g", r"^k_p2g_prep"),
+       ("p2g_prep", "p2g"), r"^k_p2g", "p2g", r"^k_p2g_lds<0>"),
     _p("divergence", "div",
        lambda u: 4 * u["F"] + 5 * u["C"],
        ("divergence",), r"^k_divergence$", "divergence", r"^k_divergence$"),
@@ -58,10 +58,10 @@ PASSES = [
        r"^k_pcg_a$|^k_fcg_a$"),
     _p("gradient", "grad-sub + second enforce_boundary",
        lambda u: 8 * u["C"] + 8 * u["F"] + (8 * u["F"] + u["C"]),
-       ("gradient",), r"^k_gradient", "gradient", r"^k_gradient<0>"),
+       ("gradient",), r"^k_gradient", "gradient", r"^k_gradient3$|^k_gradient<0>"),
     _p("extrapolate", "extrap + subtract_grid",
        lambda u: u["C"] + 8 * u["F"] + 12 * u["F"],
-       ("extrapolate",), r"^k_extrapolate", "extrapolate", r"^k_extrapolate<0>"),
+       ("extrapolate",), r"^k_extrapolate", "extrapolate", r"^k_extrapolate3$|^k_extrapolate<0>"),
     _p("g2p_flip", "G2P + FLIP",
        lambda u: 36 * u["Np"] + 8 * u["F"],
        ("g2p_flip",), r"^k_g2p_flip", "g2p_flip", r"^k_g2p_flip"),
