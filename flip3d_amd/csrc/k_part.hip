@@ -466,20 +466,28 @@ __host__ __device__ __forceinline__ void p2g_bricks(int n, int &nbi, int &nbj, i
     nbk = (n + (COMP == 2) + P2G_BK - 1) / P2G_BK;
 }
 
+struct P2GLds {
+    float4 sp[P2G_CAP];
+    int loff[P2G_ROWS][P2G_RK + 1];
+    int rbase[P2G_ROWS];
+    int total;
+};
+
+// one brick of component COMP (brick = the block's brick index)
 template <int COMP>
-__global__ __launch_bounds__(P2G_TB) void k_p2g_lds(int n, const float4 *__restrict__ pk,
-                                                    const int32_t *__restrict__ cs, const int8_t *__restrict__ A,
-                                                    float *__restrict__ ug, float *__restrict__ us, float hh)
+__device__ __forceinline__ void p2g_brick(int n, const float4 *__restrict__ pk, const int32_t *__restrict__ cs,
+                                          const int8_t *__restrict__ A, float *__restrict__ ug,
+                                          float *__restrict__ us, float hh, int brick, P2GLds &S)
 {
-    __shared__ float4 sp[P2G_CAP];
-    __shared__ int loff[P2G_ROWS][P2G_RK + 1];
-    __shared__ int rbase[P2G_ROWS];
-    __shared__ int total;
+    float4 *sp = S.sp;
+    auto &loff = S.loff;
+    int *rbase = S.rbase;
+    int &total = S.total;
     const int ni = n + (COMP == 0), nj = n + (COMP == 1), nk = n + (COMP == 2);
     const int ihi = COMP == 0 ? 0 : 1, jhi = COMP == 1 ? 0 : 1, khi_off = COMP == 2 ? 0 : 1;
     int nbi, nbj, nbk;
     p2g_bricks<COMP>(n, nbi, nbj, nbk);
-    const int bk = blockIdx.x % nbk, bj = (blockIdx.x / nbk) % nbj, bi = blockIdx.x / (nbk * nbj);
+    const int bk = brick % nbk, bj = (brick / nbk) % nbj, bi = brick / (nbk * nbj);
     const int i0 = bi * P2G_BI, j0 = bj * P2G_BJ, k0 = bk * P2G_BK;
     // rows (si, sj) = (i0 - 1 + r / RJ, j0 - 1 + r % RJ); cells sklo..skhi
     const int RI = P2G_BI + 1 + ihi, RJ = P2G_BJ + 1 + jhi;
@@ -544,6 +552,24 @@ __global__ __launch_bounds__(P2G_TB) void k_p2g_lds(int n, const float4 *__restr
         }
     }
     p2g_store<COMP>(n, i, j, k, sumw, sumx, A, ug, us);
+}
+
+// the three components in one launch: nb0 bricks of X faces, then nb1 of Y
+// faces, then those of Z faces
+__global__ __launch_bounds__(P2G_TB) void k_p2g_lds3(int n, const float4 *__restrict__ p0,
+                                                     const float4 *__restrict__ p1, const float4 *__restrict__ p2,
+                                                     const int32_t *__restrict__ cs, const int8_t *__restrict__ A,
+                                                     float *u0, float *u1, float *u2, float *s0, float *s1, float *s2,
+                                                     float hh, int nb0, int nb1)
+{
+    __shared__ P2GLds S;
+    const int b = blockIdx.x;
+    if (b < nb0)
+        p2g_brick<0>(n, p0, cs, A, u0, s0, hh, b, S);
+    else if (b < nb0 + nb1)
+        p2g_brick<1>(n, p1, cs, A, u1, s1, hh, b - nb0, S);
+    else
+        p2g_brick<2>(n, p2, cs, A, u2, s2, hh, b - nb0 - nb1, S);
 }
 
 // mapper.cpp:17-88 one face component per launch.  COMP selects the face
@@ -1985,14 +2011,14 @@ void k_launch_p2g(FlipCtx *c, bool records_ready)
         t_begin(c, "p2g");
         int bi, bj, bk;
         p2g_bricks<0>(c->n, bi, bj, bk);
-        hipLaunchKernelGGL(k_p2g_lds<0>, bi * bj * bk, P2G_TB, 0, c->stream, c->n, c->spos, c->cell_start, c->A,
-                           c->ug[0], c->us[0], hh);
+        const int nb0 = bi * bj * bk;
         p2g_bricks<1>(c->n, bi, bj, bk);
-        hipLaunchKernelGGL(k_p2g_lds<1>, bi * bj * bk, P2G_TB, 0, c->stream, c->n, c->tmp0, c->cell_start, c->A,
-                           c->ug[1], c->us[1], hh);
+        const int nb1 = bi * bj * bk;
         p2g_bricks<2>(c->n, bi, bj, bk);
-        hipLaunchKernelGGL(k_p2g_lds<2>, bi * bj * bk, P2G_TB, 0, c->stream, c->n, c->tmp1, c->cell_start, c->A,
-                           c->ug[2], c->us[2], hh);
+        const int nb2 = bi * bj * bk;
+        hipLaunchKernelGGL(k_p2g_lds3, nb0 + nb1 + nb2, P2G_TB, 0, c->stream, c->n, c->spos, c->tmp0, c->tmp1,
+                           c->cell_start, c->A, c->ug[0], c->ug[1], c->ug[2], c->us[0], c->us[1], c->us[2], hh, nb0,
+                           nb1);
         t_end(c);
         return;
     }

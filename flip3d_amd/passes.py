@@ -45,7 +45,7 @@ PASSES = [
        ("flags",), r"^k_flags$", "flags", r"^k_flags$"),
     _p("p2g", "P2G + save_grid + first enforce_boundary",
        lambda u: 24 * u["Np"] + 4 * u["C"] + 4 * u["F"] + 8 * u["F"] + (8 * u["F"] + u["C"]),
-       ("p2g_prep", "p2g"), r"^k_p2g", "p2g", r"^k_p2g_lds<0>"),
+       ("p2g_prep", "p2g"), r"^k_p2g", "p2g", r"^k_p2g_lds3$|^k_p2g_lds<0>"),
     _p("divergence", "div",
        lambda u: 4 * u["F"] + 5 * u["C"],
        ("divergence",), r"^k_divergence$", "divergence", r"^k_divergence$"),

@@ -44,7 +44,7 @@ def test_p2g_credit_at_benchmark_size():
 @pytest.mark.parametrize("sym,want", [
     ("k_p2g_lds<0>(int, HIP_vector_type<float, 4u> const*, int const*, signed char const*, float*, float*, float)",
      "p2g"),
-    ("k_p2g_prep_unit", "p2g"), ("k_sort_gather", "sort"), ("k_sort_order_big", "sort"), ("k_fill_i32", "sort"),
+    ("k_p2g_prep_unit", "p2g"), ("k_p2g_lds3", "p2g"), ("k_sort_gather", "sort"), ("k_sort_order_big", "sort"), ("k_fill_i32", "sort"),
     ("k_density<true, true>", "density"), ("k_density_lds<true>", "density"), ("k_flags", "flags"),
     ("k_divergence", "divergence"),
     ("k_pcg_a", "solver_iteration"), ("k_pcg_b<true>", "solver_iteration"), ("k_fcg_b_pre0", "solver_iteration"),
