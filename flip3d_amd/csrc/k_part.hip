@@ -1490,27 +1490,23 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
 // round trip; every launch returns at once when nothing is listed.
 constexpr int JB = RNG_CHUNK;  // block of the single-block jitter kernels (one draw per lane)
 
-constexpr int OTB = 256;  // block of the ranking kernel (one listed particle per thread)
+constexpr int OTB = 256;  // block of the ranking kernel (one listed particle per wave)
 __global__ __launch_bounds__(OTB) void k_jitter_order(const int32_t *__restrict__ mcount,
                                                       const int32_t *__restrict__ trip, int32_t *slots,
                                                       int32_t *events)
 {
-    __shared__ int32_t ids[OTB];
     const int m = mcount[0];
-    for (int b = blockIdx.x * OTB; b < m; b += gridDim.x * OTB) {  // uniform over the block
-        const int t = b + (int)threadIdx.x;
-        const int32_t id = t < m ? trip[3 * t] : 0;
-        int rank = 0;
-        for (int t0 = 0; t0 < m; t0 += OTB) {
-            __syncthreads();
-            ids[threadIdx.x] = t0 + (int)threadIdx.x < m ? trip[3 * (t0 + threadIdx.x)] : 0x7fffffff;
-            __syncthreads();
-#pragma unroll 16
-            for (int u = 0; u < OTB; ++u) rank += ids[u] < id;  // creation indices are unique
-        }
-        if (t < m) {
-            slots[rank] = trip[3 * t + 1];
-            events[rank] = trip[3 * t + 2];
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (OTB / 64);
+    for (int t = blockIdx.x * (OTB / 64) + (int)(threadIdx.x >> 6); t < m; t += waves) {  // uniform over the wave
+        const int32_t id = trip[3 * t];
+        int below = 0;
+        for (int u = lane; u < m; u += 64) below += trip[3 * u] < id;  // creation indices are unique
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o, 64);
+        if (lane == 0) {
+            slots[below] = trip[3 * t + 1];
+            events[below] = trip[3 * t + 2];
         }
     }
 }
@@ -2211,7 +2207,7 @@ int k_correct(FlipCtx *c)
     rng_to_device(c);
     launch_timed(c, "jitter_count", k_spring_events, 256, FIX_TB, c->ev_count, c->ev_list, n, c->P4(),
                  c->cell_start, re);
-    launch_timed(c, "jitter_order", k_jitter_order, 64, OTB, c->ev_count, c->ev_list, c->ev_slot, c->ev_off);
+    launch_timed(c, "jitter_order", k_jitter_order, 256, OTB, c->ev_count, c->ev_list, c->ev_slot, c->ev_off);
     launch_timed(c, "jitter_draw", k_jitter_draw, 1, JB, c->ev_count, c->ev_off, c->ev_off, c->ev_rand,
                  (long)(3 * c->ev_cap), c->rng_dev, c->rng_table);
     launch_timed(c, "jitter_fix", k_spring_fix_wave, 256, FIX_TB, c->ev_count, c->ev_slot, c->ev_off, n, c->P4(),
