@@ -1110,6 +1110,7 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_resample_levels(const int32_t
                                                                    const int32_t *__restrict__ cs, float hh)
 {
     __shared__ float4 terms[RL_WAVES][RL_CAP];
+    __shared__ unsigned long long present[RL_WAVES][RL_CAP / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nlev = nlev_p[0];
     for (int l = 0; l < nlev; ++l) {
@@ -1132,7 +1133,10 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_resample_levels(const int32_t
             float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
             for (int base = 0; base < total; base += RL_CAP) {
                 const int cnt = min(RL_CAP, total - base);
-                for (int k = lane; k < cnt; k += 64) {
+                for (int k0 = 0; k0 < cnt; k0 += 64) {
+                    const int k = k0 + lane;
+                    bool has = false;
+                    if (k < cnt) {
                     // window position base + k -> storage slot (rows in window order)
                     int rem = base + k, a = -1;
 #pragma unroll
@@ -1149,18 +1153,26 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_resample_levels(const int32_t
                         const float w = q.w * sharp_kernel_hh(d2, hh);
                         const float4 u = vel[a];
                         t = make_float4(w * u.x, w * u.y, w * u.z, w);
+                        has = true;
                     }
                     terms[wave][k] = t;
+                    }
+                    const unsigned long long m = __ballot(has);
+                    if (lane == 0) present[wave][k0 / 64] = m;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0) {
-                    for (int k = 0; k < cnt; ++k) {
-                        const float4 t = terms[wave][k];
-                        ax += t.x;
-                        ay += t.y;
-                        az += t.z;
-                        wsum += t.w;
+                    // the terms in window order, zero terms (outside the
+                    // support, or WALL) skipped
+                    for (int k0 = 0; k0 < cnt; k0 += 64) {
+                        for (unsigned long long left = present[wave][k0 / 64]; left; left &= left - 1) {
+                            const float4 t = terms[wave][k0 + __builtin_ctzll(left)];
+                            ax += t.x;
+                            ay += t.y;
+                            az += t.z;
+                            wsum += t.w;
+                        }
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1713,6 +1725,9 @@ __global__ __launch_bounds__(FIX_TB) void k_spring_fix_wave(const int32_t *__res
                 v2 = cw * (double)nr.z;
             }
             events += __popcll(jm);
+            // only the lanes with a term are visited, lowest lane (window
+            // position) first
+            const unsigned long long nz = __ballot(kind != 0);
             tk[wv][lane] = (int8_t)kind;
             tv[wv][lane][0] = v0;
             tv[wv][lane][1] = v1;
@@ -1720,8 +1735,8 @@ __global__ __launch_bounds__(FIX_TB) void k_spring_fix_wave(const int32_t *__res
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
             if (lane == 0) {
-                const int lim = min(64, len - e0);
-                for (int u = 0; u < lim; ++u) {
+                for (unsigned long long left = nz; left; left &= left - 1) {
+                    const int u = __builtin_ctzll(left);
                     const int kd = tk[wv][u];
                     if (kd == 1) {
                         sx += (float)tv[wv][u][0];
