@@ -228,7 +228,9 @@ void ensure_particle_capacity(FlipCtx *c, int64_t need);  // grow, keeping the p
 void set_emitter(FlipCtx *c, float x, float z, float r, int pour_time);
 // exclusive scan, out[n] = total; `blocks` holds n / 2048 + 1 block sums
 // (default: the context's scratch, sized for max(particles, cells))
-void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks = nullptr);
+// gate (optional, device): the scan does nothing while *gate == 0
+void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks = nullptr,
+                   const int32_t *gate = nullptr);
 // surface mesher (k_mesh.hip)
 void mesh_build(FlipCtx *c, int mg, long *nv, long *nf);
 void mesh_copy_out(FlipCtx *c, double *v, double *nrm, int32_t *faces);

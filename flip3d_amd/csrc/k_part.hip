@@ -2183,7 +2183,8 @@ void k_stuck_and_reposition(FlipCtx *c)
         t_begin(c, "reposition");
         hipLaunchKernelGGL(k_deep, nblk(c->C, TB), TB, 0, c->stream, c->C, n, c->A, c->cell_count,
                            c->ev_count + EV_STUCK);
-        k_launch_scan(c, c->cell_count, c->scan_tmp, c->C);
+        // skipped on the device when no particle is stuck
+        k_launch_scan(c, c->cell_count, c->scan_tmp, c->C, nullptr, c->ev_count + EV_STUCK);
         hipLaunchKernelGGL(k_deep_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->cell_count, c->scan_tmp,
                            c->ev_count + EV_STUCK, c->ev_off);
         rng_to_device(c);

@@ -60,8 +60,11 @@ __device__ __forceinline__ int block_excl_scan(int v, int *lds, int *total)
     return off + incl - v;
 }
 
-__global__ __launch_bounds__(SB) void k_scan_reduce(const int32_t *__restrict__ in, long n, int32_t *bsum)
+// gate (optional): the scan does nothing when *gate == 0
+__global__ __launch_bounds__(SB) void k_scan_reduce(const int32_t *__restrict__ in, long n, int32_t *bsum,
+                                                    const int32_t *__restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ int lds[SB / 64];
     long base = (long)blockIdx.x * TILE;
     int s = 0;
@@ -76,8 +79,10 @@ __global__ __launch_bounds__(SB) void k_scan_reduce(const int32_t *__restrict__ 
 }
 
 // single block: exclusive scan of nb block sums in place; out_total <- sum
-__global__ __launch_bounds__(SB) void k_scan_top(int32_t *bsum, int nb, int32_t *out_total)
+__global__ __launch_bounds__(SB) void k_scan_top(int32_t *bsum, int nb, int32_t *out_total,
+                                                 const int32_t *__restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ int lds[SB / 64];
     int carry = 0;
     for (int base = 0; base < nb; base += SB) {
@@ -92,8 +97,10 @@ __global__ __launch_bounds__(SB) void k_scan_top(int32_t *bsum, int nb, int32_t 
 }
 
 __global__ __launch_bounds__(SB) void k_scan_apply(const int32_t *__restrict__ in, long n,
-                                                   const int32_t *__restrict__ bsum, int32_t *out)
+                                                   const int32_t *__restrict__ bsum, int32_t *out,
+                                                   const int32_t *__restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ int lds[SB / 64];
     long base = (long)blockIdx.x * TILE + (long)threadIdx.x * SI;
     int v[SI];
@@ -283,22 +290,23 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }  // namespace
 
 namespace {
-void scan_on(hipStream_t st, const int32_t *in, int32_t *out, long n, int32_t *blocks)
+void scan_on(hipStream_t st, const int32_t *in, int32_t *out, long n, int32_t *blocks,
+             const int32_t *gate = nullptr)
 {
     if (n <= 0) {
         hipLaunchKernelGGL(k_fill_i32, 1, 1, 0, st, out, 1L, 0);
         return;
     }
     int nb = (int)((n + TILE - 1) / TILE);
-    hipLaunchKernelGGL(k_scan_reduce, nb, SB, 0, st, in, n, blocks);
-    hipLaunchKernelGGL(k_scan_top, 1, SB, 0, st, blocks, nb, out + n);
-    hipLaunchKernelGGL(k_scan_apply, nb, SB, 0, st, in, n, blocks, out);
+    hipLaunchKernelGGL(k_scan_reduce, nb, SB, 0, st, in, n, blocks, gate);
+    hipLaunchKernelGGL(k_scan_top, 1, SB, 0, st, blocks, nb, out + n, gate);
+    hipLaunchKernelGGL(k_scan_apply, nb, SB, 0, st, in, n, blocks, out, gate);
 }
 }  // namespace
 
-void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks)
+void k_launch_scan(FlipCtx *c, const int32_t *in, int32_t *out, long n, int32_t *blocks, const int32_t *gate)
 {
-    scan_on(c->stream, in, out, n, blocks ? blocks : c->scan_blocks);
+    scan_on(c->stream, in, out, n, blocks ? blocks : c->scan_blocks, gate);
 }
 
 namespace {
