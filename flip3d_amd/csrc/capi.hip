@@ -490,8 +490,9 @@ static void run_stage(FlipCtx *c, int stage, flip_step_stats *st)
         k_launch_g2p(c);
         break;
     case FLIP_STAGE_ADVECT: {
-        k_launch_clear_events(c, EV_NONFINITE_ADV, EV_NONFINITE_ADV + 1);
-        k_launch_advect(c);
+        // the advection also lists the stuck particles (flip3D.cpp:933-955)
+        k_launch_clear_events(c, EV_STUCK, EV_NONFINITE_ADV + 1);
+        k_launch_advect(c, c->prm.reposition != 0);
         c->sorted = false;
         k_stuck_and_reposition(c);  // leaves the particles sorted; counts read at the next wait
         c->sorted = true;

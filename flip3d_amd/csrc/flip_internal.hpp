@@ -218,8 +218,10 @@ void k_launch_divergence(FlipCtx *c);           // div on FLUID cells
 void k_launch_gradient(FlipCtx *c);             // ghost-fluid gradient + enforce_boundary
 void k_launch_extrapolate(FlipCtx *c);          // extrapolate_velocity + subtract_grid
 void k_launch_g2p(FlipCtx *c, bool fresh = false);  // FLIP/PIC blend; fresh: right after k_launch_extrapolate
-void k_launch_advect(FlipCtx *c);               // advect + clamp + wall push-out
-void k_stuck_and_reposition(FlipCtx *c);        // on the device; counts in ev_count[EV_STUCK..]
+// advect + clamp + wall push-out; with stuck, the stuck particles' creation
+// indices are listed in ev_list (count in ev_count[EV_STUCK], cleared by the caller)
+void k_launch_advect(FlipCtx *c, bool stuck);
+void k_stuck_and_reposition(FlipCtx *c);  // the listed stuck particles; counts in ev_count[EV_STUCK..]
 void k_launch_clear_events(FlipCtx *c, int lo, int hi);  // ev_count[lo, hi) = 0 on the stream
 void k_launch_step_record(FlipCtx *c);  // the event counts and the time step into c->d_rec (mapped host memory)
 int k_correct(FlipCtx *c);                      // corrector; returns jitter events

@@ -766,17 +766,38 @@ __global__ void k_g2p_points(int64_t m, int n, const float4 *__restrict__ p, con
 }
 
 // ---------------------------------------------------------------- advect
+// flip3D.cpp:933-955 stuck test of a moved FLUID particle: in a WALL cell, or
+// thin (dens < 0.04) next to a wall in y.
+__device__ __forceinline__ bool is_stuck(int n, const float4 &p, float dens, const int8_t *__restrict__ A)
+{
+    bool rem = A[cidx(n, cell_of_b(n, p.x), cell_of_b(n, p.y), cell_of_b(n, p.z))] == WALL;
+    float g = (float)n;
+    int ii = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.x * g)));
+    int jj = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.y * g)));
+    int kk = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.z * g)));
+    // for n < 5 the reference indexes out of range; keep the window inside
+    ii = min(max(ii, 0), n - 1);
+    jj = min(max(jj, 0), n - 1);
+    kk = min(max(kk, 0), n - 1);
+    int jm = jj - 1 > 0 ? jj - 1 : 0, jp = jj + 1 < n - 1 ? jj + 1 : n - 1;
+    if ((double)dens < 0.04 && (A[cidx(n, ii, jm, kk)] == WALL || A[cidx(n, ii, jp, kk)] == WALL)) rem = true;
+    return rem;
+}
+
 // flip3D.cpp:877-930: p += dt*G2P(u) (double), clamp to [1/N, 1-1/N], then
 // push out of wall particles within 1.5*DENSITY/N, in window order, updating
 // p in place and projecting out the normal velocity.  Wall particles never
 // move, so the static wall bins are the reference's post-advect wall buckets.
+// With slist, the stuck particles' creation indices are appended to it
+// (unordered; the reposition plan orders them).
 __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__restrict__ pos, float4 *__restrict__ vel,
                                                const float *__restrict__ u0, const float *__restrict__ u1,
                                                const float *__restrict__ u2, const double *__restrict__ dtv,
                                                float rlo, double re,
                                                const int32_t *__restrict__ wst, const int8_t *__restrict__ wnear,
                                                const float4 *__restrict__ wpos, const float4 *__restrict__ wnrm,
-                                               int32_t *nonfinite)
+                                               int32_t *nonfinite, const int32_t *__restrict__ meta,
+                                               const int8_t *__restrict__ A, int32_t *slist, int32_t *scount)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
@@ -829,31 +850,10 @@ __global__ __launch_bounds__(TB) void k_advect(int64_t np, int n, float4 *__rest
     }
     pos[i] = p;
     vel[i] = v;
-}
-
-// flip3D.cpp:933-955 stuck flags: in a WALL cell, or thin (dens < 0.04) next
-// to a wall in y.  Appends creation indices (unordered; the host sorts them).
-__global__ void k_stuck(int64_t np, int n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                        const int32_t *__restrict__ meta, const int8_t *__restrict__ A, int32_t *list, int32_t *count)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    float4 p = pos[i];
-    if (is_wall_w(p.w)) return;
-    bool rem = A[cidx(n, cell_of_b(n, p.x), cell_of_b(n, p.y), cell_of_b(n, p.z))] == WALL;
-    float g = (float)n;
-    int ii = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.x * g)));
-    int jj = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.y * g)));
-    int kk = (int)fmin((double)(n - 3), fmax(2.0, (double)(p.z * g)));
-    // for n < 5 the reference indexes out of range; keep the window inside
-    ii = min(max(ii, 0), n - 1);
-    jj = min(max(jj, 0), n - 1);
-    kk = min(max(kk, 0), n - 1);
-    int jm = jj - 1 > 0 ? jj - 1 : 0, jp = jj + 1 < n - 1 ? jj + 1 : n - 1;
-    if ((double)vel[i].w < 0.04 && (A[cidx(n, ii, jm, kk)] == WALL || A[cidx(n, ii, jp, kk)] == WALL)) rem = true;
-    if (rem) {
-        int32_t at = atomicAdd(count, 1);
-        list[at] = meta_id(meta[i]);
+    // with slist: the stuck test (flip3D.cpp:933-955) on the moved particle
+    if (slist && is_stuck(n, p, v.w, A)) {
+        const int32_t at = atomicAdd(scount, 1);
+        slist[at] = meta_id(meta[i]);
     }
 }
 
@@ -2065,12 +2065,13 @@ void k_launch_g2p_points(FlipCtx *c, const float *g0, const float *g1, const flo
     if (m > 0) hipLaunchKernelGGL(k_g2p_points, nblk(m, TB), TB, 0, c->stream, m, c->n, p, g0, g1, g2, out);
 }
 
-void k_launch_advect(FlipCtx *c)
+void k_launch_advect(FlipCtx *c, bool stuck)
 {
     if (c->np > 0)
         launch_timed(c, "advect", k_advect, nblk(c->np, TB), TB, c->np, c->n, c->P4(), c->V4(), c->ug[0], c->ug[1],
                      c->ug[2], c->dtv, (float)(1.0 / c->n), 1.5 * c->prm.density / c->n, c->wall_start,
-                     c->wall_near, c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE_ADV);
+                     c->wall_near, c->wall_pos, c->wall_nrm, c->ev_count + EV_NONFINITE_ADV, c->M(), c->A,
+                     stuck ? c->ev_list : nullptr, c->ev_count + EV_STUCK);
 }
 
 void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hipStream_t s)
@@ -2174,12 +2175,7 @@ void k_launch_set_dt(FlipCtx *c, double dt)
 void k_stuck_and_reposition(FlipCtx *c)
 {
     int n = c->n;
-    hipLaunchKernelGGL(k_clear_events, 1, 64, 0, c->stream, c->ev_count, EV_STUCK, EV_NLEV + 1);
     if (c->prm.reposition && c->np > 0) {
-        t_begin(c, "stuck");
-        hipLaunchKernelGGL(k_stuck, nblk(c->np, TB), TB, 0, c->stream, c->np, n, c->P4(), c->V4(), c->M(), c->A,
-                           c->ev_list, c->ev_count + EV_STUCK);
-        t_end(c);
         t_begin(c, "reposition");
         hipLaunchKernelGGL(k_deep, nblk(c->C, TB), TB, 0, c->stream, c->C, n, c->A, c->cell_count,
                            c->ev_count + EV_STUCK);

@@ -65,12 +65,9 @@ PASSES = [
     _p("g2p_flip", "G2P + FLIP",
        lambda u: 36 * u["Np"] + 8 * u["F"],
        ("g2p_flip",), r"^k_g2p_flip", "g2p_flip", r"^k_g2p_flip"),
-    _p("advect", "advect",
-       lambda u: 24 * u["Np"] + 4 * u["F"],
-       ("advect",), r"^k_advect$", "advect", r"^k_advect$"),
-    _p("stuck", "stuck",
-       lambda u: 16 * u["Np"] + u["C"],
-       ("stuck",), r"^k_stuck$", "stuck", r"^k_stuck$"),
+    _p("advect", "advect + stuck flags",
+       lambda u: 24 * u["Np"] + 4 * u["F"] + (16 * u["Np"] + u["C"]),
+       ("advect", "stuck"), r"^k_advect$|^k_stuck$", "advect", r"^k_advect$"),
     _p("corrector", "corrector (springs, jitter, resample, update)",
        lambda u: 72 * u["Np"],
        ("spring", "jitter_count", "jitter_order", "jitter_draw", "jitter_fix", "resample", "correct_update"),
@@ -82,7 +79,7 @@ COEFFS = {
     "sort": (68, 8, 0, 0), "density": (16, 0, 0, 0), "flags": (4, 10, 0, 0), "p2g": (24, 5, 20, 0),
     "divergence": (0, 5, 4, 0), "solver_setup": (0, 0, 0, 103), "solver_iteration": (0, 0, 0, 117),
     "gradient": (0, 9, 16, 0), "extrapolate": (0, 1, 20, 0), "g2p_flip": (36, 0, 8, 0),
-    "advect": (24, 0, 4, 0), "stuck": (16, 1, 0, 0), "corrector": (72, 0, 0, 0),
+    "advect": (40, 1, 4, 0), "corrector": (72, 0, 0, 0),
 }
 
 

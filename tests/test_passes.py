@@ -50,7 +50,7 @@ def test_p2g_credit_at_benchmark_size():
     ("k_pcg_a", "solver_iteration"), ("k_pcg_b<true>", "solver_iteration"), ("k_fcg_b_pre0", "solver_iteration"),
     ("k_gradient<2>", "gradient"), ("k_extrapolate<1>", "extrapolate"), ("k_gradient3", "gradient"),
     ("k_extrapolate3", "extrapolate"), ("k_g2p_flip2", "g2p_flip"),
-    ("k_advect", "advect"), ("k_stuck", "stuck"), ("k_spring", "corrector"), ("k_spring_fix_wave", "corrector"),
+    ("k_advect", "advect"), ("k_stuck", "advect"), ("k_spring", "corrector"), ("k_spring_fix_wave", "corrector"),
     ("k_jitter_draw", "corrector"), ("k_resample", "corrector"), ("k_correct_update", "corrector"),
     ("k_resample_levels", None), ("k_ml_pre0", None), ("k_conv3_fwd_sk<16, 16, 1, true, 4>", None),
     ("k_pcg_check", None), ("k_scan_apply", None), ("k_pcg_begin", None), ("k_pcg_init<true>", None),
