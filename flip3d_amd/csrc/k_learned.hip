@@ -432,31 +432,6 @@ __global__ __launch_bounds__(TBL) void k_ml_resid0(const int32_t *__restrict__ i
     }
 }
 
-// dense level residual: res = b - A x on FLUID cells, 0 elsewhere
-__global__ __launch_bounds__(TBL) void k_ml_resid(const int32_t *__restrict__ fi, int R, const int8_t *__restrict__ A,
-                                                  const uint32_t *__restrict__ meta, const float *__restrict__ L,
-                                                  int subcell, double h2, const float *__restrict__ b,
-                                                  const float *__restrict__ x, float *res)
-{
-    if (fi[FI_DONE]) return;
-    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long C = (long)R * R * R;
-    if (c >= C) return;
-    if (A[c] != FLUID) {
-        res[c] = 0.0f;
-        return;
-    }
-    const uint32_t m = meta[c];
-    float xv[6];
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-        const int kd = (m >> (2 * u)) & 3;
-        const float w = x[c + (kd == NB_FLUID ? nb_offset(u, R) : 0)];
-        xv[u] = kd == NB_FLUID ? w : 0.0f;
-    }
-    res[c] = b[c] - ax_row(m, xv, x[c], c, R, L, subcell, h2);
-}
-
 // restriction: bc[I] = sum of the 8 children's residuals (zero off FLUID).
 // With Ac given (the network's level) it also writes per-block partials
 // (sum bc^2 over FLUID coarse cells, FLUID count) for the network's scaling.
@@ -498,6 +473,76 @@ __global__ __launch_bounds__(TBL) void k_ml_sum8(const int32_t *__restrict__ fi,
     if (!Ac) return;
     __shared__ double s0[TBL / 64], s1[TBL / 64];
     const bool fl = g < Cc && Ac[g] == FLUID;
+    double a2 = fl ? (double)sum * (double)sum : 0.0, cnt = fl ? 1.0 : 0.0;
+    a2 = wave_sum(a2);
+    cnt = wave_sum(cnt);
+    if ((threadIdx.x & 63) == 0) {
+        s0[threadIdx.x >> 6] = a2;
+        s1[threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tc = 0.0;
+        for (int w = 0; w < TBL / 64; ++w) {
+            ta += s0[w];
+            tc += s1[w];
+        }
+        part[2 * blockIdx.x] = ta;
+        part[2 * blockIdx.x + 1] = tc;
+    }
+}
+
+// The dense level residual b - A x on the fine level Rf (zero off FLUID)
+// restricted into the coarse level Rc in one pass: eight threads per coarse
+// cell form its children's residuals, and the first of them adds them in
+// k_ml_sum8's order (children inside the fine grid, d = 0..7), so bc is bit
+// for bit the restriction of the stored residual; the fine residual itself is
+// not stored.  With Ac: per-block partials as k_ml_sum8's, one per 32 coarse
+// cells (TBL / 8).
+__global__ __launch_bounds__(TBL) void k_ml_resid_sum8(const int32_t *__restrict__ fi, int Rc, int Rf,
+                                                       const int8_t *__restrict__ A,
+                                                       const uint32_t *__restrict__ meta,
+                                                       const float *__restrict__ L, int subcell, double h2,
+                                                       const float *__restrict__ b, const float *__restrict__ x,
+                                                       float *bc, const int8_t *__restrict__ Ac, double *part)
+{
+    if (fi[FI_DONE]) return;
+    const long g = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int d = threadIdx.x & 7;
+    const long Cc = (long)Rc * Rc * Rc;
+    float res = 0.0f;
+    int inside = 0;
+    if (g < Cc) {
+        const IJK p = ijk_of(g, Rc);
+        const int ci = 2 * p.i + (d >> 2), cj = 2 * p.j + ((d >> 1) & 1), ck = 2 * p.k + (d & 1);
+        inside = ci < Rf && cj < Rf && ck < Rf;
+        if (inside) {
+            const long c = ((long)ci * Rf + cj) * Rf + ck;
+            if (A[c] == FLUID) {
+                const uint32_t m = meta[c];
+                float xv[6];
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    const int kd = (m >> (2 * u)) & 3;
+                    const float w = x[c + (kd == NB_FLUID ? nb_offset(u, Rf) : 0)];
+                    xv[u] = kd == NB_FLUID ? w : 0.0f;
+                }
+                res = b[c] - ax_row(m, xv, x[c], c, Rf, L, subcell, h2);
+            }
+        }
+    }
+    const int base = (threadIdx.x & 63) & ~7;
+    float sum = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float r = __shfl(res, base + e, 64);
+        if (__shfl(inside, base + e, 64)) sum += r;
+    }
+    const bool lead = g < Cc && d == 0;
+    if (lead) bc[g] = sum;
+    if (!Ac) return;
+    __shared__ double s0[TBL / 64], s1[TBL / 64];
+    const bool fl = lead && Ac[g] == FLUID;
     double a2 = fl ? (double)sum * (double)sum : 0.0, cnt = fl ? 1.0 : 0.0;
     a2 = wave_sum(a2);
     cnt = wave_sum(cnt);
@@ -811,7 +856,7 @@ static LearnedSolver *learned_get(FlipCtx *c)
         dmalloc(&ls->part, 4 * GMAX);
         dmalloc(&ls->fs, 8);
         dmalloc(&ls->fi, 4);
-        dmalloc(&ls->upart, 2 * (long)nb(ls->lv[ls->K].C));
+        dmalloc(&ls->upart, 2 * (long)nb(8 * ls->lv[ls->K].C));
         FLIP_CHECK(hipHostMalloc((void **)&ls->h_flag, sizeof(int32_t) * 2, hipHostMallocMapped));
         FLIP_CHECK(hipHostGetDevicePointer((void **)&ls->d_flag, ls->h_flag, 0));
         FLIP_CHECK(hipEventCreateWithFlags(&ls->ev[0], hipEventDisableTiming));
@@ -852,10 +897,8 @@ static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell,
                                l == K ? L.A : nullptr, ls->upart);
         } else {
             Level &F = ls->lv[l - 1];
-            launch_timed(c, "ml_resid", k_ml_resid, nb(F.C), TBL, fi, F.R, F.A, F.meta, F.L, subcell, F.h2, F.b,
-                               F.x, F.t);
-            launch_timed(c, "ml_sum8", k_ml_sum8, nb(L.C), TBL, fi, L.R, F.R, F.t, L.b, l == K ? L.A : nullptr,
-                               ls->upart);
+            launch_timed(c, "ml_resid", k_ml_resid_sum8, nb(8 * L.C), TBL, fi, L.R, F.R, F.A, F.meta, F.L, subcell,
+                         F.h2, F.b, F.x, L.b, l == K ? L.A : nullptr, ls->upart);
         }
         if (l < K)
             launch_timed(c, "ml_pre", k_ml_pre, nb(L.C), TBL, fi, L.R, L.A, L.meta, L.L, subcell, L.h2, L.b,
@@ -863,7 +906,10 @@ static void vcycle(FlipCtx *c, LearnedSolver *ls, int G, double h2, int subcell,
     }
     Level &KL = ls->lv[K];
     t_begin(c, "unet");
-    unet_infer_layers(ls->net, KL.A, KL.L, KL.b, ls->upart, unet_partials_count(ls->net), fi + FI_DONE, KL.x,
+    // the network's scaling partials: k_ml_resid_sum8's (one per 32 coarse
+    // cells) below two levels, else k_ml_sum8's
+    const int nparts = K >= 2 ? (int)nb(8 * KL.C) : unet_partials_count(ls->net);
+    unet_infer_layers(ls->net, KL.A, KL.L, KL.b, ls->upart, nparts, fi + FI_DONE, KL.x,
                       ls->sigma);
     t_end(c);
     // up: correct level l with level l+1, smooth
