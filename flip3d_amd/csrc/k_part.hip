@@ -1332,9 +1332,10 @@ __device__ __forceinline__ void spring_centre(int n, const float4 &p, float &cx,
     ck = (int)z;
 }
 
-// corrector.cpp:55-88 spring for one particle.  RAND == false: stops at the
-// first random-jitter event and reports it; RAND == true: consumes the
-// host-drawn rand()%101 values in window order.
+// corrector.cpp:55-88 spring for one particle.  RAND == false: at the first
+// random-jitter event stops forming the spring, counts the window's events
+// and returns minus their number; RAND == true: consumes the host-drawn
+// rand()%101 values in window order.
 template <bool RAND>
 __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__restrict__ pos,
                                           const int32_t *__restrict__ meta, const float4 *__restrict__ nrm_id,
@@ -1356,6 +1357,7 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
     // keeps the test conservative.
     const float far2 = re * re * 1.0001f;
     const WinCull wc(n, cx, cy, cz, ci, cj, ck, far2);
+    bool bail = false;
     for (int si = ci - 1; si <= ci + 1; ++si) {
         if (si < 0 || si > n - 1) continue;
         for (int sj = cj - 1; sj <= cj + 1; ++sj) {
@@ -1366,11 +1368,14 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
             wc.cells(n, ck, dij, klo, khi);
             if (klo > khi) continue;
             int b = cs[cidx(n, si, sj, klo)], e = cs[cidx(n, si, sj, khi) + 1];
-            bool bail = false;
             for_range4(pos, b, e, [&](const float4 &q, int a) {
-                if (bail || a == i) return;
+                if (a == i) return;
                 if (length2(p.x, p.y, p.z, q.x, q.y, q.z) > far2) return;
                 float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+                if (!RAND && bail) {
+                    events += !((double)dist > rre) && !is_wall_w(q.w);
+                    return;
+                }
                 float w = (float)(50.0 * (double)mass_of(q.w) * (double)smooth_kernel(dist * dist, re));
                 if ((double)dist > rre) {
                     sx += w * (p.x - q.x) / dist * re;
@@ -1379,6 +1384,7 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
                 } else if (!is_wall_w(q.w)) {
                     if (!RAND) {
                         bail = true;
+                        events = 1;
                         return;
                     }
                     sx = (float)((double)sx + cj_ * (double)rv[3 * events + 0] / 100.0);
@@ -1392,9 +1398,9 @@ __device__ __forceinline__ int spring_one(int64_t i, int n, const float4 *__rest
                     sz = (float)((double)sz + cw * (double)nr.z);
                 }
             });
-            if (bail) return -1;
         }
     }
+    if (!RAND && bail) return -events;
     tmp0[i] = make_float4(p.x + dt * sx, p.y + dt * sy, p.z + dt * sz, 0.0f);
     return events;
 }
@@ -1468,6 +1474,10 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
             const int a = near[k][tid];
             const float4 q = pos[a];
             float dist = length3(p.x, p.y, p.z, q.x, q.y, q.z);
+            if (r < 0) {  // after the first event: only count the others
+                r -= !((double)dist > rre) && !is_wall_w(q.w);
+                continue;
+            }
             float w = (float)(50.0 * (double)mass_of(q.w) * (double)smooth_kernel(dist * dist, re));
             if ((double)dist > rre) {
                 sx += w * (p.x - q.x) / dist * re;
@@ -1475,7 +1485,6 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
                 sz += w * (p.z - q.z) / dist * re;
             } else if (!is_wall_w(q.w)) {
                 r = -1;  // a random-jitter event: the fix-up pass replays this particle
-                break;
             } else {
                 float4 nr = nrm_id[meta_id(meta[a])];
                 sx = (float)((double)sx + cw * (double)nr.x);
@@ -1485,16 +1494,18 @@ __global__ __launch_bounds__(SPRING_TB) void k_spring(int64_t np, int n, const f
         }
         if (r == 0) tmp0[i] = make_float4(p.x + dt * sx, p.y + dt * sy, p.z + dt * sz, 0.0f);
     }
-    if (r < 0) {
+    if (r < 0) {  // listed with its number of jitter events, -r
         int32_t at = atomicAdd(count, 1);
         list[3 * at] = meta_id(meta[i]);
         list[3 * at + 1] = (int32_t)i;
+        list[3 * at + 2] = -r;
     }
 }
 
 // ------------------------------------------- corrector jitter events
 // The spring pass lists the particles with a jitter event as (creation index,
-// slot, event count) triples, count in mcount[0].  The reference draws 3
+// slot, event count) triples, count in mcount[0]; the event count is every
+// FLUID neighbour within 0.1 re (corrector.cpp:73-77).  The reference draws 3
 // rand()%101 per event in particle order at one thread (corrector.cpp:73-77);
 // here the listed particles are ranked by creation index (k_jitter_order),
 // the draws made in that order from the device copy of the generator
@@ -1625,39 +1636,7 @@ struct SpringWindow {
     }
 };
 
-// the jitter events of each listed particle (FLUID neighbours within 0.1 re,
-// corrector.cpp:73-77) into the third slot of its triple, one wave each
-constexpr int FIX_TB = 256;
-__global__ __launch_bounds__(FIX_TB) void k_spring_events(const int32_t *__restrict__ mcount, int32_t *trip, int n,
-                                                          const float4 *__restrict__ pos,
-                                                          const int32_t *__restrict__ cs, float re)
-{
-    const int m = mcount[0];
-    const int lane = threadIdx.x & 63;
-    const int nw = gridDim.x * (FIX_TB / 64);
-    const double rre = 0.1 * (double)re;
-    const float far2 = re * re * 1.0001f;
-    for (int t = blockIdx.x * (FIX_TB / 64) + (threadIdx.x >> 6); t < m; t += nw) {
-        const int64_t i = trip[3 * t + 1];
-        const SpringWindow win(i, n, pos, cs, far2);
-        int ev = 0;
-        for (int e0 = 0; e0 < win.len; e0 += 64) {
-            bool jit = false;
-            if (e0 + lane < win.len) {
-                const int a = win.at(e0 + lane);
-                if (a != i) {
-                    const float4 q = pos[a];
-                    if (!(length2(win.p.x, win.p.y, win.p.z, q.x, q.y, q.z) > far2)) {
-                        const float dist = length3(win.p.x, win.p.y, win.p.z, q.x, q.y, q.z);
-                        jit = !((double)dist > rre) && !is_wall_w(q.w);
-                    }
-                }
-            }
-            ev += __popcll(__ballot(jit));
-        }
-        if (lane == 0) trip[3 * t + 2] = ev;
-    }
-}
+constexpr int FIX_TB = 256;  // block of the fix-up kernel (one listed particle per wave)
 
 // Listed particles' springs with the jitter draws, one wave per particle:
 // lanes form the terms of 64 window particles at a time (window order; the
@@ -2217,8 +2196,6 @@ int k_correct(FlipCtx *c)
     // the jitter events in creation order with the reference's rand() draws,
     // on the device (corrector.cpp:74-76 at one thread)
     rng_to_device(c);
-    launch_timed(c, "jitter_count", k_spring_events, 256, FIX_TB, c->ev_count, c->ev_list, n, c->P4(),
-                 c->cell_start, re);
     launch_timed(c, "jitter_order", k_jitter_order, 256, OTB, c->ev_count, c->ev_list, c->ev_slot, c->ev_off);
     launch_timed(c, "jitter_draw", k_jitter_draw, 1, JB, c->ev_count, c->ev_off, c->ev_off, c->ev_rand,
                  (long)(3 * c->ev_cap), c->rng_dev, c->rng_table);
