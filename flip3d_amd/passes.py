@@ -70,7 +70,7 @@ PASSES = [
        ("advect", "stuck"), r"^k_advect$|^k_stuck$", "advect", r"^k_advect$"),
     _p("corrector", "corrector (springs, jitter, resample, update)",
        lambda u: 72 * u["Np"],
-       ("spring", "jitter_count", "jitter_order", "jitter_draw", "jitter_fix", "resample", "correct_update"),
+       ("spring", "jitter_order", "jitter_draw", "jitter_fix", "resample", "correct_update"),
        r"^k_spring|^k_jitter_|^k_resample$|^k_correct_update$", "spring", r"^k_spring$"),
 ]
 
