@@ -70,6 +70,7 @@ void free_particles(FlipCtx *c)
     dfree(c->inv);
     dfree(c->spos);
     dfree(c->xyz3);
+    dfree(c->dw);
     dfree(c->tmp0);
     dfree(c->tmp1);
     dfree(c->pkey);
@@ -102,6 +103,7 @@ void alloc_particles(FlipCtx *c, int64_t cap)
     dalloc(&c->inv, n);
     dalloc(&c->spos, n);
     dalloc(&c->xyz3, 3 * n);
+    dalloc(&c->dw, n);
     dalloc(&c->tmp0, n);
     dalloc(&c->tmp1, n);
     dalloc(&c->pkey, n);
@@ -970,7 +972,7 @@ static std::vector<uint64_t> step_graph_sig(const FlipCtx *c)
         ptr(c->vel[b]);
         ptr(c->meta[b]);
     }
-    const void *bufs[] = {c->nrm_id,   c->inv,       c->spos,      c->xyz3,     c->tmp0,     c->tmp1,
+    const void *bufs[] = {c->nrm_id,   c->inv,       c->spos,      c->xyz3,     c->dw,       c->tmp0,     c->tmp1,
                           c->pkey,     c->pslot,     c->perm,      c->sort_big, c->deep,     c->ev_list,
                           c->ev_off,   c->ev_slot,   c->repo_ids,  c->repo_order, c->repo_loff, c->ev_rand,
                           c->wall_start, c->wall_near, c->wall_pos, c->wall_nrm};
@@ -986,10 +988,11 @@ static void step_launch_pre(FlipCtx *c)
     // flip3D.cpp:1221-1222 sort + computeDensity
     k_launch_sort(c);
     c->sorted = true;
-    // with add_ExtForce (:1230) fused in, and P2G's packed records written
+    // with add_ExtForce (:1230) fused in, and the inputs of flags and P2G
+    // written (densities with the wall mark, P2G's packed records)
     const bool records = k_launch_density(c, true, true);
     // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
-    k_launch_flags(c);
+    k_launch_flags(c, records);
     k_launch_p2g(c, records);
     k_launch_divergence(c);  // project (:807-868): div, solve, gradient
 }

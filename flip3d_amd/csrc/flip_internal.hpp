@@ -78,6 +78,7 @@ struct FlipCtx {
     // every sort: the 12-byte window records of the density and resample
     // scans, valid while `sorted` holds (positions unchanged since the sort)
     float *xyz3 = nullptr;
+    float *dw = nullptr;  // per particle: density, or -1 for WALL (the step's density pass writes it for flags)
     float4 *tmp0 = nullptr, *tmp1 = nullptr;  // corrector pseudo-position / velocity
     int32_t *pkey = nullptr, *pslot = nullptr, *perm = nullptr;
     int32_t *sort_big = nullptr;  // buckets over 16 particles: count, then cell ids
@@ -209,10 +210,11 @@ struct FlipCtx {
 // ---- kernels (launch wrappers; each enqueues on ctx->stream) ----
 void k_launch_sort(FlipCtx *c);                 // counting sort + per-cell id order + permute
 // with_force: add_ExtForce fused in; p2g_records: also write P2G's packed
-// records (returns whether it did)
+// records and the flags pass's dw (returns whether it did)
 bool k_launch_density(FlipCtx *c, bool with_force = false, bool p2g_records = false);
 void k_launch_ext_force(FlipCtx *c);
-void k_launch_flags(FlipCtx *c);                // A and L from the sorted buckets
+// A and L from the sorted buckets; dw_ready: read the density pass's dw
+void k_launch_flags(FlipCtx *c, bool dw_ready = false);
 void k_launch_p2g(FlipCtx *c, bool records_ready = false);  // P2G + save_grid + enforce_boundary
 void k_launch_divergence(FlipCtx *c);           // div on FLUID cells
 void k_launch_gradient(FlipCtx *c);             // ghost-fluid gradient + enforce_boundary

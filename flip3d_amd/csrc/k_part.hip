@@ -141,7 +141,7 @@ __global__ __launch_bounds__(TB) void k_density_lds(int64_t np, int n, const flo
                                                     float4 *__restrict__ vel, const int32_t *__restrict__ cs,
                                                     float inv_hh, float max_dens, const double *__restrict__ dtv,
                                                     double gravity, const P3 *__restrict__ xyz3, float4 *px,
-                                                    float4 *py, float4 *pz)
+                                                    float4 *py, float4 *pz, float *dw)
 {
     __shared__ P3 st[DL_CAP];
     __shared__ int skey[TB];
@@ -220,6 +220,7 @@ __global__ __launch_bounds__(TB) void k_density_lds(int64_t np, int n, const flo
         v.w = 1.0f;
         vel[i] = v;
         if (px) p2g_records_wall(i, px, py, pz);
+        if (dw) dw[i] = -1.0f;
         return;
     }
     const int klo = max(ck - 1, 0), khi = min(ck + 1, n - 1);
@@ -246,6 +247,7 @@ __global__ __launch_bounds__(TB) void k_density_lds(int64_t np, int n, const flo
     v.w = wsum / max_dens;
     vel[i] = v;
     if (px) p2g_records(i, n, p, v, px, py, pz);
+    if (dw) dw[i] = v.w;
 }
 
 // flip3D.cpp:870-875 add_ExtForce: u_y = (float)((double)u_y + gdt), gdt = -dt*g
@@ -269,7 +271,7 @@ constexpr int FLAGS_TB = 256, FLAGS_CAP = 4096;
 
 __global__ __launch_bounds__(FLAGS_TB) void k_flags(long C, const int32_t *__restrict__ cs,
                                                     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                                    int8_t *A, float *L, double c0)
+                                                    const float *__restrict__ dw, int8_t *A, float *L, double c0)
 {
     __shared__ float s_dens[FLAGS_CAP];
     __shared__ uint8_t s_wall[FLAGS_CAP];
@@ -279,8 +281,14 @@ __global__ __launch_bounds__(FLAGS_TB) void k_flags(long C, const int32_t *__res
     const bool staged = pe - pb <= FLAGS_CAP;
     if (staged) {
         for (int k = threadIdx.x; k < pe - pb; k += FLAGS_TB) {
-            s_dens[k] = vel[pb + k].w;
-            s_wall[k] = is_wall_w(pos[pb + k].w) ? 1 : 0;
+            if (dw) {  // the density pass's copy: the density, or -1 for WALL
+                const float d = dw[pb + k];
+                s_dens[k] = d;
+                s_wall[k] = d < 0.0f ? 1 : 0;
+            } else {
+                s_dens[k] = vel[pb + k].w;
+                s_wall[k] = is_wall_w(pos[pb + k].w) ? 1 : 0;
+            }
         }
     }
     __syncthreads();
@@ -1961,7 +1969,7 @@ bool k_launch_density(FlipCtx *c, bool with_force, bool p2g_records)
         launch_timed(c, name, with_force ? k_density_lds<true> : k_density_lds<false>, nblk(c->np, TB), TB, c->np,
                      c->n, c->P4(), c->V4(), c->cell_start, inv_hh, c->max_dens, c->dtv, c->prm.gravity,
                      reinterpret_cast<const P3 *>(c->xyz3), rec ? c->spos : nullptr, rec ? c->tmp0 : nullptr,
-                     rec ? c->tmp1 : nullptr);
+                     rec ? c->tmp1 : nullptr, rec ? c->dw : nullptr);
         return rec;
     }
     auto kern = pow2 ? (with_force ? k_density<true, true> : k_density<true, false>)
@@ -1979,12 +1987,12 @@ void k_launch_ext_force(FlipCtx *c)
     t_end(c);
 }
 
-void k_launch_flags(FlipCtx *c)
+void k_launch_flags(FlipCtx *c, bool dw_ready)
 {
     float d = (float)c->prm.density;
     float n0 = (float)(1.0 / (d * d * d));
-    launch_timed(c, "flags", k_flags, nblk(c->C, FLAGS_TB), FLAGS_TB, c->C, c->cell_start, c->P4(), c->V4(), c->A,
-                 c->L, 0.2 * (double)n0);
+    launch_timed(c, "flags", k_flags, nblk(c->C, FLAGS_TB), FLAGS_TB, c->C, c->cell_start, c->P4(), c->V4(),
+                 dw_ready ? c->dw : nullptr, c->A, c->L, 0.2 * (double)n0);
 }
 
 void k_launch_p2g(FlipCtx *c, bool records_ready)
