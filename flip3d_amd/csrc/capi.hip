@@ -982,6 +982,10 @@ static std::vector<uint64_t> step_graph_sig(const FlipCtx *c)
 
 // ---- the step in parts (flip_gpu_step, flip_gpu_step_batch)
 // flip3D.cpp:1194-1264 before the pressure solve
+// whether the step's passes prepare the FP32 PCG's setup (the FP64 and
+// refinement solves prepare their own)
+static bool step_solve_prepared(const FlipCtx *c) { return !c->prm.solve_fp64 && c->prm.refine <= 0; }
+
 static void step_launch_pre(FlipCtx *c)
 {
     k_launch_choose_dt(c);  // dt stays on the device; read back with the step's end
@@ -991,10 +995,13 @@ static void step_launch_pre(FlipCtx *c)
     // with add_ExtForce (:1230) fused in, and the inputs of flags and P2G
     // written (densities with the wall mark, P2G's packed records)
     const bool records = k_launch_density(c, true, true);
+    // the FP32 PCG's setup (negated div, FLUID flags) formed by the passes
+    // that produce its inputs (step_solve tells the solve)
+    const bool prep = step_solve_prepared(c);
     // solve_picflip (:1057-1103); its sort (:1059) sees unchanged positions
-    k_launch_flags(c, records);
+    k_launch_flags(c, records, prep);
     k_launch_p2g(c, records);
-    k_launch_divergence(c);  // project (:807-868): div, solve, gradient
+    k_launch_divergence(c, prep);  // project (:807-868): div, solve, gradient
 }
 
 // and after it, ending with the step record in mapped host memory
@@ -1062,6 +1069,7 @@ static void step_capture(FlipCtx *c, int which, flip_step_stats *st)
 // the pressure solve between the parts, on the context's stream
 static void step_solve(FlipCtx *c, flip_step_stats *st)
 {
+    c->solve_prepared = step_solve_prepared(c);
     FLIP_CHECK(hipEventRecord(c->ev_s0, c->stream));
     const int rc = k_solve(c, st);
     FLIP_CHECK(hipEventRecord(c->ev_s1, c->stream));

@@ -120,6 +120,10 @@ struct FlipCtx {
     bool solve_timed = false;  // ev_s0/ev_s1 bracket a solve whose time is not read yet
     bool corrected = false;    // a corrector ran; its jitter-event count is not read yet
     bool advected = false;     // an advection ran; its reposition count and checks are not read yet
+    // the step's divergence pass wrote -div (solver.cpp:231's flip) and its
+    // flags pass the FLUID flags and the non-FLUID press zeroes: the next
+    // PCG setup skips both
+    bool solve_prepared = false;
     hipGraphExec_t ic_graph = nullptr;  // captured batch of wavefront-preconditioned iterations
     long ic_graph_key = -1;             // subcell, preconditioner mode and block count it was captured for
     double *qv = nullptr;        // forward-sweep intermediate of the wavefront preconditioner
@@ -213,10 +217,11 @@ void k_launch_sort(FlipCtx *c);                 // counting sort + per-cell id o
 // records and the flags pass's dw (returns whether it did)
 bool k_launch_density(FlipCtx *c, bool with_force = false, bool p2g_records = false);
 void k_launch_ext_force(FlipCtx *c);
-// A and L from the sorted buckets; dw_ready: read the density pass's dw
-void k_launch_flags(FlipCtx *c, bool dw_ready = false);
+// A and L from the sorted buckets; dw_ready: read the density pass's dw;
+// fluid_flags: also the PCG setup's FLUID flags (cell_count) and press zeroes
+void k_launch_flags(FlipCtx *c, bool dw_ready = false, bool fluid_flags = false);
 void k_launch_p2g(FlipCtx *c, bool records_ready = false);  // P2G + save_grid + enforce_boundary
-void k_launch_divergence(FlipCtx *c);           // div on FLUID cells
+void k_launch_divergence(FlipCtx *c, bool negate = false);  // negate: write -div
 void k_launch_gradient(FlipCtx *c);             // ghost-fluid gradient + enforce_boundary
 void k_launch_extrapolate(FlipCtx *c);          // extrapolate_velocity + subtract_grid
 void k_launch_g2p(FlipCtx *c, bool fresh = false);  // FLIP/PIC blend; fresh: right after k_launch_extrapolate

@@ -16,7 +16,7 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 // FLUID) zeroes them, and they are never read before that.
 __global__ void k_divergence(int n, const int8_t *__restrict__ A, const float *__restrict__ u0,
                              const float *__restrict__ u1, const float *__restrict__ u2, float h, float *div,
-                             float *press, int clear_press)
+                             float *press, int clear_press, int negate)
 {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long C = (long)n * n * n;
@@ -29,7 +29,7 @@ __global__ void k_divergence(int n, const int8_t *__restrict__ A, const float *_
         d = d - u1[fidx1(n, i, j, k)];
         d = d + u2[fidx2(n, i, j, k + 1)];
         d = d - u2[fidx2(n, i, j, k)];
-        div[c] = d / h;
+        div[c] = negate ? -(d / h) : d / h;
     } else if (clear_press) {
         press[c] = 0.0f;
     }
@@ -154,11 +154,11 @@ __global__ void k_extrapolate3(int n, const int8_t *__restrict__ A, float *u0, f
 
 }  // namespace
 
-void k_launch_divergence(FlipCtx *c)
+void k_launch_divergence(FlipCtx *c, bool negate)
 {
     t_begin(c, "divergence");
     hipLaunchKernelGGL(k_divergence, nblk(c->C, TB), TB, 0, c->stream, c->n, c->A, c->ug[0], c->ug[1], c->ug[2],
-                       (float)(1.0 / c->n), c->div, c->press, 0);
+                       (float)(1.0 / c->n), c->div, c->press, 0, negate ? 1 : 0);
     t_end(c);
 }
 

@@ -271,7 +271,8 @@ constexpr int FLAGS_TB = 256, FLAGS_CAP = 4096;
 
 __global__ __launch_bounds__(FLAGS_TB) void k_flags(long C, const int32_t *__restrict__ cs,
                                                     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-                                                    const float *__restrict__ dw, int8_t *A, float *L, double c0)
+                                                    const float *__restrict__ dw, int8_t *A, float *L, double c0,
+                                                    int32_t *fflag, float *press)
 {
     __shared__ float s_dens[FLAGS_CAP];
     __shared__ uint8_t s_wall[FLAGS_CAP];
@@ -314,13 +315,18 @@ __global__ __launch_bounds__(FLAGS_TB) void k_flags(long C, const int32_t *__res
             accm += vel[a].w;
         }
     }
+    int8_t a = WALL;
     if (wall) {
-        A[c] = WALL;
         L[c] = 1.0f;
     } else {
         float l = (float)(c0 - (double)accm);
         L[c] = l;
-        A[c] = l < 0.0f ? FLUID : AIR;
+        a = l < 0.0f ? FLUID : AIR;
+    }
+    A[c] = a;
+    if (fflag) {  // the PCG setup's FLUID flags (k_fluid_flag)
+        fflag[c] = a == FLUID ? 1 : 0;
+        if (a != FLUID) press[c] = 0.0f;
     }
 }
 
@@ -1987,12 +1993,13 @@ void k_launch_ext_force(FlipCtx *c)
     t_end(c);
 }
 
-void k_launch_flags(FlipCtx *c, bool dw_ready)
+void k_launch_flags(FlipCtx *c, bool dw_ready, bool fluid_flags)
 {
     float d = (float)c->prm.density;
     float n0 = (float)(1.0 / (d * d * d));
     launch_timed(c, "flags", k_flags, nblk(c->C, FLAGS_TB), FLAGS_TB, c->C, c->cell_start, c->P4(), c->V4(),
-                 dw_ready ? c->dw : nullptr, c->A, c->L, 0.2 * (double)n0);
+                 dw_ready ? c->dw : nullptr, c->A, c->L, 0.2 * (double)n0, fluid_flags ? c->cell_count : nullptr,
+                 c->press);
 }
 
 void k_launch_p2g(FlipCtx *c, bool records_ready)

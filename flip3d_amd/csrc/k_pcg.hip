@@ -583,9 +583,10 @@ void launch_pair(FlipCtx *c, bool jacobi, int G, double h2, int subcell, double 
 // ordered list of FLUID cells and their neighbour metadata (zeroes non-FLUID
 // press); returns the count (host sync)
 // the list without reading its length back (the count stays at scan_tmp[C])
-static void fluid_list_async(FlipCtx *c)
+static void fluid_list_async(FlipCtx *c, bool flagged = false)
 {
-    hipLaunchKernelGGL(k_fluid_flag, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->cell_count, c->press);
+    if (!flagged)
+        hipLaunchKernelGGL(k_fluid_flag, nblk(c->C, TB), TB, 0, c->stream, c->C, c->A, c->cell_count, c->press);
     k_launch_scan(c, c->cell_count, c->scan_tmp, c->C);
     hipLaunchKernelGGL(k_fluid_compact, nblk(c->C, TB), TB, 0, c->stream, c->C, c->n, c->A, c->scan_tmp, c->fl,
                        c->flmeta);
@@ -636,7 +637,7 @@ static void build_factor_wavefront(FlipCtx *c, int mode, int subcell)
 // solve's status to mapped host memory; the host keeps the next batch queued
 // while it waits for the previous one, so the device does not idle between
 // batches.
-static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
+static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup, bool prepared = false)
 {
     const int n = c->n;
     const int subcell = c->prm.subcell;
@@ -652,8 +653,9 @@ static int pcg32(FlipCtx *c, flip_step_stats *st, bool setup)
 
     t_begin(c, "pcg_setup");
     if (setup) {
-        hipLaunchKernelGGL(k_negate, nblk(c->C, TB), TB, 0, c->stream, c->C, c->div);  // solver.cpp:231 flip(b)
-        fluid_list_async(c);
+        // solver.cpp:231 flip(b), unless the step's divergence pass wrote -div
+        if (!prepared) hipLaunchKernelGGL(k_negate, nblk(c->C, TB), TB, 0, c->stream, c->C, c->div);
+        fluid_list_async(c, prepared);
     }
     const int32_t *nfl_src = setup ? c->scan_tmp + c->C : nullptr;
     const int start = (learned || c->prm.tol_policy == 1) ? 1 : 0;
@@ -1163,7 +1165,11 @@ static int solve_refine(FlipCtx *c, flip_step_stats *st)
 // or its FP64-residual refinement, or the FP64 reference solve.
 int k_solve(FlipCtx *c, flip_step_stats *st)
 {
+    // set for this call only by the step (the FP64 and refinement solves
+    // prepare their own inputs)
+    const bool prepared = c->solve_prepared;
+    c->solve_prepared = false;
     if (c->prm.solve_fp64) return solve_fp64(c, st);
     if (c->prm.refine > 0) return solve_refine(c, st);
-    return pcg32(c, st, true);
+    return pcg32(c, st, true, prepared);
 }
