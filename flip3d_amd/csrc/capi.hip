@@ -988,10 +988,12 @@ static bool step_solve_prepared(const FlipCtx *c) { return !c->prm.solve_fp64 &&
 
 static void step_launch_pre(FlipCtx *c)
 {
-    k_launch_choose_dt(c);  // dt stays on the device; read back with the step's end
-    // flip3D.cpp:1221-1222 sort + computeDensity
-    k_launch_sort(c);
+    // flip3D.cpp:1221-1222 sort + computeDensity; the time step (kept on the
+    // device, read back with the step's end) from the speeds the sort's
+    // gather forms
+    k_launch_sort(c, c->prm.dt_policy == 1 ? c->partials : nullptr);
     c->sorted = true;
+    k_launch_choose_dt(c, true);
     // with add_ExtForce (:1230) fused in, and the inputs of flags and P2G
     // written (densities with the wall mark, P2G's packed records)
     const bool records = k_launch_density(c, true, true);

@@ -212,7 +212,9 @@ struct FlipCtx {
 };
 
 // ---- kernels (launch wrappers; each enqueues on ctx->stream) ----
-void k_launch_sort(FlipCtx *c);                 // counting sort + per-cell id order + permute
+// counting sort + per-cell id order + permute; speed_part: also the gather's
+// per-block largest FLUID |u|^2 (one per 256 particles)
+void k_launch_sort(FlipCtx *c, double *speed_part = nullptr);
 // with_force: add_ExtForce fused in; p2g_records: also write P2G's packed
 // records and the flags pass's dw (returns whether it did)
 bool k_launch_density(FlipCtx *c, bool with_force = false, bool p2g_records = false);
@@ -269,7 +271,9 @@ void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hi
 void rng_upload_table(FlipCtx *c);  // the chunked-draw coefficients (context creation)
 void rng_to_device(FlipCtx *c);  // the rand() state to the device, if it is on the host
 void rng_to_host(FlipCtx *c);    // and back (waits for the stream)
-void k_launch_choose_dt(FlipCtx *c);            // the step's dt into dtv[0] on the device
+// the step's time step on the device; from_sort: the speeds from the sort's
+// gather (k_launch_sort with c->partials) instead of k_speed2
+void k_launch_choose_dt(FlipCtx *c, bool from_sort = false);
 void k_launch_set_dt(FlipCtx *c, double dt);    // a given dt into dtv[0]
 
 // records the message for flip_gpu_last_error and returns code

@@ -2142,12 +2142,16 @@ void k_launch_step_record(FlipCtx *c)
 }
 
 // the step's time step into c->dtv[0] on the device (no host round trip)
-void k_launch_choose_dt(FlipCtx *c)
+void k_launch_choose_dt(FlipCtx *c, bool from_sort)
 {
     unsigned nb = 0;
     if (c->prm.dt_policy == 1 && c->np > 0) {
-        nb = std::min(nblk(c->np, TB), 1024u);
-        hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), c->partials);
+        if (from_sort) {
+            nb = nblk(c->np, 256);  // the gather's blocks
+        } else {
+            nb = std::min(nblk(c->np, TB), 1024u);
+            hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), c->partials);
+        }
     }
     hipLaunchKernelGGL(k_choose_dt, 1, TB, 0, c->stream, (int)nb, c->partials, c->prm.dt, c->prm.cfl, c->n, c->dtv);
 }

@@ -262,24 +262,41 @@ __global__ __launch_bounds__(OB) void k_sort_order_big(const int32_t *__restrict
     }
 }
 
-// 5. permute particle state
-__global__ void k_sort_gather(int64_t np, const int32_t *__restrict__ perm, const float4 *__restrict__ pin,
-                              const float4 *__restrict__ vin, const int32_t *__restrict__ min, float4 *pout,
-                              float4 *vout, int32_t *mout, int32_t *inv, float *xyz3)
+// 5. permute particle state.  With spart, also the block's largest FLUID
+// |u|^2 (k_speed2's quantity, for the CFL time step) into spart[blockIdx.x].
+constexpr int GB = 256;  // block of the gather
+__global__ __launch_bounds__(GB) void k_sort_gather(int64_t np, const int32_t *__restrict__ perm,
+                                                    const float4 *__restrict__ pin, const float4 *__restrict__ vin,
+                                                    const int32_t *__restrict__ min, float4 *pout, float4 *vout,
+                                                    int32_t *mout, int32_t *inv, float *xyz3, double *spart)
 {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= np) return;
-    int32_t s = perm[j];
-    const float4 p = pin[s];
-    pout[j] = p;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = j < np;
+    const int32_t s = ok ? perm[j] : 0;
+    const float4 p = ok ? pin[s] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    const float4 v = ok ? vin[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     // the packed window record: a WALL particle (sign bit of m) sits at
     // +inf, so every squared distance to it is +inf
     const bool wall = __builtin_signbit(p.w);
+    if (spart) {
+        __shared__ double lds[GB / 64];
+        double m = ok && !wall ? (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z : 0.0;
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int w = 0; w < GB / 64; ++w) r = fmax(r, lds[w]);
+            spart[blockIdx.x] = r;
+        }
+    }
+    if (!ok) return;
+    pout[j] = p;
     const float inf = __builtin_inff();
     xyz3[3 * j + 0] = wall ? inf : p.x;
     xyz3[3 * j + 1] = wall ? inf : p.y;
     xyz3[3 * j + 2] = wall ? inf : p.z;
-    vout[j] = vin[s];
+    vout[j] = v;
     int32_t m = min[s];
     mout[j] = m;
     inv[meta_id(m)] = (int32_t)j;
@@ -332,7 +349,7 @@ void launch_bucket_sort(hipStream_t st, bool xyz, const float *pos, int64_t np, 
 
 }  // namespace
 
-void k_launch_sort(FlipCtx *c)
+void k_launch_sort(FlipCtx *c, double *speed_part)
 {
     int64_t np = c->np;
     t_begin(c, "sort_count");
@@ -357,8 +374,9 @@ void k_launch_sort(FlipCtx *c)
     t_end(c);
     t_begin(c, "sort_gather");
     int nx = c->cur ^ 1;
-    hipLaunchKernelGGL(k_sort_gather, nblk(np, 256), 256, 0, c->stream, np, c->perm, c->pos[c->cur],
-                       c->vel[c->cur], c->meta[c->cur], c->pos[nx], c->vel[nx], c->meta[nx], c->inv, c->xyz3);
+    hipLaunchKernelGGL(k_sort_gather, nblk(np, GB), GB, 0, c->stream, np, c->perm, c->pos[c->cur],
+                       c->vel[c->cur], c->meta[c->cur], c->pos[nx], c->vel[nx], c->meta[nx], c->inv, c->xyz3,
+                       speed_part);
     c->cur = nx;
     t_end(c);
 }
