@@ -993,7 +993,7 @@ static void step_launch_pre(FlipCtx *c)
     // gather forms
     k_launch_sort(c, c->prm.dt_policy == 1 ? c->partials : nullptr);
     c->sorted = true;
-    k_launch_choose_dt(c, true);
+    k_launch_choose_dt(c);
     // with add_ExtForce (:1230) fused in, and the inputs of flags and P2G
     // written (densities with the wall mark, P2G's packed records)
     const bool records = k_launch_density(c, true, true);

@@ -271,9 +271,9 @@ void k_launch_selftest(int64_t m, const float *x, const float *y, float *out, hi
 void rng_upload_table(FlipCtx *c);  // the chunked-draw coefficients (context creation)
 void rng_to_device(FlipCtx *c);  // the rand() state to the device, if it is on the host
 void rng_to_host(FlipCtx *c);    // and back (waits for the stream)
-// the step's time step on the device; from_sort: the speeds from the sort's
-// gather (k_launch_sort with c->partials) instead of k_speed2
-void k_launch_choose_dt(FlipCtx *c, bool from_sort = false);
+// the step's time step on the device, from the speeds of the sort's gather
+// (k_launch_sort with c->partials, just before)
+void k_launch_choose_dt(FlipCtx *c);
 void k_launch_set_dt(FlipCtx *c, double dt);    // a given dt into dtv[0]
 
 // records the message for flip_gpu_last_error and returns code

@@ -1873,27 +1873,6 @@ __global__ void k_correct_update(int64_t np, float4 *pos, float4 *vel, const flo
     if (!ok) atomicOr(nonfinite, 1);
 }
 
-// max |u|^2 over FLUID particles: per-block maxima (the FP64 max is exact and
-// order-free), then k_choose_dt forms the time step on the device
-__global__ void k_speed2(int64_t np, const float4 *__restrict__ pos, const float4 *__restrict__ vel, double *part)
-{
-    __shared__ double lds[TB / 64];
-    double m = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
-        if (is_wall_w(pos[i].w)) continue;
-        float4 v = vel[i];
-        m = fmax(m, (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
-    }
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0.0;
-        for (int w = 0; w < TB / 64; ++w) r = fmax(r, lds[w]);
-        part[blockIdx.x] = r;
-    }
-}
-
 // dt policy 1: dt = min(cap, cfl / (N max|u|)) (no FLUID motion: the cap);
 // nb == 0 writes the cap (policy 0).  fmax drops NaN speeds, as the max does.
 __global__ __launch_bounds__(TB) void k_choose_dt(int nb, const double *__restrict__ part, double cap, double cfl,
@@ -2142,17 +2121,11 @@ void k_launch_step_record(FlipCtx *c)
 }
 
 // the step's time step into c->dtv[0] on the device (no host round trip)
-void k_launch_choose_dt(FlipCtx *c, bool from_sort)
+void k_launch_choose_dt(FlipCtx *c)
 {
-    unsigned nb = 0;
-    if (c->prm.dt_policy == 1 && c->np > 0) {
-        if (from_sort) {
-            nb = nblk(c->np, 256);  // the gather's blocks
-        } else {
-            nb = std::min(nblk(c->np, TB), 1024u);
-            hipLaunchKernelGGL(k_speed2, nb, TB, 0, c->stream, c->np, c->P4(), c->V4(), c->partials);
-        }
-    }
+    // the largest FLUID |u|^2 per block of the sort's gather (k_launch_sort
+    // with c->partials), one per 256 particles
+    const unsigned nb = c->prm.dt_policy == 1 && c->np > 0 ? nblk(c->np, 256) : 0;
     hipLaunchKernelGGL(k_choose_dt, 1, TB, 0, c->stream, (int)nb, c->partials, c->prm.dt, c->prm.cfl, c->n, c->dtv);
 }
 

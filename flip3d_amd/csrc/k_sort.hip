@@ -263,7 +263,7 @@ __global__ __launch_bounds__(OB) void k_sort_order_big(const int32_t *__restrict
 }
 
 // 5. permute particle state.  With spart, also the block's largest FLUID
-// |u|^2 (k_speed2's quantity, for the CFL time step) into spart[blockIdx.x].
+// |u|^2 (double, for the CFL time step) into spart[blockIdx.x].
 constexpr int GB = 256;  // block of the gather
 __global__ __launch_bounds__(GB) void k_sort_gather(int64_t np, const int32_t *__restrict__ perm,
                                                     const float4 *__restrict__ pin, const float4 *__restrict__ vin,
