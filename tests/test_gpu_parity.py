@@ -15,7 +15,7 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-REL_TOL = 1e-5  # north-star field tolerance (relative, max-norm scaled)
+REL_TOL = 1e-5  # field tolerance at the benchmark's solver stop (relative, max-norm scaled)
 
 
 def gold(name):
